@@ -1,0 +1,82 @@
+"""Build libmpcf_hip.so (gfx950) in-tree with hipcc.
+
+    python cubism-jr-gpu-mpcf_amd/build.py        # or __graft_entry__.build()
+
+Objects:
+  parity.o  mpcf_kernels.hip, MPCF_FAST=0, -ffp-contract=off  (bit-exact scheme)
+  fast.o    mpcf_kernels.hip, MPCF_FAST=1, -ffp-contract=fast (FMA-contracted)
+  capi.o    the C-ABI (include/mpcf_hip.h)
+  facades.o the C++ mirror of the reference's kernel facades and integrator
+All HIP code is compiled with packed-FP32 instructions disabled
+(-target-feature -packed-fp32-ops).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+import concurrent.futures as cf
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+BUILD = os.path.join(PKG, "_build")
+LIB = os.path.join(PKG, "libmpcf_hip.so")
+
+ARCH = os.environ.get("MPCF_OFFLOAD_ARCH", "gfx950")
+
+
+def hipcc() -> str:
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def common_flags():
+    return [
+        f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
+        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+        "-Wno-unused-command-line-argument",
+    ]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build step failed: " + " ".join(cmd))
+    return r
+
+
+def build(verbose: bool = False) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    cc = hipcc()
+    units = [
+        ("parity.o", "mpcf_kernels.hip", ["-DMPCF_FAST=0", "-ffp-contract=off"]),
+        ("fast.o", "mpcf_kernels.hip", ["-DMPCF_FAST=1", "-ffp-contract=fast"]),
+        ("capi.o", "mpcf_capi.cpp", ["-x", "hip"]),
+        ("facades.o", "facades.cpp", ["-x", "hip"]),
+        ("ic.o", "ic.cpp", ["-x", "hip"]),
+    ]
+    cmds = []
+    for obj, src, extra in units:
+        path = os.path.join(CSRC, src)
+        if not os.path.exists(path):
+            continue
+        cmds.append([cc, *common_flags(), *extra, "-c", path, "-o", os.path.join(BUILD, obj)])
+    with cf.ThreadPoolExecutor(max_workers=min(4, len(cmds))) as ex:
+        for r, cmd in zip(ex.map(_run, cmds), cmds):
+            if verbose:
+                print(" ".join(cmd))
+    objs = [c[-1] for c in cmds]
+    tmp = LIB + ".tmp"
+    _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

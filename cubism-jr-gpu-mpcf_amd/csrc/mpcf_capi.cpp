@@ -1,0 +1,355 @@
+// mpcf_capi.cpp -- the C-ABI declared in include/mpcf_hip.h.
+//
+// A context keeps one node block resident in HBM for its whole life:
+//   u[2]  two solution buffers (stage input / stage output, swapped per stage)
+//   R     the LSRK3 register (the reference's NodeBlock::tmp)
+//   acc   10 accumulator fields carried between the x, y and z sweeps; also
+//         the staging area of uploads and downloads
+//   st    device StepState (max speed of sound, dt, dt/h)
+// This replaces the reference's chunked host pipeline (GPUlab.cpp:147-403,
+// 703-793) and its per-stage PCIe traffic: after mpcf_upload nothing crosses
+// PCIe until mpcf_download, and dt is produced and consumed on the device.
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "mpcf_hip.h"
+#include "mpcf_launch.h"
+
+using mpcf::Geom;
+using mpcf::StepState;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg)
+{
+    g_err = msg;
+    return code;
+}
+
+#define HIP_TRY(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(MPCF_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCH_CHECK(what)                                                                \
+    do {                                                                                  \
+        hipError_t e_ = hipGetLastError();                                                \
+        if (e_ != hipSuccess)                                                             \
+            return fail(MPCF_EHIP, std::string(what) + " launch: " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct Launchers
+{
+    decltype(&mpcf::parity::launch_ghosts) ghosts;
+    decltype(&mpcf::parity::launch_sos) sos;
+    decltype(&mpcf::parity::launch_dt) dt;
+    decltype(&mpcf::parity::launch_rhs) rhs;
+    decltype(&mpcf::parity::launch_update) update;
+    decltype(&mpcf::parity::launch_pack) pack;
+    decltype(&mpcf::parity::launch_test_weno) test_weno;
+    decltype(&mpcf::parity::launch_test_face) test_face;
+};
+
+const Launchers kParity = {
+    mpcf::parity::launch_ghosts, mpcf::parity::launch_sos, mpcf::parity::launch_dt, mpcf::parity::launch_rhs,
+    mpcf::parity::launch_update, mpcf::parity::launch_pack, mpcf::parity::launch_test_weno,
+    mpcf::parity::launch_test_face};
+const Launchers kFast = {
+    mpcf::fast::launch_ghosts, mpcf::fast::launch_sos, mpcf::fast::launch_dt, mpcf::fast::launch_rhs,
+    mpcf::fast::launch_update, mpcf::fast::launch_pack, mpcf::fast::launch_test_weno,
+    mpcf::fast::launch_test_face};
+
+// LSRK3 coefficients (LSRK3_IntegratorMPI.cpp:46,51,56), passed as binary32
+// like GPUlab::process_all's Real arguments (GPUlab.h:258)
+const double kA[3] = {0.0, -17. / 32, -32. / 27};
+const double kB[3] = {1. / 4, 8. / 9, 3. / 4};
+
+} // namespace
+
+struct mpcf_ctx
+{
+    Geom g;
+    double h;
+    unsigned flags;
+    int bc[6];
+    int device;
+    float* base;        // single allocation
+    float* u[2];
+    int cur;
+    float* R;
+    float* acc;
+    StepState* st;      // device
+    StepState* st_host; // pinned host mirror
+    hipStream_t stream;
+    bool own_stream;
+    size_t bytes;
+    const Launchers* L;
+};
+
+extern "C" const char* mpcf_last_error(void) { return g_err.c_str(); }
+
+extern "C" const char* mpcf_version(void) { return "mpcf-mi355x 0.1 (gfx950)"; }
+
+extern "C" int mpcf_create(mpcf_ctx** out, int nx, int ny, int nz, double h, unsigned flags)
+{
+    if (!out) return fail(MPCF_EINVAL, "mpcf_create: out is NULL");
+    *out = nullptr;
+    if (nx < 5 || ny < 5 || nz < 5) return fail(MPCF_EINVAL, "mpcf_create: extents must be >= 5");
+    if (!(h > 0) || !std::isfinite(h)) return fail(MPCF_EINVAL, "mpcf_create: h must be positive and finite");
+    if (flags & ~(unsigned)(MPCF_FLAG_FAST | MPCF_FLAG_SOS_CPU)) return fail(MPCF_EINVAL, "mpcf_create: unknown flag");
+    const Geom g = mpcf::make_geom(nx, ny, nz);
+    if ((int64_t)g.px * g.py * g.pz >= (int64_t(1) << 31))
+        return fail(MPCF_EINVAL, "mpcf_create: block too large for 32-bit element indices");
+
+    mpcf_ctx* c = new mpcf_ctx();
+    c->g = g; c->h = h; c->flags = flags; c->cur = 0;
+    for (int i = 0; i < 6; ++i) c->bc[i] = MPCF_BC_ABSORBING;
+    c->L = (flags & MPCF_FLAG_FAST) ? &kFast : &kParity;
+    hipError_t e = hipGetDevice(&c->device);
+    if (e != hipSuccess) { delete c; return fail(MPCF_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e)); }
+
+    const size_t nfields = 7 + 7 + 7 + mpcf::NACC;
+    c->bytes = nfields * (size_t)g.fstride * sizeof(float) + 256;
+    e = hipMalloc((void**)&c->base, c->bytes);
+    if (e != hipSuccess) {
+        delete c;
+        return fail(MPCF_ENOMEM, std::string("hipMalloc of block state: ") + hipGetErrorString(e));
+    }
+    e = hipMemset(c->base, 0, c->bytes);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&c->st_host, sizeof(StepState), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        hipFree(c->base);
+        delete c;
+        return fail(MPCF_EHIP, std::string("mpcf_create: ") + hipGetErrorString(e));
+    }
+    c->own_stream = true;
+    c->u[0] = c->base;
+    c->u[1] = c->base + 7 * g.fstride;
+    c->R = c->base + 14 * g.fstride;
+    c->acc = c->base + 21 * g.fstride;
+    c->st = (StepState*)(c->base + nfields * (size_t)g.fstride);
+    std::memset(c->st_host, 0, sizeof(StepState));
+    *out = c;
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_destroy(mpcf_ctx* c)
+{
+    if (!c) return MPCF_OK;
+    hipStreamSynchronize(c->stream);
+    if (c->own_stream) hipStreamDestroy(c->stream);
+    hipFree(c->base);
+    hipHostFree(c->st_host);
+    delete c;
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_set_stream(mpcf_ctx* c, void* stream)
+{
+    if (!c) return fail(MPCF_EINVAL, "mpcf_set_stream: NULL context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (c->own_stream) HIP_TRY(hipStreamDestroy(c->stream));
+    if (stream) {
+        c->stream = (hipStream_t)stream;
+        c->own_stream = false;
+    } else {
+        HIP_TRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        c->own_stream = true;
+    }
+    return MPCF_OK;
+}
+
+extern "C" void* mpcf_get_stream(mpcf_ctx* c) { return c ? (void*)c->stream : nullptr; }
+
+extern "C" int mpcf_set_bc(mpcf_ctx* c, const int bc[6])
+{
+    if (!c || !bc) return fail(MPCF_EINVAL, "mpcf_set_bc: NULL argument");
+    for (int i = 0; i < 6; ++i)
+        if (bc[i] != MPCF_BC_ABSORBING && bc[i] != MPCF_BC_REFLECTING)
+            return fail(MPCF_EINVAL, "mpcf_set_bc: unknown boundary kind");
+    for (int i = 0; i < 6; ++i) c->bc[i] = bc[i];
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_upload(mpcf_ctx* c, int which, const float* const fields[7])
+{
+    if (!c || !fields) return fail(MPCF_EINVAL, "mpcf_upload: NULL argument");
+    if (which != MPCF_DATA && which != MPCF_TMP) return fail(MPCF_EINVAL, "mpcf_upload: which must be DATA or TMP");
+    for (int q = 0; q < 7; ++q)
+        if (!fields[q]) return fail(MPCF_EINVAL, "mpcf_upload: NULL field pointer");
+    const size_t n = (size_t)c->g.nx * c->g.ny * c->g.nz;
+    for (int q = 0; q < 7; ++q)
+        HIP_TRY(hipMemcpyAsync(c->acc + q * n, fields[q], n * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    float* dst = which == MPCF_DATA ? c->u[c->cur] : c->R;
+    c->L->pack(c->g, dst, c->acc, 1, 7, c->stream);
+    LAUNCH_CHECK("pack");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_download(mpcf_ctx* c, int which, float* const fields[7])
+{
+    if (!c || !fields) return fail(MPCF_EINVAL, "mpcf_download: NULL argument");
+    if (which != MPCF_DATA && which != MPCF_TMP) return fail(MPCF_EINVAL, "mpcf_download: which must be DATA or TMP");
+    for (int q = 0; q < 7; ++q)
+        if (!fields[q]) return fail(MPCF_EINVAL, "mpcf_download: NULL field pointer");
+    const size_t n = (size_t)c->g.nx * c->g.ny * c->g.nz;
+    float* src = which == MPCF_DATA ? c->u[c->cur] : c->R;
+    c->L->pack(c->g, src, c->acc, 0, 7, c->stream);
+    LAUNCH_CHECK("unpack");
+    for (int q = 0; q < 7; ++q)
+        HIP_TRY(hipMemcpyAsync(fields[q], c->acc + q * n, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_load_ghosts(mpcf_ctx* c)
+{
+    if (!c) return fail(MPCF_EINVAL, "mpcf_load_ghosts: NULL context");
+    c->L->ghosts(c->g, c->u[c->cur], c->bc, c->stream);
+    LAUNCH_CHECK("ghosts");
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_max_sos(mpcf_ctx* c, float* sos)
+{
+    if (!c || !sos) return fail(MPCF_EINVAL, "mpcf_max_sos: NULL argument");
+    c->L->sos(c->g, c->u[c->cur], c->st, (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0, c->stream);
+    LAUNCH_CHECK("sos");
+    unsigned int bits = 0;
+    HIP_TRY(hipMemcpyAsync(&bits, &c->st->sos_bits, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    float f;
+    std::memcpy(&f, &bits, sizeof(f));
+    *sos = f;
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_convection(mpcf_ctx* c, float a, float dtinvh)
+{
+    if (!c) return fail(MPCF_EINVAL, "mpcf_convection: NULL context");
+    c->L->rhs(c->g, c->u[c->cur], c->R, c->acc, nullptr, a, 0.0f, dtinvh, nullptr, 0, c->stream);
+    LAUNCH_CHECK("convection");
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_update(mpcf_ctx* c, float b)
+{
+    if (!c) return fail(MPCF_EINVAL, "mpcf_update: NULL context");
+    c->L->update(c->g, c->u[c->cur], c->R, b, c->stream);
+    LAUNCH_CHECK("update");
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_process_all(mpcf_ctx* c, float a, float b, float dtinvh)
+{
+    if (!c) return fail(MPCF_EINVAL, "mpcf_process_all: NULL context");
+    c->L->rhs(c->g, c->u[c->cur], c->R, c->acc, c->u[1 - c->cur], a, b, dtinvh, nullptr, 1, c->stream);
+    LAUNCH_CHECK("process_all");
+    c->cur = 1 - c->cur;
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_step(mpcf_ctx* c, double cfl, double dt_max, double* dt)
+{
+    if (!c) return fail(MPCF_EINVAL, "mpcf_step: NULL context");
+    if (!(cfl > 0) || !std::isfinite(cfl)) return fail(MPCF_EINVAL, "mpcf_step: cfl must be positive and finite");
+    if (!(dt_max > 0)) return fail(MPCF_EINVAL, "mpcf_step: dt_max must be positive");
+    c->L->sos(c->g, c->u[c->cur], c->st, (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0, c->stream);
+    c->L->dt(c->st, cfl, c->h, dt_max, c->stream);
+    LAUNCH_CHECK("sos/dt");
+    for (int s = 0; s < 3; ++s) {
+        c->L->ghosts(c->g, c->u[c->cur], c->bc, c->stream);
+        c->L->rhs(c->g, c->u[c->cur], c->R, c->acc, c->u[1 - c->cur], (float)kA[s], (float)kB[s], 0.0f, c->st, 1,
+                  c->stream);
+        LAUNCH_CHECK("stage");
+        c->cur = 1 - c->cur;
+    }
+    if (dt) {
+        double d;
+        float sos;
+        const int rc = mpcf_last_dt(c, &d, &sos);
+        if (rc != MPCF_OK) return rc;
+        *dt = d;
+        if (!(d > 0) || !std::isfinite(d) || !(sos > 0) || !std::isfinite(sos))
+            return fail(MPCF_ENUMERIC, "mpcf_step: non-finite or non-positive dt / speed of sound");
+    }
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_last_dt(mpcf_ctx* c, double* dt, float* sos)
+{
+    if (!c) return fail(MPCF_EINVAL, "mpcf_last_dt: NULL context");
+    HIP_TRY(hipMemcpyAsync(c->st_host, c->st, sizeof(StepState), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    if (dt) *dt = c->st_host->dt;
+    if (sos) *sos = c->st_host->sos;
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_synchronize(mpcf_ctx* c)
+{
+    if (!c) return fail(MPCF_EINVAL, "mpcf_synchronize: NULL context");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_query(mpcf_ctx* c, long long out[8])
+{
+    if (!c || !out) return fail(MPCF_EINVAL, "mpcf_query: NULL argument");
+    out[0] = c->g.nx; out[1] = c->g.ny; out[2] = c->g.nz;
+    out[3] = c->g.px; out[4] = c->g.py; out[5] = c->g.pz;
+    out[6] = (long long)c->g.fstride;
+    out[7] = (long long)c->bytes;
+    return MPCF_OK;
+}
+
+namespace {
+
+struct DevBuf
+{
+    void* p = nullptr;
+    ~DevBuf() { if (p) hipFree(p); }
+};
+
+} // namespace
+
+extern "C" int mpcf_test_weno(int n, const float* in5, float* minus, float* plus, unsigned flags)
+{
+    if (n < 0 || (n > 0 && (!in5 || !minus || !plus))) return fail(MPCF_EINVAL, "mpcf_test_weno: bad argument");
+    if (n == 0) return MPCF_OK;
+    const Launchers* L = (flags & MPCF_FLAG_FAST) ? &kFast : &kParity;
+    DevBuf a, m, p;
+    HIP_TRY(hipMalloc(&a.p, sizeof(float) * 5 * (size_t)n));
+    HIP_TRY(hipMalloc(&m.p, sizeof(float) * (size_t)n));
+    HIP_TRY(hipMalloc(&p.p, sizeof(float) * (size_t)n));
+    HIP_TRY(hipMemcpy(a.p, in5, sizeof(float) * 5 * (size_t)n, hipMemcpyHostToDevice));
+    L->test_weno(n, (const float*)a.p, (float*)m.p, (float*)p.p, nullptr);
+    LAUNCH_CHECK("test_weno");
+    HIP_TRY(hipMemcpy(minus, m.p, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(plus, p.p, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_test_face(int n, int dir, const float* st, float* out, unsigned flags)
+{
+    if (n < 0 || dir < 0 || dir > 2 || (n > 0 && (!st || !out))) return fail(MPCF_EINVAL, "mpcf_test_face: bad argument");
+    if (n == 0) return MPCF_OK;
+    const Launchers* L = (flags & MPCF_FLAG_FAST) ? &kFast : &kParity;
+    DevBuf a, o;
+    HIP_TRY(hipMalloc(&a.p, sizeof(float) * 42 * (size_t)n));
+    HIP_TRY(hipMalloc(&o.p, sizeof(float) * 12 * (size_t)n));
+    HIP_TRY(hipMemcpy(a.p, st, sizeof(float) * 42 * (size_t)n, hipMemcpyHostToDevice));
+    L->test_face(n, dir, (const float*)a.p, (float*)o.p, nullptr);
+    LAUNCH_CHECK("test_face");
+    HIP_TRY(hipMemcpy(out, o.p, sizeof(float) * 12 * (size_t)n, hipMemcpyDeviceToHost));
+    return MPCF_OK;
+}

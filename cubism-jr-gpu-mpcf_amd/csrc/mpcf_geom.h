@@ -1,0 +1,74 @@
+// mpcf_geom.h -- HBM layout of one node block (host + device).
+//
+// Every per-cell array (the solution u, the LSRK3 register R, the sweep
+// accumulators) uses one padded structure-of-arrays layout:
+//
+//   field q, cell (x,y,z) with x in [-3, nx+3), y in [-3, ny+3), z in [-3, nz+3)
+//   element  = (x + XOFF) + px * ((y + 3) + py * (z + 3))
+//   address  = base + q * fstride + element
+//
+// The three-cell ghost band around the interior replaces the reference's
+// separate x/y ghost buffers and z-halo slices (source/Types.h:37-43,
+// source/GPU/GPUhousehold.cu:105-134): ghosts are filled on the device and
+// every sweep reads its six-point stencil with the same index arithmetic.
+// XOFF = 4 puts the first interior cell of a row on a 16-byte boundary; px is
+// a multiple of 32 elements (128 bytes, one L2 line) so rows start aligned.
+// Only the face-adjacent ghost slabs are ever written or read (the stencils
+// are one-directional), so edge and corner padding stays unused.
+#pragma once
+
+#include <stdint.h>
+
+#ifndef MPCF_HD
+#if defined(__HIPCC__)
+#define MPCF_HD __host__ __device__ __forceinline__
+#else
+#define MPCF_HD inline
+#endif
+#endif
+
+namespace mpcf {
+
+enum { NVAR = 7, NACC = 10, GHOST = 3, XOFF = 4 };
+enum Field { FR = 0, FU, FV, FW, FE, FG, FP };
+// accumulator slots of the direction sweeps: 7 flux differences, then the
+// cell sums of the reconstructed G and P and the face-velocity difference
+enum Acc { A_SUMG = 7, A_SUMP = 8, A_DIVU = 9 };
+
+struct Geom
+{
+    int nx, ny, nz;       // interior extents
+    int px, py, pz;       // padded extents (elements, rows, slices)
+    int64_t fstride;      // elements per field
+
+    MPCF_HD int idx(int x, int y, int z) const { return (x + XOFF) + px * ((y + GHOST) + py * (z + GHOST)); }
+    MPCF_HD int n(int d) const { return d == 0 ? nx : d == 1 ? ny : nz; }
+    // element stride of one step along direction d
+    MPCF_HD int stride(int d) const { return d == 0 ? 1 : d == 1 ? px : px * py; }
+};
+
+inline Geom make_geom(int nx, int ny, int nz)
+{
+    Geom g;
+    g.nx = nx; g.ny = ny; g.nz = nz;
+    g.px = ((nx + XOFF + GHOST + 31) / 32) * 32;
+    g.py = ny + 2 * GHOST;
+    g.pz = nz + 2 * GHOST;
+    const int64_t e = (int64_t)g.px * g.py * g.pz;
+    g.fstride = ((e + 63) / 64) * 64;   // 256-byte aligned field starts
+    return g;
+}
+
+// device-resident step state: the max speed of sound of the last reduction,
+// the time step derived from it and the binary32 dt/h every stage reads
+struct StepState
+{
+    double dt;            // binary64, LSRK3_IntegratorMPI.cpp:37-38
+    double h, cfl, dt_max;
+    float  dtinvh;        // (float)(dt/h), GPUlab.h:258
+    float  sos;           // max speed of sound as binary32
+    unsigned int sos_bits;// atomicMax target (non-negative floats order as uints)
+    int    pad;
+};
+
+} // namespace mpcf

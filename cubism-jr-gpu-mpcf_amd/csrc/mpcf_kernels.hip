@@ -1,0 +1,453 @@
+// mpcf_kernels.hip -- CDNA4 kernels of one LSRK3 stage and the dt reduction.
+//
+// Built twice (see build.py):
+//   MPCF_FAST=0, -ffp-contract=off  -> namespace mpcf::parity  (bit-exact scheme)
+//   MPCF_FAST=1, -ffp-contract=fast -> namespace mpcf::fast
+//
+// One RK stage (GPUlab::process_all, source/GPUlab.cpp:703-793) is
+//   ghost fill -> x sweep -> y sweep -> z sweep (+ register update)
+// Each sweep marches its direction with a rolling five-cell primitive window
+// in registers: every cell is converted to primitives once, reconstructed
+// once (both face states), and every face is solved once.  The x sweep
+// stages x-runs of 64 pencils through LDS so its loads stay contiguous; the
+// y and z sweeps put the 64 lanes of a wave along x and march a strided axis.
+// Accumulators carry the partial flux sums between sweeps in the reference's
+// summation order (((((fxp-fxm)+fyp)-fym)+fzp)-fzm) (GPUkernels.cu:1111).
+#include <hip/hip_runtime.h>
+#include "mpcf_scheme.h"
+
+#if MPCF_FAST
+#define MPCF_NS fast
+#else
+#define MPCF_NS parity
+#endif
+
+namespace mpcf {
+namespace MPCF_NS {
+
+enum { MODE_FIRST = 0, MODE_ACC = 1, MODE_CONV = 2, MODE_STAGE = 3 };
+
+struct StageArgs
+{
+    const Real* __restrict__ u;    // input solution (7 fields, ghosts filled)
+    Real* __restrict__ R;          // RK register (7 fields), updated in place
+    Real* __restrict__ acc;        // sweep accumulators (10 fields)
+    Real* __restrict__ uo;         // output solution (MODE_STAGE)
+    int64_t fs;                    // field stride (elements)
+    Real a, b, dtinvh;
+    const StepState* st;           // if non-null, dtinvh is read from st->dtinvh
+};
+
+// write the contribution of one cell.  F = flux at the cell's right face,
+// Fl = flux at its left face; cG/cP = plus-state + minus-state of G/P.
+template <int MODE>
+__device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, const int e,
+                                     const FaceFlux& F, const FaceFlux& Fl, const Real cG, const Real cP)
+{
+    Real* __restrict__ acc = A.acc;
+    const int64_t fs = A.fs;
+    if (MODE == MODE_FIRST) {
+        // _xflux + _xextraterm_hllc ("=" assignment, GPUkernels.cu:57-59)
+#pragma unroll
+        for (int q = 0; q < NVAR; ++q) acc[q*fs + e] = F.f[q] - Fl.f[q];
+        acc[A_SUMG*fs + e] = cG;
+        acc[A_SUMP*fs + e] = cP;
+        acc[A_DIVU*fs + e] = F.vel - Fl.vel;
+    } else if (MODE == MODE_ACC) {
+        // _yextraterm_hllc ("+=", GPUkernels.cu:439-441)
+#pragma unroll
+        for (int q = 0; q < NVAR; ++q) acc[q*fs + e] = (acc[q*fs + e] + F.f[q]) - Fl.f[q];
+        acc[A_SUMG*fs + e] = acc[A_SUMG*fs + e] + cG;
+        acc[A_SUMP*fs + e] = acc[A_SUMP*fs + e] + cP;
+        acc[A_DIVU*fs + e] = acc[A_DIVU*fs + e] + (F.vel - Fl.vel);
+    } else {
+        // _zextraterm_hllc + _divergence (+ _update), GPUkernels.cu:812-814,1104-1136,1164-1170
+        const Real sG = acc[A_SUMG*fs + e] + cG;
+        const Real sP = acc[A_SUMP*fs + e] + cP;
+        const Real dU = acc[A_DIVU*fs + e] + (F.vel - Fl.vel);
+        const Real factor6 = 1.0f / 6.0f;
+#pragma unroll
+        for (int q = 0; q < NVAR; ++q) {
+            const Real t = (acc[q*fs + e] + F.f[q]) - Fl.f[q];
+            Real rhs;
+            if (q == FG)      rhs = dtinvh*(t - dU * sG * factor6);
+            else if (q == FP) rhs = dtinvh*(t - dU * sP * factor6);
+            else              rhs = dtinvh*t;
+            const Real Rn = A.a*A.R[q*fs + e] - rhs;
+            A.R[q*fs + e] = Rn;
+            if (MODE == MODE_STAGE) A.uo[q*fs + e] = A.b*Rn + A.u[q*fs + e];
+        }
+    }
+}
+
+// marching state along direction D
+template <int D, int MODE>
+struct March
+{
+    Prim w0, w1, w2, w3, w4;   // primitives of cells k-2 .. k+2
+    Prim pm;                   // minus-state (face k-1/2) of cell k-1
+    Real pGp, pPp;             // plus-state G,P (face k-3/2) of cell k-1
+    FaceFlux Fl;               // flux at face k-3/2
+
+    __device__ __forceinline__ void push(const Prim& q) { w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = q; }
+
+    // reconstruct cell k, solve face k-1/2, emit cell k-1
+    __device__ __forceinline__ void step(const StageArgs& A, const Real dtinvh, const int k, const int c0, const int e_km1,
+                                         const bool store = true)
+    {
+        const CellRec r = reconstruct(w0, w1, w2, w3, w4);
+        if (k >= c0) {
+            const FaceFlux F = riemann<D>(pm, r.p);
+            if (k >= c0 + 1 && store) emit<MODE>(A, dtinvh, e_km1, F, Fl, pGp + pm.G, pPp + pm.P);
+            Fl = F;
+        }
+        pm = r.m; pGp = r.p.G; pPp = r.p.P;
+    }
+};
+
+__device__ __forceinline__ Prim load_prim(const Real* __restrict__ u, const int64_t fs, const int e)
+{
+    return to_prim(u[e], u[fs + e], u[2*fs + e], u[3*fs + e], u[4*fs + e], u[5*fs + e], u[6*fs + e]);
+}
+
+__device__ __forceinline__ Real stage_dtinvh(const StageArgs& A)
+{
+    return A.st ? A.st->dtinvh : A.dtinvh;
+}
+
+// ---------------------------------------------------------------------------
+// y / z sweeps: block (64, 4); lanes along x, threadIdx.y + blockIdx.y*4 along
+// the remaining transverse axis, blockIdx.z = chunk of the marched axis.
+// ---------------------------------------------------------------------------
+template <int D, int MODE>
+__global__ __launch_bounds__(256)
+void k_sweep_yz(const Geom g, const StageArgs A, const int chunk)
+{
+    const int x = blockIdx.x * 64 + threadIdx.x;
+    const int t = blockIdx.y * 4 + threadIdx.y;
+    const int nt = D == 1 ? g.nz : g.ny;
+    const int nd = D == 1 ? g.ny : g.nz;
+    if (x >= g.nx || t >= nt) return;   // no barriers below
+    const int c0 = blockIdx.z * chunk;
+    const int c1 = min(c0 + chunk, nd);
+    const int s = g.stride(D);
+    const int e0 = D == 1 ? g.idx(x, 0, t) : g.idx(x, t, 0);
+    const Real dtinvh = stage_dtinvh(A);
+
+    March<D, MODE> m;
+    m.w1 = load_prim(A.u, A.fs, e0 + (c0 - 3) * s);
+    m.w2 = load_prim(A.u, A.fs, e0 + (c0 - 2) * s);
+    m.w3 = load_prim(A.u, A.fs, e0 + (c0 - 1) * s);
+    m.w4 = load_prim(A.u, A.fs, e0 + (c0    ) * s);
+    for (int k = c0 - 1; k <= c1; ++k) {
+        m.push(load_prim(A.u, A.fs, e0 + (k + 2) * s));
+        m.step(A, dtinvh, k, c0, e0 + (k - 1) * s);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// x sweep: one wave per block, lane = pencil (y,z) flattened; x-runs of SX
+// cells of 64 pencils are converted to primitives and staged in LDS, then each
+// lane marches its own pencil through them.
+// ---------------------------------------------------------------------------
+enum { SX = 8 };
+
+template <int MODE>
+__global__ __launch_bounds__(64)
+void k_sweep_x(const Geom g, const StageArgs A, const int chunk)
+{
+    __shared__ Real sh[NVAR][64][SX + 1];
+    const int lane = threadIdx.x;
+    const int npen = g.ny * g.nz;
+    const int pb = blockIdx.x * 64;
+    const int c0 = blockIdx.y * chunk;
+    const int c1 = min(c0 + chunk, g.nx);
+    const Real dtinvh = stage_dtinvh(A);
+
+    // element index of x = 0 for the 8 pencils this lane stages, and for its own pencil
+    int sbase[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        int p = pb + i * 8 + lane / SX;
+        p = p < npen ? p : npen - 1;
+        sbase[i] = g.idx(0, p % g.ny, p / g.ny);
+    }
+    const int myp = pb + lane;
+    const bool active = myp < npen;
+    const int mybase = active ? g.idx(0, myp % g.ny, myp / g.ny) : 0;
+    const int xl = lane % SX;
+
+    March<0, MODE> m;
+    for (int xs = c0 - 3; xs <= c1 + 2; xs += SX) {
+        __syncthreads();
+        const int xc = xs + xl;
+        if (xc <= c1 + 2) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const Prim q = load_prim(A.u, A.fs, sbase[i] + xc);
+                const int pl = i * 8 + lane / SX;
+                sh[FR][pl][xl] = q.r; sh[FU][pl][xl] = q.u; sh[FV][pl][xl] = q.v; sh[FW][pl][xl] = q.w;
+                sh[FE][pl][xl] = q.p; sh[FG][pl][xl] = q.G; sh[FP][pl][xl] = q.P;
+            }
+        }
+        __syncthreads();
+        const int nj = min(SX, c1 + 3 - xs);
+        for (int j = 0; j < nj; ++j) {
+            Prim q;
+            q.r = sh[FR][lane][j]; q.u = sh[FU][lane][j]; q.v = sh[FV][lane][j]; q.w = sh[FW][lane][j];
+            q.p = sh[FE][lane][j]; q.G = sh[FG][lane][j]; q.P = sh[FP][lane][j];
+            m.push(q);
+            const int k = xs + j - 2;
+            // lanes past the last pencil march a duplicate of it and never store
+            if (k >= c0 - 1) m.step(A, dtinvh, k, c0, mybase + (k - 1), active);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// ghost fill (BoundaryConditions.h:60-109 applied to the padded band).
+// blockIdx.y = face = 2*dir + side.  kind: 0 absorbing, 1 reflecting.
+// ---------------------------------------------------------------------------
+struct BCSet { int kind[6]; };
+
+__global__ __launch_bounds__(256)
+void k_ghosts(const Geom g, Real* __restrict__ u, const int64_t fs, const BCSet bc)
+{
+    const int face = blockIdx.y;
+    const int dir = face >> 1, side = face & 1;
+    const int na = dir == 0 ? g.ny : g.nx;
+    const int nb = dir == 2 ? g.ny : g.nz;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= na * nb) return;
+    const int ta = t % na, tb = t / na;
+    const int n = g.n(dir);
+    const int kind = bc.kind[face];
+#pragma unroll
+    for (int gl = 0; gl < 3; ++gl) {
+        const int dst = side == 0 ? gl - 3 : n + gl;
+        const int src = kind == 0 ? (side == 0 ? 0 : n - 1) : (side == 0 ? 2 - gl : n - 1 - gl);
+        int ed, es;
+        if (dir == 0)      { ed = g.idx(dst, ta, tb); es = g.idx(src, ta, tb); }
+        else if (dir == 1) { ed = g.idx(ta, dst, tb); es = g.idx(ta, src, tb); }
+        else               { ed = g.idx(ta, tb, dst); es = g.idx(ta, tb, src); }
+#pragma unroll
+        for (int q = 0; q < NVAR; ++q) {
+            const Real v = u[q*fs + es];
+            u[q*fs + ed] = (kind == 1 && q == 1 + dir) ? -v : v;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// max speed of sound: reset, reduce, finalize dt (GPUkernels.cu:1182-1222,
+// MaxSpeedOfSound.cpp:18-55, LSRK3_IntegratorMPI.cpp:37-38)
+// ---------------------------------------------------------------------------
+__global__ void k_sos_reset(StepState* st) { st->sos_bits = 0u; }
+
+template <int CPU_FORM>
+__global__ __launch_bounds__(256)
+void k_sos(const Geom g, const Real* __restrict__ u, const int64_t fs, StepState* st)
+{
+    __shared__ Real wmax[4];
+    const int rows = g.ny * g.nz;
+    Real sos = 0.0f;
+    for (int row = blockIdx.x * 4 + threadIdx.y; row < rows; row += gridDim.x * 4) {
+        const int e0 = g.idx(0, row % g.ny, row / g.ny);
+        for (int x = threadIdx.x; x < g.nx; x += 64) {
+            const int e = e0 + x;
+            const Real r = u[e], mu = u[fs + e], mv = u[2*fs + e], mw = u[3*fs + e];
+            const Real E = u[4*fs + e], G = u[5*fs + e], P = u[6*fs + e];
+            Real p;
+            if (CPU_FORM) p = (Real)((E - (mu*mu + mv*mv + mw*mw)*(0.5/r) - P)/G);
+            else          p = (E - (mu*mu + mv*mv + mw*mw)*(0.5f/r) - P) / G;
+            const Real c = sqrtf(((p + P) / G + p) / r);
+            sos = fmaxf(sos, c + fmaxf(fmaxf(fabsf(mu), fabsf(mv)), fabsf(mw)) / r);
+        }
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) sos = fmaxf(sos, __shfl_xor(sos, off, 64));
+    if (threadIdx.x == 0) wmax[threadIdx.y] = sos;
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const Real m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+        atomicMax(&st->sos_bits, __float_as_uint(m));
+    }
+}
+
+__global__ void k_dt_finalize(StepState* st, const double cfl, const double h, const double dt_max)
+{
+    const float sos = __uint_as_float(st->sos_bits);
+    st->sos = sos;
+    double dt = cfl*h/sos;
+    dt = dt_max < dt ? dt_max : dt;
+    st->dt = dt;
+    st->h = h; st->cfl = cfl; st->dt_max = dt_max;
+    st->dtinvh = (float)(dt/h);
+}
+
+// ---------------------------------------------------------------------------
+// Update_CUDA (GPUkernels.cu:1164-1170): u <- b*R + u, in place
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256)
+void k_update(const Geom g, Real* __restrict__ u, const Real* __restrict__ R, const int64_t fs, const Real b)
+{
+    const int rows = g.ny * g.nz;
+    for (int row = blockIdx.x * 4 + threadIdx.y; row < rows; row += gridDim.x * 4) {
+        const int e0 = g.idx(0, row % g.ny, row / g.ny);
+        for (int x = threadIdx.x; x < g.nx; x += 64) {
+            const int e = e0 + x;
+#pragma unroll
+            for (int q = 0; q < NVAR; ++q) u[q*fs + e] = b*R[q*fs + e] + u[q*fs + e];
+        }
+    }
+}
+
+// dense reference SoA (NX*NY*NZ per field, Types.h:31) <-> padded layout
+template <int TO_PADDED>
+__global__ __launch_bounds__(256)
+void k_pack(const Geom g, Real* __restrict__ pad, Real* __restrict__ dense, const int64_t fs, const int nq)
+{
+    const int rows = g.ny * g.nz;
+    const int64_t n = (int64_t)g.nx * g.ny * g.nz;
+    for (int row = blockIdx.x * 4 + threadIdx.y; row < rows; row += gridDim.x * 4) {
+        const int y = row % g.ny, z = row / g.ny;
+        const int e0 = g.idx(0, y, z);
+        const int64_t d0 = (int64_t)g.nx * row;
+        for (int x = threadIdx.x; x < g.nx; x += 64)
+            for (int q = 0; q < nq; ++q) {
+                if (TO_PADDED) pad[q*fs + e0 + x] = dense[q*n + d0 + x];
+                else           dense[q*n + d0 + x] = pad[q*fs + e0 + x];
+            }
+    }
+}
+
+// device-function test hooks (the counterpart of GPU::TestKernel,
+// GPUkernels.cu:1354-1382): run weno_cell / riemann on arrays
+__global__ void k_test_weno(const int n, const Real* __restrict__ in5, Real* __restrict__ m, Real* __restrict__ p)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Real* a = in5 + 5*(int64_t)i;
+    const Rec r = weno_cell(a[0], a[1], a[2], a[3], a[4]);
+    m[i] = r.m; p[i] = r.p;
+}
+
+// st: n x 7 x 6 conserved values at stencil points i-3..i+2 of face i-1/2;
+// out: n x 12 (7 fluxes, vel, Gm, Gp, Pm, Pp)
+template <int D>
+__global__ void k_test_face(const int n, const Real* __restrict__ st, Real* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const Real* s = st + 42*(int64_t)i;
+    Prim c[6];
+    for (int k = 0; k < 6; ++k)
+        c[k] = to_prim(s[0*6+k], s[1*6+k], s[2*6+k], s[3*6+k], s[4*6+k], s[5*6+k], s[6*6+k]);
+    const CellRec left = reconstruct(c[0], c[1], c[2], c[3], c[4]);   // cell i-1
+    const CellRec right = reconstruct(c[1], c[2], c[3], c[4], c[5]);  // cell i
+    const FaceFlux F = riemann<D>(left.m, right.p);
+    Real* o = out + 12*(int64_t)i;
+    for (int q = 0; q < NVAR; ++q) o[q] = F.f[q];
+    o[7] = F.vel; o[8] = left.m.G; o[9] = right.p.G; o[10] = left.m.P; o[11] = right.p.P;
+}
+
+// ---------------------------------------------------------------------------
+// host launchers
+// ---------------------------------------------------------------------------
+static int pick_chunk(const int nd, const int64_t columns, const int waves_per_col_group)
+{
+    // enough marching threads for ~4 waves per SIMD (256 CUs x 4 SIMDs)
+    (void)waves_per_col_group;
+    const int64_t target_threads = 256LL * 4 * 4 * 64;
+    int nchunks = (int)((target_threads + columns - 1) / columns);
+    if (nchunks < 1) nchunks = 1;
+    int chunk = (nd + nchunks - 1) / nchunks;
+    if (chunk < 16) chunk = nd < 16 ? nd : 16;
+    return chunk;
+}
+
+static int rows_grid(const Geom& g)
+{
+    const int rows = g.ny * g.nz;
+    int blocks = (rows + 3) / 4;
+    return blocks < 8192 ? blocks : 8192;
+}
+
+void launch_ghosts(const Geom& g, Real* u, const int bc[6], hipStream_t s)
+{
+    BCSet b;
+    for (int i = 0; i < 6; ++i) b.kind[i] = bc[i];
+    int tmax = g.ny * g.nz;
+    if (g.nx * g.nz > tmax) tmax = g.nx * g.nz;
+    if (g.nx * g.ny > tmax) tmax = g.nx * g.ny;
+    hipLaunchKernelGGL(k_ghosts, dim3((tmax + 255) / 256, 6), dim3(256), 0, s, g, u, g.fstride, b);
+}
+
+void launch_sos(const Geom& g, const Real* u, StepState* st, const int cpu_form, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sos_reset, dim3(1), dim3(1), 0, s, st);
+    const int nb = rows_grid(g) < 2048 ? rows_grid(g) : 2048;
+    if (cpu_form) hipLaunchKernelGGL(k_sos<1>, dim3(nb), dim3(64, 4), 0, s, g, u, g.fstride, st);
+    else          hipLaunchKernelGGL(k_sos<0>, dim3(nb), dim3(64, 4), 0, s, g, u, g.fstride, st);
+}
+
+void launch_dt(StepState* st, const double cfl, const double h, const double dt_max, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, s, st, cfl, h, dt_max);
+}
+
+// mode: MODE_CONV (R only) or MODE_STAGE (R and u_out)
+void launch_rhs(const Geom& g, const Real* u, Real* R, Real* acc, Real* uo, const Real a, const Real b,
+                const Real dtinvh, const StepState* st, const int stage, hipStream_t s)
+{
+    StageArgs A;
+    A.u = u; A.R = R; A.acc = acc; A.uo = uo; A.fs = g.fstride;
+    A.a = a; A.b = b; A.dtinvh = dtinvh; A.st = st;
+    // x
+    {
+        const int npen = g.ny * g.nz;
+        const int chunk = pick_chunk(g.nx, npen, 1);
+        dim3 grid((npen + 63) / 64, (g.nx + chunk - 1) / chunk);
+        hipLaunchKernelGGL(k_sweep_x<MODE_FIRST>, grid, dim3(64), 0, s, g, A, chunk);
+    }
+    // y
+    {
+        const int chunk = pick_chunk(g.ny, (int64_t)g.nx * g.nz, 1);
+        dim3 grid((g.nx + 63) / 64, (g.nz + 3) / 4, (g.ny + chunk - 1) / chunk);
+        hipLaunchKernelGGL((k_sweep_yz<1, MODE_ACC>), grid, dim3(64, 4), 0, s, g, A, chunk);
+    }
+    // z
+    {
+        const int chunk = pick_chunk(g.nz, (int64_t)g.nx * g.ny, 1);
+        dim3 grid((g.nx + 63) / 64, (g.ny + 3) / 4, (g.nz + chunk - 1) / chunk);
+        if (stage) hipLaunchKernelGGL((k_sweep_yz<2, MODE_STAGE>), grid, dim3(64, 4), 0, s, g, A, chunk);
+        else       hipLaunchKernelGGL((k_sweep_yz<2, MODE_CONV>), grid, dim3(64, 4), 0, s, g, A, chunk);
+    }
+}
+
+void launch_update(const Geom& g, Real* u, const Real* R, const Real b, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_update, dim3(rows_grid(g)), dim3(64, 4), 0, s, g, u, R, g.fstride, b);
+}
+
+void launch_pack(const Geom& g, Real* pad, Real* dense, const int to_padded, const int nq, hipStream_t s)
+{
+    if (to_padded) hipLaunchKernelGGL(k_pack<1>, dim3(rows_grid(g)), dim3(64, 4), 0, s, g, pad, dense, g.fstride, nq);
+    else           hipLaunchKernelGGL(k_pack<0>, dim3(rows_grid(g)), dim3(64, 4), 0, s, g, pad, dense, g.fstride, nq);
+}
+
+void launch_test_weno(const int n, const Real* in5, Real* m, Real* p, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_test_weno, dim3((n + 255) / 256), dim3(256), 0, s, n, in5, m, p);
+}
+
+void launch_test_face(const int n, const int dir, const Real* st, Real* out, hipStream_t s)
+{
+    dim3 grid((n + 127) / 128), block(128);
+    if (dir == 0)      hipLaunchKernelGGL(k_test_face<0>, grid, block, 0, s, n, st, out);
+    else if (dir == 1) hipLaunchKernelGGL(k_test_face<1>, grid, block, 0, s, n, st, out);
+    else               hipLaunchKernelGGL(k_test_face<2>, grid, block, 0, s, n, st, out);
+}
+
+} // namespace MPCF_NS
+} // namespace mpcf

@@ -1,0 +1,209 @@
+"""ctypes binding of libmpcf_hip.so (include/mpcf_hip.h).
+
+Host arrays are numpy float32 arrays of shape (7, NZ, NY, NX): the reference's
+seven SoA fields (source/NodeBlock.h:26-29,47-48) in ID3 order (Types.h:31).
+
+The library must be built (cubism-jr-gpu-mpcf_amd/build.py); loading fails
+loudly if it is missing -- there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmpcf_hip.so")
+
+OK, EINVAL, EHIP, ENOMEM, ENUMERIC = 0, 1, 2, 3, 4
+BC_ABSORBING, BC_REFLECTING = 0, 1
+FLAG_FAST, FLAG_SOS_CPU = 1, 2
+DATA, TMP = 0, 1
+NVAR = 7
+
+# LSRK3 coefficients of LSRK3_IntegratorMPI.cpp:46,51,56
+LSRK3_A = (0.0, -17.0 / 32, -32.0 / 27)
+LSRK3_B = (1.0 / 4, 8.0 / 9, 3.0 / 4)
+
+EXPORTS = [
+    "mpcf_last_error", "mpcf_version", "mpcf_create", "mpcf_destroy", "mpcf_set_stream",
+    "mpcf_get_stream", "mpcf_set_bc", "mpcf_upload", "mpcf_download", "mpcf_load_ghosts",
+    "mpcf_max_sos", "mpcf_convection", "mpcf_update", "mpcf_process_all", "mpcf_step",
+    "mpcf_last_dt", "mpcf_synchronize", "mpcf_query", "mpcf_test_weno", "mpcf_test_face",
+]
+
+_LIB = None
+
+
+class MpcfError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mpcf error {code}: {msg}")
+        self.code = code
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run cubism-jr-gpu-mpcf_amd/build.py")
+        L = C.CDLL(LIB_PATH)
+        P, I, F, D = C.c_void_p, C.c_int, C.c_float, C.c_double
+        L.mpcf_last_error.restype = C.c_char_p
+        L.mpcf_version.restype = C.c_char_p
+        L.mpcf_create.argtypes = [C.POINTER(P), I, I, I, D, C.c_uint]
+        L.mpcf_destroy.argtypes = [P]
+        L.mpcf_set_stream.argtypes = [P, P]
+        L.mpcf_get_stream.argtypes = [P]
+        L.mpcf_get_stream.restype = P
+        L.mpcf_set_bc.argtypes = [P, P]
+        L.mpcf_upload.argtypes = [P, I, P]
+        L.mpcf_download.argtypes = [P, I, P]
+        L.mpcf_load_ghosts.argtypes = [P]
+        L.mpcf_max_sos.argtypes = [P, C.POINTER(F)]
+        L.mpcf_convection.argtypes = [P, F, F]
+        L.mpcf_update.argtypes = [P, F]
+        L.mpcf_process_all.argtypes = [P, F, F, F]
+        L.mpcf_step.argtypes = [P, D, D, P]
+        L.mpcf_last_dt.argtypes = [P, C.POINTER(D), C.POINTER(F)]
+        L.mpcf_synchronize.argtypes = [P]
+        L.mpcf_query.argtypes = [P, P]
+        L.mpcf_test_weno.argtypes = [I, P, P, P, C.c_uint]
+        L.mpcf_test_face.argtypes = [I, I, P, P, C.c_uint]
+        for name in EXPORTS:
+            if name not in ("mpcf_last_error", "mpcf_version", "mpcf_get_stream"):
+                getattr(L, name).restype = I
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != OK:
+        raise MpcfError(rc, lib().mpcf_last_error().decode())
+
+
+def _ptr(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _fields(arr):
+    ptrs = (C.c_void_p * NVAR)()
+    for q in range(NVAR):
+        ptrs[q] = arr[q].ctypes.data
+    return ptrs
+
+
+class Block:
+    """One resident node block (an mpcf_ctx)."""
+
+    def __init__(self, nx, ny, nz, h, fast=False, sos_cpu=False, bc=None):
+        self.shape = (nz, ny, nx)
+        flags = (FLAG_FAST if fast else 0) | (FLAG_SOS_CPU if sos_cpu else 0)
+        h_ = C.c_void_p()
+        _check(lib().mpcf_create(C.byref(h_), nx, ny, nz, float(h), flags))
+        self._h = h_
+        if bc is not None:
+            self.set_bc(bc)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mpcf_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def set_bc(self, bc):
+        b = np.ascontiguousarray(bc, dtype=np.int32)
+        _check(lib().mpcf_set_bc(self._h, _ptr(b)))
+
+    def upload(self, which, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float32)
+        assert a.shape == (NVAR,) + self.shape, a.shape
+        _check(lib().mpcf_upload(self._h, which, _fields(a)))
+
+    def download(self, which):
+        out = np.empty((NVAR,) + self.shape, dtype=np.float32)
+        _check(lib().mpcf_download(self._h, which, _fields(out)))
+        return out
+
+    @property
+    def data(self):
+        return self.download(DATA)
+
+    @data.setter
+    def data(self, a):
+        self.upload(DATA, a)
+
+    @property
+    def tmp(self):
+        return self.download(TMP)
+
+    @tmp.setter
+    def tmp(self, a):
+        self.upload(TMP, a)
+
+    def load_ghosts(self):
+        _check(lib().mpcf_load_ghosts(self._h))
+
+    def max_sos(self):
+        s = C.c_float()
+        _check(lib().mpcf_max_sos(self._h, C.byref(s)))
+        return s.value
+
+    def convection(self, a, dtinvh):
+        _check(lib().mpcf_convection(self._h, float(a), float(dtinvh)))
+
+    def update(self, b):
+        _check(lib().mpcf_update(self._h, float(b)))
+
+    def process_all(self, a, b, dtinvh):
+        _check(lib().mpcf_process_all(self._h, float(a), float(b), float(dtinvh)))
+
+    def step(self, cfl, dt_max=1e30, sync=True):
+        if sync:
+            dt = C.c_double()
+            _check(lib().mpcf_step(self._h, float(cfl), float(dt_max), C.cast(C.byref(dt), C.c_void_p)))
+            return dt.value
+        _check(lib().mpcf_step(self._h, float(cfl), float(dt_max), None))
+        return None
+
+    def last_dt(self):
+        d, s = C.c_double(), C.c_float()
+        _check(lib().mpcf_last_dt(self._h, C.byref(d), C.byref(s)))
+        return d.value, s.value
+
+    def synchronize(self):
+        _check(lib().mpcf_synchronize(self._h))
+
+    @property
+    def stream(self):
+        return lib().mpcf_get_stream(self._h)
+
+    def query(self):
+        out = (C.c_longlong * 8)()
+        _check(lib().mpcf_query(self._h, C.cast(out, C.c_void_p)))
+        return list(out)
+
+
+def test_weno(in5, fast=False):
+    a = np.ascontiguousarray(in5, dtype=np.float32)
+    n = a.shape[0]
+    m = np.empty(n, np.float32)
+    p = np.empty(n, np.float32)
+    _check(lib().mpcf_test_weno(n, _ptr(a), _ptr(m), _ptr(p), FLAG_FAST if fast else 0))
+    return m, p
+
+
+def test_face(st, d, fast=False):
+    s = np.ascontiguousarray(st, dtype=np.float32)
+    n = s.shape[0]
+    out = np.empty((n, 12), np.float32)
+    _check(lib().mpcf_test_face(n, d, _ptr(s), _ptr(out), FLAG_FAST if fast else 0))
+    return out

@@ -1,0 +1,138 @@
+/*
+ * mpcf_hip.h -- C-ABI of the MI355X RHS / LSRK3 path for the 7-field
+ * stiffened-gas two-phase Euler system (rho, rho u, rho v, rho w, E,
+ * G = 1/(gamma-1), P = gamma pc/(gamma-1)).
+ *
+ * One context owns one node block resident in HBM: the solution u, the
+ * low-storage RK register R (the reference's "tmp"), sweep accumulators and a
+ * device step state (max speed of sound, dt, dt/h).  Every call returns
+ * MPCF_OK or an error code; mpcf_last_error() gives the message of the last
+ * failure on the calling thread.  Calls on one context must come from one
+ * host thread at a time; contexts are independent.  Work is queued on the
+ * context's HIP stream; calls that hand results to the host synchronize it.
+ *
+ * Host arrays use the reference's storage: seven separate binary32 arrays
+ * (NodeBlock::pdata()/ptmp(), source/NodeBlock.h:47-48,110-113), each
+ * NX*NY*NZ values indexed ID3(ix,iy,iz,NX,NY) = ix + NX*(iy + NY*iz)
+ * (source/Types.h:31), in the order R, U, V, W, E, G, P (NodeBlock.h:26-29).
+ */
+#ifndef MPCF_HIP_H
+#define MPCF_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mpcf_ctx mpcf_ctx;
+
+/* status codes */
+enum {
+    MPCF_OK = 0,
+    MPCF_EINVAL = 1,   /* bad argument (null pointer, size, enum value)        */
+    MPCF_EHIP = 2,     /* a HIP runtime call failed                             */
+    MPCF_ENOMEM = 3,   /* device allocation failed                              */
+    MPCF_ENUMERIC = 4  /* non-finite or non-positive dt / speed of sound        */
+};
+
+/* boundary kinds per face (face = 2*dir + side; dir 0,1,2 = x,y,z; side 0 = low) */
+enum {
+    MPCF_BC_ABSORBING = 0,   /* BoundaryConditions::applyBC_absorbing  (BoundaryConditions.h:60-82)  */
+    MPCF_BC_REFLECTING = 1   /* BoundaryConditions::applyBC_reflecting (BoundaryConditions.h:85-109) */
+};
+
+/* context flags */
+enum {
+    MPCF_FLAG_FAST = 1u,     /* FMA-contracted kernels (tolerance parity) instead of the bit-exact build */
+    MPCF_FLAG_SOS_CPU = 2u   /* speed of sound with p recovered in binary64 as MaxSpeedOfSound_CPP
+                                (source/MaxSpeedOfSound.cpp:43) instead of the GPU form (GPUkernels.cu:1206) */
+};
+
+/* which host array set */
+enum { MPCF_DATA = 0, MPCF_TMP = 1 };
+
+/* Message of the last failed call on this thread ("" if none). */
+const char* mpcf_last_error(void);
+
+/* Library version string. */
+const char* mpcf_version(void);
+
+/* Create a context for an NX x NY x NZ block with spacing h on the current
+ * HIP device, with all boundaries absorbing.  Replaces the GPUlab constructor
+ * and GPU::alloc (source/GPUlab.cpp:25-71, source/GPU/GPUhousehold.cu:81-184);
+ * there is no nslices: the whole block stays in HBM.  NX, NY, NZ >= 5
+ * (GPUkernels.cu:18-24 asks for >= 5 in x and y). */
+int mpcf_create(mpcf_ctx** out, int nx, int ny, int nz, double h, unsigned flags);
+
+/* Release a context (GPUlab::~GPUlab -> GPU::dealloc, GPUlab.h:251). */
+int mpcf_destroy(mpcf_ctx* ctx);
+
+/* Use the given hipStream_t (NULL = create and own one).  Default: own stream. */
+int mpcf_set_stream(mpcf_ctx* ctx, void* hip_stream);
+void* mpcf_get_stream(mpcf_ctx* ctx);
+
+/* Boundary kind per face, bc[6] in face order x-lo, x-hi, y-lo, y-hi, z-lo, z-hi
+ * (the per-face choice the reference makes in each GPUlab<Case>::_apply_bc,
+ * e.g. source/Sim/Sim_SodMPI.h:30-39). */
+int mpcf_set_bc(mpcf_ctx* ctx, const int bc[6]);
+
+/* Copy seven host arrays into the block (which = MPCF_DATA: u, MPCF_TMP: R),
+ * or back out.  Synchronous.  (The reference streams these over PCIe every
+ * stage, GPU::h2d_3DArray / h2d_tmp / d2h_*, GPU.h:52-57.) */
+int mpcf_upload(mpcf_ctx* ctx, int which, const float* const fields[7]);
+int mpcf_download(mpcf_ctx* ctx, int which, float* const fields[7]);
+
+/* Fill the three-cell ghost band of u from the boundary conditions
+ * (GPUlab::load_ghosts -> _apply_bc, source/GPUlab.cpp:622-644). */
+int mpcf_load_ghosts(mpcf_ctx* ctx);
+
+/* Max over the block of c + max(|rho u|,|rho v|,|rho w|)/rho
+ * (GPUlab::max_sos / MaxSpeedOfSound_CUDA::compute, GPUlab.cpp:647-700,
+ * MaxSpeedOfSound_CUDA.cpp:13-18).  Synchronous. */
+int mpcf_max_sos(mpcf_ctx* ctx, float* sos);
+
+/* R <- a R - dtinvh (div F + quasi-conservative G/P terms), from the current u
+ * with its ghosts (Convection_CUDA(a, dtinvh).compute(), Convection_CUDA.cpp:15-23).
+ * Asynchronous. */
+int mpcf_convection(mpcf_ctx* ctx, float a, float dtinvh);
+
+/* u <- b R + u (Update_CUDA(b).compute(), Update_CUDA.cpp:12-17).  Asynchronous. */
+int mpcf_update(mpcf_ctx* ctx, float b);
+
+/* One full stage: ghosts must be loaded; R <- a R - dtinvh L(u), then
+ * u <- u + b R, all reads of u at the old time (GPUlab::process_all,
+ * GPUlab.cpp:703-793).  Same results as mpcf_convection + mpcf_update; the
+ * updated u goes to a second resident buffer.  Asynchronous. */
+int mpcf_process_all(mpcf_ctx* ctx, float a, float b, float dtinvh);
+
+/* One LSRK3 step (LSRK3_IntegratorMPI::operator(), LSRK3_IntegratorMPI.cpp:27-62):
+ * max speed of sound, dt = min(cfl h / sos, dt_max) in binary64, then three
+ * stages (a,b) = (0,1/4), (-17/32,8/9), (-32/27,3/4) with ghosts refilled
+ * before each.  dt never leaves the device between the reduction and the
+ * stages.  Writes dt (synchronizes) unless dt is NULL (asynchronous). */
+int mpcf_step(mpcf_ctx* ctx, double cfl, double dt_max, double* dt);
+
+/* dt and max speed of sound of the last mpcf_step (synchronizes). */
+int mpcf_last_dt(mpcf_ctx* ctx, double* dt, float* sos);
+
+/* Block until the context's queued work is done. */
+int mpcf_synchronize(mpcf_ctx* ctx);
+
+/* Sizes: out[0..2] = NX, NY, NZ; out[3..5] = padded extents; out[6] = field
+ * stride in elements (low 31 bits); out[7] = device bytes held by the context. */
+int mpcf_query(mpcf_ctx* ctx, long long out[8]);
+
+/* Device-function test hooks (the counterpart of GPU::TestKernel,
+ * source/GPU/GPUkernels.cu:1354-1382).  Host arrays in and out; synchronous.
+ *   mpcf_test_weno: in5 = n x 5 cell values (j-2..j+2); minus[i], plus[i] =
+ *     _weno_minus_clipped / _weno_pluss_clipped of that window
+ *     (GPUonly.cuh:177-194).
+ *   mpcf_test_face: st = n x 7 x 6 conserved values at the stencil points
+ *     i-3..i+2 of face i-1/2 in direction dir; out = n x 12: the seven fluxes,
+ *     the HLLC face velocity, Gm, Gp, Pm, Pp (GPUkernels.cu:203-287). */
+int mpcf_test_weno(int n, const float* in5, float* minus, float* plus, unsigned flags);
+int mpcf_test_face(int n, int dir, const float* st, float* out, unsigned flags);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPCF_HIP_H */

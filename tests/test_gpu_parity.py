@@ -1,0 +1,181 @@
+"""GPU parity: the HIP path through the C-ABI against the CPU oracle.
+
+Bar (bit-exact build, the default context): every value equal to the oracle
+(np.array_equal, so +0 == -0), dt bit-identical.  The FMA-contracted build
+(MPCF_FLAG_FAST) is checked at a stated tolerance in test_gpu_fast.py.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import states
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-parity]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def assert_equal(got, ref, what):
+    got = np.asarray(got)
+    ref = np.asarray(ref)
+    assert got.shape == ref.shape, (what, got.shape, ref.shape)
+    if not np.array_equal(got, ref):
+        bad = got != ref
+        d = np.abs(got.astype(np.float64) - ref.astype(np.float64))
+        idx = np.argwhere(bad)[:5]
+        raise AssertionError(f"{what}: {bad.sum()} of {bad.size} differ, max |diff| {d.max():.3e}, "
+                             f"first at {idx.tolist()}: got {got[bad][:5]}, ref {ref[bad][:5]}")
+
+
+def weno_windows(n, seed):
+    rng = np.random.default_rng(seed)
+    t = np.arange(5)[None, :]
+    parts = [
+        rng.uniform(-1, 1, (n, 5)),                                             # noise
+        np.sin(rng.uniform(0, 1, (n, 1)) * t + rng.uniform(0, 6, (n, 1))),      # smooth
+        1000.0 * (t >= rng.integers(1, 5, (n, 1))) + 1.0,                       # jumps
+        4828.79 + 1e-3 * rng.uniform(-1, 1, (n, 5)),                            # near-constant, large
+        np.full((n, 5), 2.5),                                                   # constant
+        rng.uniform(0, 1, (n, 1)) * t + rng.uniform(-5, 5, (n, 1)),             # linear
+    ]
+    return np.ascontiguousarray(np.concatenate(parts).astype(np.float32))
+
+
+def test_weno_bitexact(mpcf, oracle):
+    w = weno_windows(40000, 1)
+    say("weno windows", w.shape)
+    m, p = mpcf.test_weno(w)
+    rm, rp = oracle.weno(w)
+    assert_equal(m, rm, "weno minus-state")
+    assert_equal(p, rp, "weno plus-state")
+
+
+@pytest.mark.parametrize("d", [0, 1, 2])
+def test_face_bitexact(mpcf, oracle, d):
+    st = states.random_stencils(60000, seed=10 + d)
+    say("face batch dir", d, st.shape)
+    got = mpcf.test_face(st, d)
+    ref = oracle.face_batch(st, d)
+    assert_equal(got, ref, f"face outputs dir {d}")
+
+
+@pytest.mark.parametrize("cpu_form", [False, True])
+def test_max_sos_bitexact(mpcf, oracle, cpu_form):
+    nx, ny, nz = 24, 20, 28
+    s = states.random_smooth(nx, ny, nz, seed=3)
+    say("max_sos", (nx, ny, nz), "cpu_form", cpu_form)
+    with mpcf.Block(nx, ny, nz, 1.0 / 28, sos_cpu=cpu_form) as b:
+        b.data = s
+        got = b.max_sos()
+    ob = oracle.Block(nx, ny, nz)
+    ob.data = s
+    ref = ob.max_sos(cpu_form)
+    assert np.float32(got) == np.float32(ref), (got, ref)
+
+
+BCS = [[0] * 6, [1] * 6, [0, 1, 1, 0, 1, 0]]
+SHAPES = [(20, 24, 28), (33, 17, 9), (7, 66, 12)]
+
+
+@pytest.mark.parametrize("shape,bc", list(zip(SHAPES, BCS)))
+def test_ghosts_and_convection_bitexact(mpcf, oracle, shape, bc):
+    nx, ny, nz = shape
+    s = states.random_smooth(nx, ny, nz, seed=nx + ny)
+    R0 = states.random_smooth(nx, ny, nz, seed=99, two_phase=False) * np.float32(1e-3)
+    a, dtinvh = np.float32(-17.0 / 32), np.float32(0.0123)
+    say("convection", shape, "bc", bc)
+    with mpcf.Block(nx, ny, nz, 1.0 / max(shape), bc=bc) as b:
+        b.data = s
+        b.tmp = R0
+        b.load_ghosts()
+        b.convection(a, dtinvh)
+        got_R = b.tmp
+        got_u = b.data
+    ob = oracle.Block(nx, ny, nz)
+    ob.data = s
+    ob.tmp = R0
+    ob.apply_bc(bc)
+    ob.rhs(a, dtinvh)
+    assert_equal(got_u, s, "u untouched by convection")
+    for q in range(7):
+        assert_equal(got_R[q], ob.tmp[q], f"R' field {q}")
+
+
+def test_process_all_matches_convection_update(mpcf, oracle):
+    nx, ny, nz = 24, 16, 20
+    s = states.random_smooth(nx, ny, nz, seed=5)
+    R0 = states.random_smooth(nx, ny, nz, seed=6, two_phase=False) * np.float32(1e-3)
+    a, b_, dtinvh = np.float32(-32.0 / 27), np.float32(0.75), np.float32(0.02)
+    say("process_all vs convection+update", (nx, ny, nz))
+    with mpcf.Block(nx, ny, nz, 1.0 / 24, bc=[0, 0, 1, 1, 0, 1]) as b1, \
+         mpcf.Block(nx, ny, nz, 1.0 / 24, bc=[0, 0, 1, 1, 0, 1]) as b2:
+        for b in (b1, b2):
+            b.data = s
+            b.tmp = R0
+            b.load_ghosts()
+        b1.process_all(a, b_, dtinvh)
+        b2.convection(a, dtinvh)
+        b2.update(b_)
+        u1, R1, u2, R2 = b1.data, b1.tmp, b2.data, b2.tmp
+    assert_equal(u1, u2, "u (process_all vs convection+update)")
+    assert_equal(R1, R2, "R (process_all vs convection+update)")
+    ob = oracle.Block(nx, ny, nz)
+    ob.data = s
+    ob.tmp = R0
+    ob.apply_bc([0, 0, 1, 1, 0, 1])
+    ob.rhs(a, dtinvh)
+    ob.update(b_)
+    assert_equal(u1, ob.data, "u vs oracle")
+    assert_equal(R1, ob.tmp, "R vs oracle")
+
+
+@pytest.mark.parametrize("cpu_sos", [False, True])
+def test_lsrk3_steps_bitexact(mpcf, oracle, cpu_sos):
+    nx, ny, nz = 28, 24, 20
+    h = 1.0 / 28
+    bc = [0, 1, 0, 0, 1, 0]
+    s = states.random_smooth(nx, ny, nz, seed=11)
+    say("lsrk3 3 steps", (nx, ny, nz), "cpu_sos", cpu_sos)
+    ob = oracle.Block(nx, ny, nz)
+    ob.data = s
+    with mpcf.Block(nx, ny, nz, h, sos_cpu=cpu_sos, bc=bc) as b:
+        b.data = s
+        for it in range(3):
+            dt = b.step(0.3)
+            dref, _ = ob.step(h, 0.3, 1e30, bc, cpu_sos)
+            assert dt == dref, (it, dt, dref)
+        u, R = b.data, b.tmp
+    assert_equal(u, ob.data, "u after 3 steps")
+    assert_equal(R, ob.tmp, "R after 3 steps")
+
+
+def test_dt_max_clamp(mpcf, oracle):
+    nx = ny = nz = 16
+    s = states.uniform(nx, ny, nz, rho=1.0, u=0.1, p=1.0)
+    with mpcf.Block(nx, ny, nz, 1.0 / 16) as b:
+        b.data = s
+        dt = b.step(0.3, dt_max=1e-4)
+    assert dt == 1e-4
+
+
+def test_config1_shock_tube_10_steps_bitexact(mpcf, oracle):
+    """64^3 air|water tube, CFL 0.3, transmissive, 10 LSRK3 steps."""
+    n = 64
+    s = states.shock_tube(n, n, n, axis=0, seed=2024)
+    say("config-1 shock tube 64^3 x 10 steps")
+    ob = oracle.Block(n, n, n)
+    ob.data = s
+    dts = []
+    with mpcf.Block(n, n, n, 1.0 / n, bc=[0] * 6) as b:
+        b.data = s
+        for it in range(10):
+            dts.append(b.step(0.3))
+        u = b.data
+    for it in range(10):
+        dref, _ = ob.step(1.0 / n, 0.3, 1e30, [0] * 6)
+        assert dts[it] == dref, (it, dts[it], dref)
+    assert_equal(u, ob.data, "config-1 state after 10 steps")
