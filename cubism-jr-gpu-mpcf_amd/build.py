@@ -60,6 +60,7 @@ def build(verbose: bool = False) -> str:
         ("capi.o", "mpcf_capi.cpp", ["-x", "hip"]),
         ("facades.o", "facades.cpp", ["-x", "hip"]),
         ("ic.o", "ic.cpp", ["-x", "hip"]),
+        ("flops.o", "flops.cpp", ["-x", "hip", "-ffp-contract=off"]),
     ]
     cmds = []
     for obj, src, extra in units:

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "mpcf_hip.h"
 #include "mpcf_launch.h"
@@ -50,7 +51,7 @@ struct Launchers
     decltype(&mpcf::parity::launch_ghosts) ghosts;
     decltype(&mpcf::parity::launch_sos) sos;
     decltype(&mpcf::parity::launch_dt) dt;
-    decltype(&mpcf::parity::launch_rhs) rhs;
+    decltype(&mpcf::parity::launch_sweep) sweep;
     decltype(&mpcf::parity::launch_update) update;
     decltype(&mpcf::parity::launch_pack) pack;
     decltype(&mpcf::parity::launch_test_weno) test_weno;
@@ -58,11 +59,11 @@ struct Launchers
 };
 
 const Launchers kParity = {
-    mpcf::parity::launch_ghosts, mpcf::parity::launch_sos, mpcf::parity::launch_dt, mpcf::parity::launch_rhs,
+    mpcf::parity::launch_ghosts, mpcf::parity::launch_sos, mpcf::parity::launch_dt, mpcf::parity::launch_sweep,
     mpcf::parity::launch_update, mpcf::parity::launch_pack, mpcf::parity::launch_test_weno,
     mpcf::parity::launch_test_face};
 const Launchers kFast = {
-    mpcf::fast::launch_ghosts, mpcf::fast::launch_sos, mpcf::fast::launch_dt, mpcf::fast::launch_rhs,
+    mpcf::fast::launch_ghosts, mpcf::fast::launch_sos, mpcf::fast::launch_dt, mpcf::fast::launch_sweep,
     mpcf::fast::launch_update, mpcf::fast::launch_pack, mpcf::fast::launch_test_weno,
     mpcf::fast::launch_test_face};
 
@@ -91,7 +92,69 @@ struct mpcf_ctx
     bool own_stream;
     size_t bytes;
     const Launchers* L;
+    // optional per-kernel event timing (mpcf_timing_enable)
+    bool timing;
+    std::vector<hipEvent_t> ev_pool;
+    struct Mark { int kind; hipEvent_t b, e; };
+    std::vector<Mark> marks;
 };
+
+namespace {
+
+// kernel classes reported by mpcf_timing_read
+enum { K_GHOSTS = 0, K_SOS = 1, K_DT = 2, K_SWEEP_X = 3, K_SWEEP_Y = 4, K_SWEEP_Z = 5, K_UPDATE = 6, K_OTHER = 7 };
+
+hipEvent_t take_event(mpcf_ctx* c)
+{
+    if (!c->ev_pool.empty()) {
+        hipEvent_t e = c->ev_pool.back();
+        c->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+// brackets one launch with events on the context stream when timing is on
+struct Timed
+{
+    mpcf_ctx* c;
+    int kind;
+    hipEvent_t b = nullptr;
+    Timed(mpcf_ctx* c_, int k) : c(c_), kind(k)
+    {
+        if (c->timing) {
+            b = take_event(c);
+            if (b) hipEventRecord(b, c->stream);
+        }
+    }
+    ~Timed()
+    {
+        if (c->timing && b) {
+            hipEvent_t e = take_event(c);
+            if (e) {
+                hipEventRecord(e, c->stream);
+                c->marks.push_back({kind, b, e});
+            } else {
+                c->ev_pool.push_back(b);
+            }
+        }
+    }
+};
+
+// R <- a R - dtinvh L(u) (stage == 0), or that and u_out <- u + b R (stage == 1)
+void run_rhs(mpcf_ctx* c, const float* u, float* uo, float a, float b, float dtinvh, const StepState* st, int stage)
+{
+    mpcf::SweepParams p;
+    p.u = u; p.R = c->R; p.acc = c->acc; p.uo = uo;
+    p.a = a; p.b = b; p.dtinvh = dtinvh; p.st = st;
+    { Timed t(c, K_SWEEP_X); c->L->sweep(c->g, 0, p, stage, c->stream); }
+    { Timed t(c, K_SWEEP_Y); c->L->sweep(c->g, 1, p, stage, c->stream); }
+    { Timed t(c, K_SWEEP_Z); c->L->sweep(c->g, 2, p, stage, c->stream); }
+}
+
+} // namespace
 
 extern "C" const char* mpcf_last_error(void) { return g_err.c_str(); }
 
@@ -146,6 +209,8 @@ extern "C" int mpcf_destroy(mpcf_ctx* c)
     if (!c) return MPCF_OK;
     hipStreamSynchronize(c->stream);
     if (c->own_stream) hipStreamDestroy(c->stream);
+    for (auto& m : c->marks) { hipEventDestroy(m.b); hipEventDestroy(m.e); }
+    for (auto e : c->ev_pool) hipEventDestroy(e);
     hipFree(c->base);
     hipHostFree(c->st_host);
     delete c;
@@ -214,7 +279,7 @@ extern "C" int mpcf_download(mpcf_ctx* c, int which, float* const fields[7])
 extern "C" int mpcf_load_ghosts(mpcf_ctx* c)
 {
     if (!c) return fail(MPCF_EINVAL, "mpcf_load_ghosts: NULL context");
-    c->L->ghosts(c->g, c->u[c->cur], c->bc, c->stream);
+    { Timed t(c, K_GHOSTS); c->L->ghosts(c->g, c->u[c->cur], c->bc, c->stream); }
     LAUNCH_CHECK("ghosts");
     return MPCF_OK;
 }
@@ -222,7 +287,7 @@ extern "C" int mpcf_load_ghosts(mpcf_ctx* c)
 extern "C" int mpcf_max_sos(mpcf_ctx* c, float* sos)
 {
     if (!c || !sos) return fail(MPCF_EINVAL, "mpcf_max_sos: NULL argument");
-    c->L->sos(c->g, c->u[c->cur], c->st, (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0, c->stream);
+    { Timed t(c, K_SOS); c->L->sos(c->g, c->u[c->cur], c->st, (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0, c->stream); }
     LAUNCH_CHECK("sos");
     unsigned int bits = 0;
     HIP_TRY(hipMemcpyAsync(&bits, &c->st->sos_bits, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
@@ -236,7 +301,7 @@ extern "C" int mpcf_max_sos(mpcf_ctx* c, float* sos)
 extern "C" int mpcf_convection(mpcf_ctx* c, float a, float dtinvh)
 {
     if (!c) return fail(MPCF_EINVAL, "mpcf_convection: NULL context");
-    c->L->rhs(c->g, c->u[c->cur], c->R, c->acc, nullptr, a, 0.0f, dtinvh, nullptr, 0, c->stream);
+    run_rhs(c, c->u[c->cur], nullptr, a, 0.0f, dtinvh, nullptr, 0);
     LAUNCH_CHECK("convection");
     return MPCF_OK;
 }
@@ -244,7 +309,7 @@ extern "C" int mpcf_convection(mpcf_ctx* c, float a, float dtinvh)
 extern "C" int mpcf_update(mpcf_ctx* c, float b)
 {
     if (!c) return fail(MPCF_EINVAL, "mpcf_update: NULL context");
-    c->L->update(c->g, c->u[c->cur], c->R, b, c->stream);
+    { Timed t(c, K_UPDATE); c->L->update(c->g, c->u[c->cur], c->R, b, c->stream); }
     LAUNCH_CHECK("update");
     return MPCF_OK;
 }
@@ -252,7 +317,7 @@ extern "C" int mpcf_update(mpcf_ctx* c, float b)
 extern "C" int mpcf_process_all(mpcf_ctx* c, float a, float b, float dtinvh)
 {
     if (!c) return fail(MPCF_EINVAL, "mpcf_process_all: NULL context");
-    c->L->rhs(c->g, c->u[c->cur], c->R, c->acc, c->u[1 - c->cur], a, b, dtinvh, nullptr, 1, c->stream);
+    run_rhs(c, c->u[c->cur], c->u[1 - c->cur], a, b, dtinvh, nullptr, 1);
     LAUNCH_CHECK("process_all");
     c->cur = 1 - c->cur;
     return MPCF_OK;
@@ -263,13 +328,12 @@ extern "C" int mpcf_step(mpcf_ctx* c, double cfl, double dt_max, double* dt)
     if (!c) return fail(MPCF_EINVAL, "mpcf_step: NULL context");
     if (!(cfl > 0) || !std::isfinite(cfl)) return fail(MPCF_EINVAL, "mpcf_step: cfl must be positive and finite");
     if (!(dt_max > 0)) return fail(MPCF_EINVAL, "mpcf_step: dt_max must be positive");
-    c->L->sos(c->g, c->u[c->cur], c->st, (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0, c->stream);
-    c->L->dt(c->st, cfl, c->h, dt_max, c->stream);
+    { Timed t(c, K_SOS); c->L->sos(c->g, c->u[c->cur], c->st, (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0, c->stream); }
+    { Timed t(c, K_DT); c->L->dt(c->st, cfl, c->h, dt_max, c->stream); }
     LAUNCH_CHECK("sos/dt");
     for (int s = 0; s < 3; ++s) {
-        c->L->ghosts(c->g, c->u[c->cur], c->bc, c->stream);
-        c->L->rhs(c->g, c->u[c->cur], c->R, c->acc, c->u[1 - c->cur], (float)kA[s], (float)kB[s], 0.0f, c->st, 1,
-                  c->stream);
+        { Timed t(c, K_GHOSTS); c->L->ghosts(c->g, c->u[c->cur], c->bc, c->stream); }
+        run_rhs(c, c->u[c->cur], c->u[1 - c->cur], (float)kA[s], (float)kB[s], 0.0f, c->st, 1);
         LAUNCH_CHECK("stage");
         c->cur = 1 - c->cur;
     }
@@ -309,6 +373,31 @@ extern "C" int mpcf_query(mpcf_ctx* c, long long out[8])
     out[3] = c->g.px; out[4] = c->g.py; out[5] = c->g.pz;
     out[6] = (long long)c->g.fstride;
     out[7] = (long long)c->bytes;
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_timing_enable(mpcf_ctx* c, int enable)
+{
+    if (!c) return fail(MPCF_EINVAL, "mpcf_timing_enable: NULL context");
+    c->timing = enable != 0;
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_timing_read(mpcf_ctx* c, double ms[8], long long count[8])
+{
+    if (!c || !ms || !count) return fail(MPCF_EINVAL, "mpcf_timing_read: NULL argument");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    for (int k = 0; k < 8; ++k) { ms[k] = 0; count[k] = 0; }
+    for (auto& m : c->marks) {
+        float t = 0.0f;
+        HIP_TRY(hipEventElapsedTime(&t, m.b, m.e));
+        const int k = (m.kind >= 0 && m.kind < 8) ? m.kind : K_OTHER;
+        ms[k] += t;
+        count[k] += 1;
+        c->ev_pool.push_back(m.b);
+        c->ev_pool.push_back(m.e);
+    }
+    c->marks.clear();
     return MPCF_OK;
 }
 

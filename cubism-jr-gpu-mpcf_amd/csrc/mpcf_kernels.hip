@@ -15,6 +15,7 @@
 // summation order (((((fxp-fxm)+fyp)-fym)+fzp)-fzm) (GPUkernels.cu:1111).
 #include <hip/hip_runtime.h>
 #include "mpcf_scheme.h"
+#include "mpcf_launch.h"
 
 #if MPCF_FAST
 #define MPCF_NS fast
@@ -24,6 +25,11 @@
 
 namespace mpcf {
 namespace MPCF_NS {
+
+typedef FaceFluxT<Real> FaceFlux;
+typedef CellRecT<Real> CellRec;
+typedef RecT<Real> Rec;
+constexpr bool kFast = MPCF_FAST != 0;
 
 enum { MODE_FIRST = 0, MODE_ACC = 1, MODE_CONV = 2, MODE_STAGE = 3 };
 
@@ -38,10 +44,36 @@ struct StageArgs
     const StepState* st;           // if non-null, dtinvh is read from st->dtinvh
 };
 
+// operands of one cell's accumulator update, loaded one step ahead
+template <int MODE>
+struct EmitIn
+{
+    Real acc[NACC];
+    Real R[NVAR];
+    Real u[NVAR];
+};
+
+template <int MODE>
+__device__ __forceinline__ void emit_load(const StageArgs& A, const int e, EmitIn<MODE>& in)
+{
+    const int64_t fs = A.fs;
+    if (MODE == MODE_FIRST) return;
+#pragma unroll
+    for (int q = 0; q < NACC; ++q) in.acc[q] = A.acc[q*fs + e];
+    if (MODE == MODE_CONV || MODE == MODE_STAGE) {
+#pragma unroll
+        for (int q = 0; q < NVAR; ++q) in.R[q] = A.R[q*fs + e];
+    }
+    if (MODE == MODE_STAGE) {
+#pragma unroll
+        for (int q = 0; q < NVAR; ++q) in.u[q] = A.u[q*fs + e];
+    }
+}
+
 // write the contribution of one cell.  F = flux at the cell's right face,
 // Fl = flux at its left face; cG/cP = plus-state + minus-state of G/P.
 template <int MODE>
-__device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, const int e,
+__device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, const int e, const EmitIn<MODE>& in,
                                      const FaceFlux& F, const FaceFlux& Fl, const Real cG, const Real cP)
 {
     Real* __restrict__ acc = A.acc;
@@ -56,26 +88,26 @@ __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, cons
     } else if (MODE == MODE_ACC) {
         // _yextraterm_hllc ("+=", GPUkernels.cu:439-441)
 #pragma unroll
-        for (int q = 0; q < NVAR; ++q) acc[q*fs + e] = (acc[q*fs + e] + F.f[q]) - Fl.f[q];
-        acc[A_SUMG*fs + e] = acc[A_SUMG*fs + e] + cG;
-        acc[A_SUMP*fs + e] = acc[A_SUMP*fs + e] + cP;
-        acc[A_DIVU*fs + e] = acc[A_DIVU*fs + e] + (F.vel - Fl.vel);
+        for (int q = 0; q < NVAR; ++q) acc[q*fs + e] = (in.acc[q] + F.f[q]) - Fl.f[q];
+        acc[A_SUMG*fs + e] = in.acc[A_SUMG] + cG;
+        acc[A_SUMP*fs + e] = in.acc[A_SUMP] + cP;
+        acc[A_DIVU*fs + e] = in.acc[A_DIVU] + (F.vel - Fl.vel);
     } else {
         // _zextraterm_hllc + _divergence (+ _update), GPUkernels.cu:812-814,1104-1136,1164-1170
-        const Real sG = acc[A_SUMG*fs + e] + cG;
-        const Real sP = acc[A_SUMP*fs + e] + cP;
-        const Real dU = acc[A_DIVU*fs + e] + (F.vel - Fl.vel);
+        const Real sG = in.acc[A_SUMG] + cG;
+        const Real sP = in.acc[A_SUMP] + cP;
+        const Real dU = in.acc[A_DIVU] + (F.vel - Fl.vel);
         const Real factor6 = 1.0f / 6.0f;
 #pragma unroll
         for (int q = 0; q < NVAR; ++q) {
-            const Real t = (acc[q*fs + e] + F.f[q]) - Fl.f[q];
+            const Real t = (in.acc[q] + F.f[q]) - Fl.f[q];
             Real rhs;
             if (q == FG)      rhs = dtinvh*(t - dU * sG * factor6);
             else if (q == FP) rhs = dtinvh*(t - dU * sP * factor6);
             else              rhs = dtinvh*t;
-            const Real Rn = A.a*A.R[q*fs + e] - rhs;
+            const Real Rn = A.a*in.R[q] - rhs;
             A.R[q*fs + e] = Rn;
-            if (MODE == MODE_STAGE) A.uo[q*fs + e] = A.b*Rn + A.u[q*fs + e];
+            if (MODE == MODE_STAGE) A.uo[q*fs + e] = A.b*Rn + in.u[q];
         }
     }
 }
@@ -91,14 +123,14 @@ struct March
 
     __device__ __forceinline__ void push(const Prim& q) { w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = q; }
 
-    // reconstruct cell k, solve face k-1/2, emit cell k-1
+    // reconstruct cell k, solve face k-1/2, emit cell k-1 (operands in `in`)
     __device__ __forceinline__ void step(const StageArgs& A, const Real dtinvh, const int k, const int c0, const int e_km1,
-                                         const bool store = true)
+                                         const EmitIn<MODE>& in, const bool store = true)
     {
-        const CellRec r = reconstruct(w0, w1, w2, w3, w4);
+        const CellRec r = reconstruct<Real, kFast>(w0, w1, w2, w3, w4);
         if (k >= c0) {
-            const FaceFlux F = riemann<D>(pm, r.p);
-            if (k >= c0 + 1 && store) emit<MODE>(A, dtinvh, e_km1, F, Fl, pGp + pm.G, pPp + pm.P);
+            const FaceFlux F = riemann<D, Real, kFast>(pm, r.p);
+            if (k >= c0 + 1 && store) emit<MODE>(A, dtinvh, e_km1, in, F, Fl, pGp + pm.G, pPp + pm.P);
             Fl = F;
         }
         pm = r.m; pGp = r.p.G; pPp = r.p.P;
@@ -107,7 +139,7 @@ struct March
 
 __device__ __forceinline__ Prim load_prim(const Real* __restrict__ u, const int64_t fs, const int e)
 {
-    return to_prim(u[e], u[fs + e], u[2*fs + e], u[3*fs + e], u[4*fs + e], u[5*fs + e], u[6*fs + e]);
+    return to_prim<Real, kFast>(u[e], u[fs + e], u[2*fs + e], u[3*fs + e], u[4*fs + e], u[5*fs + e], u[6*fs + e]);
 }
 
 __device__ __forceinline__ Real stage_dtinvh(const StageArgs& A)
@@ -133,75 +165,139 @@ void k_sweep_yz(const Geom g, const StageArgs A, const int chunk)
     const int s = g.stride(D);
     const int e0 = D == 1 ? g.idx(x, 0, t) : g.idx(x, t, 0);
     const Real dtinvh = stage_dtinvh(A);
+    const int64_t fs = A.fs;
 
     March<D, MODE> m;
-    m.w1 = load_prim(A.u, A.fs, e0 + (c0 - 3) * s);
-    m.w2 = load_prim(A.u, A.fs, e0 + (c0 - 2) * s);
-    m.w3 = load_prim(A.u, A.fs, e0 + (c0 - 1) * s);
-    m.w4 = load_prim(A.u, A.fs, e0 + (c0    ) * s);
+    m.w1 = load_prim(A.u, fs, e0 + (c0 - 3) * s);
+    m.w2 = load_prim(A.u, fs, e0 + (c0 - 2) * s);
+    m.w3 = load_prim(A.u, fs, e0 + (c0 - 1) * s);
+    m.w4 = load_prim(A.u, fs, e0 + (c0    ) * s);
+    // software pipeline: raw values of the next incoming cell and the emit
+    // operands of the next step are loaded one step ahead
+    Real nxt[NVAR];
+#pragma unroll
+    for (int q = 0; q < NVAR; ++q) nxt[q] = A.u[q*fs + e0 + (c0 + 1) * s];
+    EmitIn<MODE> ein, ein_next;
     for (int k = c0 - 1; k <= c1; ++k) {
-        m.push(load_prim(A.u, A.fs, e0 + (k + 2) * s));
-        m.step(A, dtinvh, k, c0, e0 + (k - 1) * s);
+        const Prim q = to_prim<Real, kFast>(nxt[0], nxt[1], nxt[2], nxt[3], nxt[4], nxt[5], nxt[6]);  // cell k+2
+        if (k + 3 <= c1 + 2) {
+#pragma unroll
+            for (int f = 0; f < NVAR; ++f) nxt[f] = A.u[f*fs + e0 + (k + 3) * s];
+        }
+        if (k >= c0 && k < c1) emit_load<MODE>(A, e0 + k * s, ein_next);   // emitted at step k+1
+        m.push(q);
+        m.step(A, dtinvh, k, c0, e0 + (k - 1) * s, ein);
+        ein = ein_next;
     }
 }
 
 // ---------------------------------------------------------------------------
-// x sweep: one wave per block, lane = pencil (y,z) flattened; x-runs of SX
-// cells of 64 pencils are converted to primitives and staged in LDS, then each
-// lane marches its own pencil through them.
+// x sweep: one wave per block, lane = pencil (y,z) flattened.  x-runs of SX
+// cells of the block's 64 pencils are loaded with contiguous segments,
+// converted to primitives and staged in LDS; each lane then marches its own
+// pencil through the run.  The run's outputs are staged in LDS as well and
+// stored back as contiguous segments (lane-per-pencil stores would touch 64
+// cache lines per instruction).  The raw values of the next run are loaded
+// into registers while the current run is computed.
 // ---------------------------------------------------------------------------
 enum { SX = 8 };
 
-template <int MODE>
+struct XStage
+{
+    Real in[NVAR][64][SX + 1];     // primitives of the current run
+    Real out[NACC][64][SX + 1];    // outputs of cells xs-3 .. xs+SX-4
+};
+
+// x-sweep output for one cell, written to LDS instead of global memory
+__device__ __forceinline__ void emit_first_lds(XStage& S, const int lane, const int slot, const FaceFlux& F,
+                                               const FaceFlux& Fl, const Real cG, const Real cP)
+{
+#pragma unroll
+    for (int q = 0; q < NVAR; ++q) S.out[q][lane][slot] = F.f[q] - Fl.f[q];
+    S.out[A_SUMG][lane][slot] = cG;
+    S.out[A_SUMP][lane][slot] = cP;
+    S.out[A_DIVU][lane][slot] = F.vel - Fl.vel;
+}
+
 __global__ __launch_bounds__(64)
 void k_sweep_x(const Geom g, const StageArgs A, const int chunk)
 {
-    __shared__ Real sh[NVAR][64][SX + 1];
+    __shared__ XStage S;
     const int lane = threadIdx.x;
     const int npen = g.ny * g.nz;
     const int pb = blockIdx.x * 64;
     const int c0 = blockIdx.y * chunk;
     const int c1 = min(c0 + chunk, g.nx);
-    const Real dtinvh = stage_dtinvh(A);
-
-    // element index of x = 0 for the 8 pencils this lane stages, and for its own pencil
-    int sbase[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-        int p = pb + i * 8 + lane / SX;
-        p = p < npen ? p : npen - 1;
-        sbase[i] = g.idx(0, p % g.ny, p / g.ny);
-    }
-    const int myp = pb + lane;
-    const bool active = myp < npen;
-    const int mybase = active ? g.idx(0, myp % g.ny, myp / g.ny) : 0;
+    const int64_t fs = A.fs;
     const int xl = lane % SX;
 
-    March<0, MODE> m;
-    for (int xs = c0 - 3; xs <= c1 + 2; xs += SX) {
-        __syncthreads();
-        const int xc = xs + xl;
-        if (xc <= c1 + 2) {
+    // element index of x = 0 for the 8 pencils this lane stages/stores
+    int sbase[8];
+    bool sok[8];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-                const Prim q = load_prim(A.u, A.fs, sbase[i] + xc);
-                const int pl = i * 8 + lane / SX;
-                sh[FR][pl][xl] = q.r; sh[FU][pl][xl] = q.u; sh[FV][pl][xl] = q.v; sh[FW][pl][xl] = q.w;
-                sh[FE][pl][xl] = q.p; sh[FG][pl][xl] = q.G; sh[FP][pl][xl] = q.P;
-            }
+    for (int i = 0; i < 8; ++i) {
+        const int p = pb + i * 8 + lane / SX;
+        sok[i] = p < npen;
+        const int pc = sok[i] ? p : npen - 1;
+        sbase[i] = g.idx(0, pc % g.ny, pc / g.ny);
+    }
+    const bool active = pb + lane < npen;
+
+    // raw values of the next run: cell xs + xl of the lane's 8 staging pencils
+    Real pre[8][NVAR];
+    auto load_run = [&](const int xs) {
+        const int xc = min(xs + xl, c1 + 2);   // reads past the run end are clamped into the band
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+#pragma unroll
+            for (int q = 0; q < NVAR; ++q) pre[i][q] = A.u[q * fs + sbase[i] + xc];
+    };
+
+    March<0, MODE_FIRST> m;
+    int xs = c0 - 3;
+    load_run(xs);
+    for (; xs <= c1 + 2; xs += SX) {
+        // stage the run (previous run's consumers and storers are past the barrier below)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const Prim q = to_prim<Real, kFast>(pre[i][0], pre[i][1], pre[i][2], pre[i][3], pre[i][4], pre[i][5], pre[i][6]);
+            const int pl = i * 8 + lane / SX;
+            S.in[FR][pl][xl] = q.r; S.in[FU][pl][xl] = q.u; S.in[FV][pl][xl] = q.v; S.in[FW][pl][xl] = q.w;
+            S.in[FE][pl][xl] = q.p; S.in[FG][pl][xl] = q.G; S.in[FP][pl][xl] = q.P;
         }
+        if (xs + SX <= c1 + 2) load_run(xs + SX);
         __syncthreads();
         const int nj = min(SX, c1 + 3 - xs);
         for (int j = 0; j < nj; ++j) {
             Prim q;
-            q.r = sh[FR][lane][j]; q.u = sh[FU][lane][j]; q.v = sh[FV][lane][j]; q.w = sh[FW][lane][j];
-            q.p = sh[FE][lane][j]; q.G = sh[FG][lane][j]; q.P = sh[FP][lane][j];
+            q.r = S.in[FR][lane][j]; q.u = S.in[FU][lane][j]; q.v = S.in[FV][lane][j]; q.w = S.in[FW][lane][j];
+            q.p = S.in[FE][lane][j]; q.G = S.in[FG][lane][j]; q.P = S.in[FP][lane][j];
             m.push(q);
-            const int k = xs + j - 2;
-            // lanes past the last pencil march a duplicate of it and never store
-            if (k >= c0 - 1) m.step(A, dtinvh, k, c0, mybase + (k - 1), active);
+            const int k = xs + j - 2;              // cell reconstructed at this step
+            if (k >= c0 - 1) {
+                const CellRec r = reconstruct<Real, kFast>(m.w0, m.w1, m.w2, m.w3, m.w4);
+                if (k >= c0) {
+                    const FaceFlux F = riemann<0, Real, kFast>(m.pm, r.p);
+                    if (k >= c0 + 1) emit_first_lds(S, lane, j, F, m.Fl, m.pGp + m.pm.G, m.pPp + m.pm.P);
+                    m.Fl = F;
+                }
+                m.pm = r.m; m.pGp = r.p.G; m.pPp = r.p.P;
+            }
+        }
+        __syncthreads();
+        // outputs of this run are cells xs-3+j for the slots j written above
+        const int xo = xs - 3 + xl;
+        if (xo >= c0 && xo < c1 && xl < nj) {
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                if (!sok[i]) continue;
+                const int pl = i * 8 + lane / SX;
+#pragma unroll
+                for (int q = 0; q < NACC; ++q) A.acc[q * fs + sbase[i] + xo] = S.out[q][pl][xl];
+            }
         }
     }
+    (void)active;
 }
 
 // ---------------------------------------------------------------------------
@@ -328,7 +424,7 @@ __global__ void k_test_weno(const int n, const Real* __restrict__ in5, Real* __r
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const Real* a = in5 + 5*(int64_t)i;
-    const Rec r = weno_cell(a[0], a[1], a[2], a[3], a[4]);
+    const Rec r = weno_cell<Real, kFast>(a[0], a[1], a[2], a[3], a[4]);
     m[i] = r.m; p[i] = r.p;
 }
 
@@ -342,10 +438,10 @@ __global__ void k_test_face(const int n, const Real* __restrict__ st, Real* __re
     const Real* s = st + 42*(int64_t)i;
     Prim c[6];
     for (int k = 0; k < 6; ++k)
-        c[k] = to_prim(s[0*6+k], s[1*6+k], s[2*6+k], s[3*6+k], s[4*6+k], s[5*6+k], s[6*6+k]);
-    const CellRec left = reconstruct(c[0], c[1], c[2], c[3], c[4]);   // cell i-1
-    const CellRec right = reconstruct(c[1], c[2], c[3], c[4], c[5]);  // cell i
-    const FaceFlux F = riemann<D>(left.m, right.p);
+        c[k] = to_prim<Real, kFast>(s[0*6+k], s[1*6+k], s[2*6+k], s[3*6+k], s[4*6+k], s[5*6+k], s[6*6+k]);
+    const CellRec left = reconstruct<Real, kFast>(c[0], c[1], c[2], c[3], c[4]);   // cell i-1
+    const CellRec right = reconstruct<Real, kFast>(c[1], c[2], c[3], c[4], c[5]);  // cell i
+    const FaceFlux F = riemann<D, Real, kFast>(left.m, right.p);
     Real* o = out + 12*(int64_t)i;
     for (int q = 0; q < NVAR; ++q) o[q] = F.f[q];
     o[7] = F.vel; o[8] = left.m.G; o[9] = right.p.G; o[10] = left.m.P; o[11] = right.p.P;
@@ -396,28 +492,24 @@ void launch_dt(StepState* st, const double cfl, const double h, const double dt_
     hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, s, st, cfl, h, dt_max);
 }
 
-// mode: MODE_CONV (R only) or MODE_STAGE (R and u_out)
-void launch_rhs(const Geom& g, const Real* u, Real* R, Real* acc, Real* uo, const Real a, const Real b,
-                const Real dtinvh, const StepState* st, const int stage, hipStream_t s)
+// one direction sweep.  dir 0 assigns the accumulators, dir 1 accumulates,
+// dir 2 finishes the register update (stage == 0: R only, as Convection;
+// stage == 1: R and the new solution, as process_all).
+void launch_sweep(const Geom& g, const int dir, const SweepParams& p, const int stage, hipStream_t s)
 {
     StageArgs A;
-    A.u = u; A.R = R; A.acc = acc; A.uo = uo; A.fs = g.fstride;
-    A.a = a; A.b = b; A.dtinvh = dtinvh; A.st = st;
-    // x
-    {
+    A.u = p.u; A.R = p.R; A.acc = p.acc; A.uo = p.uo; A.fs = g.fstride;
+    A.a = p.a; A.b = p.b; A.dtinvh = p.dtinvh; A.st = p.st;
+    if (dir == 0) {
         const int npen = g.ny * g.nz;
         const int chunk = pick_chunk(g.nx, npen, 1);
         dim3 grid((npen + 63) / 64, (g.nx + chunk - 1) / chunk);
-        hipLaunchKernelGGL(k_sweep_x<MODE_FIRST>, grid, dim3(64), 0, s, g, A, chunk);
-    }
-    // y
-    {
+        hipLaunchKernelGGL(k_sweep_x, grid, dim3(64), 0, s, g, A, chunk);
+    } else if (dir == 1) {
         const int chunk = pick_chunk(g.ny, (int64_t)g.nx * g.nz, 1);
         dim3 grid((g.nx + 63) / 64, (g.nz + 3) / 4, (g.ny + chunk - 1) / chunk);
         hipLaunchKernelGGL((k_sweep_yz<1, MODE_ACC>), grid, dim3(64, 4), 0, s, g, A, chunk);
-    }
-    // z
-    {
+    } else {
         const int chunk = pick_chunk(g.nz, (int64_t)g.nx * g.ny, 1);
         dim3 grid((g.nx + 63) / 64, (g.ny + 3) / 4, (g.nz + chunk - 1) / chunk);
         if (stage) hipLaunchKernelGGL((k_sweep_yz<2, MODE_STAGE>), grid, dim3(64, 4), 0, s, g, A, chunk);

@@ -1,11 +1,11 @@
-// mpcf_scheme.h -- device arithmetic of the stiffened-gas RHS.
+// mpcf_scheme.h -- arithmetic of the stiffened-gas RHS (host + device).
 //
 // The scheme is the reference's (WENO5-JS reconstruction of primitives with
 // clipping, Einfeldt/Batten wave speeds, HLLC fluxes, quasi-conservative G/P
 // terms), re-organised around cells instead of faces:
 //
-//  * primitives (rho,u,v,w,p,G,P) are recovered once per cell, not once per
-//    stencil point per face as in source/GPU/GPUkernels.cu:209-244;
+//  * primitives (rho,u,v,w,p,G,P) are recovered once per cell and sweep, not
+//    once per stencil point per face as in source/GPU/GPUkernels.cu:209-244;
 //  * each cell is reconstructed once per direction, giving both its
 //    plus-state at face j-1/2 and its minus-state at face j+1/2.  The
 //    smoothness indicators of _weno_minus(j-2..j+2) and _weno_pluss(j-2..j+2)
@@ -13,36 +13,78 @@
 //    same operand order (minus is0 == plus is2, is1 == is1, is2 == is0), so
 //    they and the binary64 reciprocals 1/(10 is^2) are computed once;
 //  * the HLLC helpers share the side ratios chi = (s-vn)/(s-s*).
-// These are exact re-uses of identical expressions: with MPCF_FAST == 0 and
+// These are exact re-uses of identical expressions: with FAST == false and
 // -ffp-contract=off every value is bit-identical to the reference formula.
 //
-// MPCF_FAST == 1 keeps the same structure but lets the compiler contract
-// a*b+c into FMA (built with -ffp-contract=fast); results then differ from the
-// reference in the last bits and are checked at a stated tolerance.
+// FAST == true evaluates the WENO weights in binary32, replaces divisions and
+// square roots by the hardware reciprocal / square root (1 ulp) and is built
+// with FMA contraction; results then differ from the reference in the last
+// bits and are checked at a stated tolerance.
+//
+// Every function is templated on the scalar type T (float on the device) so
+// that the host can instantiate it with a counting type and report the exact
+// operation count of the scheme as implemented here.
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <cmath>
 #include "mpcf_geom.h"
-
-#ifndef MPCF_FAST
-#define MPCF_FAST 0
-#endif
 
 namespace mpcf {
 
 typedef float Real;
 
-struct Prim { Real r, u, v, w, p, G, P; };
+// binary64 companion of a binary32 scalar type
+template <typename T> struct Wide { typedef double type; };
+
+MPCF_HD float mmin(float a, float b) { return fminf(a, b); }
+MPCF_HD float mmax(float a, float b) { return fmaxf(a, b); }
+MPCF_HD float msqrt(float a) { return sqrtf(a); }
+// approximate reciprocal / square root of the FAST build: the hardware's
+// v_rcp_f32 / v_sqrt_f32 (1 ulp) instead of the correctly rounded sequences
+MPCF_HD float mrcp(float a)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(a);
+#else
+    return 1.0f / a;
+#endif
+}
+MPCF_HD float msqrt_fast(float a)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrtf(a);
+#else
+    return sqrtf(a);
+#endif
+}
+
+// a / b: IEEE division (bit-exact build) or a * rcp(b) (FAST)
+template <bool FAST, typename T> MPCF_HD T qdiv(const T a, const T b) { return FAST ? a * mrcp(b) : a / b; }
+template <bool FAST, typename T> MPCF_HD T qsqrt(const T a) { return FAST ? msqrt_fast(a) : msqrt(a); }
+
+template <typename T>
+struct PrimT { T r, u, v, w, p, G, P; };
+typedef PrimT<Real> Prim;
 
 // primitive recovery (GPUkernels.cu:209-244: u=ru/r, ..., p=(E-0.5 r|u|^2-P)/G)
-__device__ __forceinline__ Prim to_prim(Real r, Real ru, Real rv, Real rw, Real E, Real G, Real P)
+template <typename T, bool FAST = false>
+MPCF_HD PrimT<T> to_prim(T r, T ru, T rv, T rw, T E, T G, T P)
 {
-    Prim q;
+    PrimT<T> q;
     q.r = r;
-    q.u = ru / r;
-    q.v = rv / r;
-    q.w = rw / r;
-    q.p = (E - 0.5f*r*(q.u*q.u + q.v*q.v + q.w*q.w) - P) / G;
+    if (FAST) {
+        const T ir = mrcp(r);
+        q.u = ru * ir;
+        q.v = rv * ir;
+        q.w = rw * ir;
+        q.p = (E - 0.5f*r*(q.u*q.u + q.v*q.v + q.w*q.w) - P) * mrcp(G);
+    } else {
+        q.u = ru / r;
+        q.v = rv / r;
+        q.w = rw / r;
+        q.p = (E - 0.5f*r*(q.u*q.u + q.v*q.v + q.w*q.w) - P) / G;
+    }
     q.G = G;
     q.P = P;
     return q;
@@ -52,170 +94,173 @@ __device__ __forceinline__ Prim to_prim(Real r, Real ru, Real rv, Real rw, Real 
 // m: minus-state at face j+1/2   == _weno_minus_clipped(a,b,c,d,e)
 // p: plus-state  at face j-1/2   == _weno_pluss_clipped(a,b,c,d,e)
 // (GPUonly.cuh:99-194; WENOEPS = 1e-6, Makefile.config:15)
-struct Rec { Real m, p; };
+template <typename T> struct RecT { T m, p; };
 
-__device__ __forceinline__ Rec weno_cell(const Real a, const Real b, const Real c, const Real d, const Real e)
+template <typename T, bool FAST>
+MPCF_HD RecT<T> weno_cell(const T a, const T b, const T c, const T d, const T e)
 {
-    const Real k4_3 = (Real)(4./3.), k19_3 = (Real)(19./3.), k11_3 = (Real)(11./3.);
-    const Real k25_3 = (Real)(25./3.), k31_3 = (Real)(31./3.), k10_3 = (Real)(10./3.);
-    const Real k13_3 = (Real)(13./3.), k5_3 = (Real)(5./3.);
+    typedef typename Wide<T>::type W;
+    const T k4_3 = T(4./3.), k19_3 = T(19./3.), k11_3 = T(11./3.);
+    const T k25_3 = T(25./3.), k31_3 = T(31./3.), k10_3 = T(10./3.);
+    const T k13_3 = T(13./3.), k5_3 = T(5./3.);
     // the three smoothness indicators (minus is0/is1/is2 == plus is2/is1/is0)
-    const Real isA = a*(a*k4_3 - b*k19_3 + c*k11_3) + b*(b*k25_3 - c*k31_3) + c*c*k10_3;
-    const Real isB = b*(b*k4_3 - c*k13_3 + d*k5_3) + c*(c*k13_3 - d*k13_3) + d*d*k4_3;
-    const Real isC = c*(c*k10_3 - d*k31_3 + e*k11_3) + d*(d*k25_3 - e*k19_3) + e*e*k4_3;
-    const Real eps = (Real)1e-6;
-    const Real iA = isA + eps, iB = isB + eps, iC = isC + eps;
+    const T isA = a*(a*k4_3 - b*k19_3 + c*k11_3) + b*(b*k25_3 - c*k31_3) + c*c*k10_3;
+    const T isB = b*(b*k4_3 - c*k13_3 + d*k5_3) + c*(c*k13_3 - d*k13_3) + d*d*k4_3;
+    const T isC = c*(c*k10_3 - d*k31_3 + e*k11_3) + d*(d*k25_3 - e*k19_3) + e*e*k4_3;
+    const T eps = T(1e-6);
+    const T iA = isA + eps, iB = isB + eps, iC = isC + eps;
 
-    Real mA0, mA1, mA2, pA0, pA1, pA2;
-#if MPCF_FAST
-    const Real rA = 1.0f / (10.0f*iA*iA);
-    const Real rB = 1.0f / (10.0f*iB*iB);
-    const Real rC = 1.0f / (10.0f*iC*iC);
-    mA0 = rA;        mA1 = 6.0f*rB; mA2 = 3.0f*rC;
-    pA0 = rC;        pA1 = mA1;     pA2 = 3.0f*rA;
-#else
-    // binary64: the reference writes the literal 10.0 (GPUonly.cuh:111-113,150-152)
-    const double rA = 1.0 / (10.0*(double)iA*(double)iA);
-    const double rB = 1.0 / (10.0*(double)iB*(double)iB);
-    const double rC = 1.0 / (10.0*(double)iC*(double)iC);
-    mA0 = (Real)rA;  mA1 = (Real)(6.0*rB); mA2 = (Real)(3.0*rC);
-    pA0 = (Real)rC;  pA1 = mA1;            pA2 = (Real)(3.0*rA);
-#endif
-    const Real k1_3 = (Real)(1./3.), k7_6 = (Real)(7./6.), k11_6 = (Real)(11./6.);
-    const Real k1_6 = (Real)(1./6.), k5_6 = (Real)(5./6.);
+    T mA0, mA1, mA2, pA0, pA1, pA2;
+    if (FAST) {
+        const T rA = mrcp(10.0f*iA*iA);
+        const T rB = mrcp(10.0f*iB*iB);
+        const T rC = mrcp(10.0f*iC*iC);
+        mA0 = rA;        mA1 = 6.0f*rB; mA2 = 3.0f*rC;
+        pA0 = rC;        pA1 = mA1;     pA2 = 3.0f*rA;
+    } else {
+        // binary64: the reference writes the literal 10.0 (GPUonly.cuh:111-113,150-152)
+        const W rA = W(1.0) / (10.0*W(iA)*W(iA));
+        const W rB = W(1.0) / (10.0*W(iB)*W(iB));
+        const W rC = W(1.0) / (10.0*W(iC)*W(iC));
+        mA0 = T(rA);  mA1 = T(6.0*rB); mA2 = T(3.0*rC);
+        pA0 = T(rC);  pA1 = mA1;       pA2 = T(3.0*rA);
+    }
+    const T k1_3 = T(1./3.), k7_6 = T(7./6.), k11_6 = T(11./6.);
+    const T k1_6 = T(1./6.), k5_6 = T(5./6.), mk1_6 = T(-(1./6.));
 
-    Rec out;
+    RecT<T> out;
     {   // _weno_minus(a,b,c,d,e)
-        const Real sum = mA0 + mA1 + mA2;
-        const Real inv = (Real)1 / sum;
-        const Real w0 = mA0 * inv, w1 = mA1 * inv;
-        const Real w2 = 1 - w0 - w1;
-        out.m = w0*(k1_3*a - k7_6*b + k11_6*c) + w1*(-k1_6*b + k5_6*c + k1_3*d) + w2*(k1_3*c + k5_6*d - k1_6*e);
+        const T sum = mA0 + mA1 + mA2;
+        const T inv = qdiv<FAST>(T(1.0f), sum);
+        const T w0 = mA0 * inv, w1 = mA1 * inv;
+        const T w2 = T(1.0f) - w0 - w1;
+        out.m = w0*(k1_3*a - k7_6*b + k11_6*c) + w1*(mk1_6*b + k5_6*c + k1_3*d) + w2*(k1_3*c + k5_6*d - k1_6*e);
     }
     {   // _weno_pluss(a,b,c,d,e)  (its b..f are this cell's a..e)
-        const Real sum = pA0 + pA1 + pA2;
-        const Real inv = (Real)1 / sum;
-        const Real w0 = pA0 * inv, w1 = pA1 * inv;
-        const Real w2 = 1 - w0 - w1;
-        out.p = w0*(k1_3*e - k7_6*d + k11_6*c) + w1*(-k1_6*d + k5_6*c + k1_3*b) + w2*(k1_3*c + k5_6*b - k1_6*a);
+        const T sum = pA0 + pA1 + pA2;
+        const T inv = qdiv<FAST>(T(1.0f), sum);
+        const T w0 = pA0 * inv, w1 = pA1 * inv;
+        const T w2 = T(1.0f) - w0 - w1;
+        out.p = w0*(k1_3*e - k7_6*d + k11_6*c) + w1*(mk1_6*d + k5_6*c + k1_3*b) + w2*(k1_3*c + k5_6*b - k1_6*a);
     }
     // clipping to the three central cells (identical bounds for both states)
-    const Real mn = fminf(fminf(b, c), d);
-    const Real mx = fmaxf(fmaxf(b, c), d);
-    out.m = fminf(fmaxf(out.m, mn), mx);
-    out.p = fminf(fmaxf(out.p, mn), mx);
+    const T mn = mmin(mmin(b, c), d);
+    const T mx = mmax(mmax(b, c), d);
+    out.m = mmin(mmax(out.m, mn), mx);
+    out.p = mmin(mmax(out.p, mn), mx);
     return out;
 }
 
 // one cell's reconstructed face states in one direction
-struct CellRec { Prim m, p; };
+template <typename T> struct CellRecT { PrimT<T> m, p; };
 
-__device__ __forceinline__ CellRec reconstruct(const Prim& A, const Prim& B, const Prim& C, const Prim& D, const Prim& E)
+template <typename T, bool FAST>
+MPCF_HD CellRecT<T> reconstruct(const PrimT<T>& A, const PrimT<T>& B, const PrimT<T>& C, const PrimT<T>& D,
+                                const PrimT<T>& E)
 {
-    CellRec o;
-    Rec t;
-    t = weno_cell(A.r, B.r, C.r, D.r, E.r); o.m.r = t.m; o.p.r = t.p;
-    t = weno_cell(A.u, B.u, C.u, D.u, E.u); o.m.u = t.m; o.p.u = t.p;
-    t = weno_cell(A.v, B.v, C.v, D.v, E.v); o.m.v = t.m; o.p.v = t.p;
-    t = weno_cell(A.w, B.w, C.w, D.w, E.w); o.m.w = t.m; o.p.w = t.p;
-    t = weno_cell(A.p, B.p, C.p, D.p, E.p); o.m.p = t.m; o.p.p = t.p;
-    t = weno_cell(A.G, B.G, C.G, D.G, E.G); o.m.G = t.m; o.p.G = t.p;
-    t = weno_cell(A.P, B.P, C.P, D.P, E.P); o.m.P = t.m; o.p.P = t.p;
+    CellRecT<T> o;
+    RecT<T> t;
+    t = weno_cell<T, FAST>(A.r, B.r, C.r, D.r, E.r); o.m.r = t.m; o.p.r = t.p;
+    t = weno_cell<T, FAST>(A.u, B.u, C.u, D.u, E.u); o.m.u = t.m; o.p.u = t.p;
+    t = weno_cell<T, FAST>(A.v, B.v, C.v, D.v, E.v); o.m.v = t.m; o.p.v = t.p;
+    t = weno_cell<T, FAST>(A.w, B.w, C.w, D.w, E.w); o.m.w = t.m; o.p.w = t.p;
+    t = weno_cell<T, FAST>(A.p, B.p, C.p, D.p, E.p); o.m.p = t.m; o.p.p = t.p;
+    t = weno_cell<T, FAST>(A.G, B.G, C.G, D.G, E.G); o.m.G = t.m; o.p.G = t.p;
+    t = weno_cell<T, FAST>(A.P, B.P, C.P, D.P, E.P); o.m.P = t.m; o.p.P = t.p;
     return o;
 }
 
 // fluxes at one face, in field order, plus the HLLC face velocity of the
 // quasi-conservative G/P terms
-struct FaceFlux { Real f[NVAR]; Real vel; };
+template <typename T> struct FaceFluxT { T f[NVAR]; T vel; };
 
 // D = 0,1,2: normal direction.  L = minus-state (left cell), Rt = plus-state.
 // GPUonly.cuh:802-1042 with argument orders of GPUkernels.cu:259-283 (x),
 // 638-662 (y), 934-957 (z).
-template <int D>
-__device__ __forceinline__ FaceFlux riemann(const Prim& L, const Prim& Rt)
+template <int D, typename T, bool FAST>
+MPCF_HD FaceFluxT<T> riemann(const PrimT<T>& L, const PrimT<T>& Rt)
 {
+    typedef typename Wide<T>::type W;
     // normal and the two transverse velocities in the order _hllc_e receives them
-    const Real vnm  = D == 0 ? L.u  : D == 1 ? L.v  : L.w;
-    const Real vnp  = D == 0 ? Rt.u : D == 1 ? Rt.v : Rt.w;
-    const Real v1m  = D == 0 ? L.v  : L.u;
-    const Real v1p  = D == 0 ? Rt.v : Rt.u;
-    const Real v2m  = D == 2 ? L.v  : L.w;
-    const Real v2p  = D == 2 ? Rt.v : Rt.w;
-    const Real rm = L.r, rp = Rt.r, pm = L.p, pp = Rt.p;
-    const Real Gm = L.G, Gp = Rt.G, Pm = L.P, Pp = Rt.P;
+    const T vnm  = D == 0 ? L.u  : D == 1 ? L.v  : L.w;
+    const T vnp  = D == 0 ? Rt.u : D == 1 ? Rt.v : Rt.w;
+    const T v1m  = D == 0 ? L.v  : L.u;
+    const T v1p  = D == 0 ? Rt.v : Rt.u;
+    const T v2m  = D == 2 ? L.v  : L.w;
+    const T v2p  = D == 2 ? Rt.v : Rt.w;
+    const T rm = L.r, rp = Rt.r, pm = L.p, pp = Rt.p;
+    const T Gm = L.G, Gp = Rt.G, Pm = L.P, Pp = Rt.P;
 
     // _char_vel_einfeldt (GPUonly.cuh:823-846)
-    const Real Rr   = sqrtf(rp / rm);
-    const Real Rinv = 1.0f / (1.0f + Rr);
-    const Real cm2 = ((pm + Pm)/Gm + pm) / rm;
-    const Real cp2 = ((pp + Pp)/Gp + pp) / rp;
-    const Real cm  = sqrtf(cm2);
-    const Real cp  = sqrtf(cp2);
-    const Real eta_2 = 0.5f*Rr*Rinv*Rinv;
-    const Real d2    = (cm2 + Rr*cp2)*Rinv + eta_2*(vnp - vnm)*(vnp - vnm);
-    const Real dd    = sqrtf(d2);
-    const Real ubar  = (vnm + Rr*vnp)*Rinv;
-    const Real sm = fminf(ubar - dd, vnm - cm);
-    const Real sp = fmaxf(ubar + dd, vnp + cp);
+    const T Rr   = qsqrt<FAST>(qdiv<FAST>(rp, rm));
+    const T Rinv = qdiv<FAST>(T(1.0f), 1.0f + Rr);
+    const T cm2 = qdiv<FAST>(qdiv<FAST>(pm + Pm, Gm) + pm, rm);
+    const T cp2 = qdiv<FAST>(qdiv<FAST>(pp + Pp, Gp) + pp, rp);
+    const T cm  = qsqrt<FAST>(cm2);
+    const T cp  = qsqrt<FAST>(cp2);
+    const T eta_2 = 0.5f*Rr*Rinv*Rinv;
+    const T d2    = (cm2 + Rr*cp2)*Rinv + eta_2*(vnp - vnm)*(vnp - vnm);
+    const T dd    = qsqrt<FAST>(d2);
+    const T ubar  = (vnm + Rr*vnp)*Rinv;
+    const T sm = mmin(ubar - dd, vnm - cm);
+    const T sp = mmax(ubar + dd, vnp + cp);
 
     // _char_vel_star (GPUonly.cuh:863-865)
-    const Real facm = rm * (sm - vnm);
-    const Real facp = rp * (sp - vnp);
-    const Real ss = (pp - pm + vnm*facm - vnp*facp) / (facm - facp);
+    const T facm = rm * (sm - vnm);
+    const T facp = rp * (sp - vnp);
+    const T ss = qdiv<FAST>(pp - pm + vnm*facm - vnp*facp, facm - facp);
 
     // shared HLLC terms (GPUonly.cuh:883-889 and the identical lines in the
     // other helpers)
-    const int sign_star = ss == 0 ? 0 : (ss < 0 ? -1 : 1);
-    const Real s_minus = fminf(0.0f, sm);
-    const Real s_pluss = fmaxf(0.0f, sp);
-    const Real chi_m = (sm - vnm) / (sm - ss);
-    const Real chi_p = (sp - vnp) / (sp - ss);
-    const Real wl = 0.5f*(1 + sign_star);
-    const Real wr = 0.5f*(1 - sign_star);
+    const int sign_star = ss == 0.0f ? 0 : (ss < 0.0f ? -1 : 1);
+    const T s_minus = mmin(T(0.0f), sm);
+    const T s_pluss = mmax(T(0.0f), sp);
+    const T chi_m = qdiv<FAST>(sm - vnm, sm - ss);
+    const T chi_p = qdiv<FAST>(sp - vnp, sp - ss);
+    const float wl = 0.5f*(1 + sign_star);
+    const float wr = 0.5f*(1 - sign_star);
+    const double wr_d = 0.5*(1 - sign_star);
 
-    FaceFlux o;
+    FaceFluxT<T> o;
     // _hllc_rho for rho, G, P: right weight is binary64 (GPUonly.cuh:900)
-    auto hllc_rho = [&](const Real qm, const Real qp) -> Real {
-        const Real q_deltam = qm*chi_m - qm;
-        const Real q_deltap = qp*chi_p - qp;
-        const Real fm = qm*vnm;
-        const Real fp = qp*vnp;
-#if MPCF_FAST
-        return wl * (fm + s_minus*q_deltam) + wr * (fp + s_pluss*q_deltap);
-#else
-        return (Real)((double)(wl * (fm + s_minus*q_deltam)) + (0.5*(1 - sign_star)) * (double)(fp + s_pluss*q_deltap));
-#endif
+    auto hllc_rho = [&](const T qm, const T qp) -> T {
+        const T q_deltam = qm*chi_m - qm;
+        const T q_deltap = qp*chi_p - qp;
+        const T fm = qm*vnm;
+        const T fp = qp*vnp;
+        if (FAST) return wl * (fm + s_minus*q_deltam) + wr * (fp + s_pluss*q_deltap);
+        return T(W(wl * (fm + s_minus*q_deltam)) + wr_d * W(fp + s_pluss*q_deltap));
     };
     // _hllc_vel for a transverse momentum (GPUonly.cuh:926-938)
-    auto hllc_vel = [&](const Real vtm, const Real vtp) -> Real {
-        const Real qm = rm*vtm;
-        const Real qp = rp*vtp;
-        const Real q_deltam = qm*chi_m - qm;
-        const Real q_deltap = qp*chi_p - qp;
-        const Real fm = qm*vnm;
-        const Real fp = qp*vnp;
+    auto hllc_vel = [&](const T vtm, const T vtp) -> T {
+        const T qm = rm*vtm;
+        const T qp = rp*vtp;
+        const T q_deltam = qm*chi_m - qm;
+        const T q_deltap = qp*chi_p - qp;
+        const T fm = qm*vnm;
+        const T fp = qp*vnp;
         return wl * (fm + s_minus*q_deltam) + wr * (fp + s_pluss*q_deltap);
     };
     // _hllc_pvel, normal momentum (GPUonly.cuh:967-979)
-    Real fpv;
+    T fpv;
     {
-        const Real qm = rm*vnm;
-        const Real qp = rp*vnp;
-        const Real q_deltam = rm*ss*chi_m - qm;
-        const Real q_deltap = rp*ss*chi_p - qp;
-        const Real fm = qm*vnm + pm;
-        const Real fp = qp*vnp + pp;
+        const T qm = rm*vnm;
+        const T qp = rp*vnp;
+        const T q_deltam = rm*ss*chi_m - qm;
+        const T q_deltap = rp*ss*chi_p - qp;
+        const T fm = qm*vnm + pm;
+        const T fp = qp*vnp + pp;
         fpv = wl * (fm + s_minus*q_deltam) + wr * (fp + s_pluss*q_deltap);
     }
     // _hllc_e (GPUonly.cuh:1011-1023)
-    Real fe;
+    T fe;
     {
-        const Real qm = Gm*pm + Pm + 0.5f*rm*(vnm*vnm + v1m*v1m + v2m*v2m);
-        const Real qp = Gp*pp + Pp + 0.5f*rp*(vnp*vnp + v1p*v1p + v2p*v2p);
-        const Real q_deltam = chi_m*(qm + (ss - vnm)*(rm*ss + pm/(sm - vnm))) - qm;
-        const Real q_deltap = chi_p*(qp + (ss - vnp)*(rp*ss + pp/(sp - vnp))) - qp;
-        const Real fm = vnm*(qm + pm);
-        const Real fp = vnp*(qp + pp);
+        const T qm = Gm*pm + Pm + 0.5f*rm*(vnm*vnm + v1m*v1m + v2m*v2m);
+        const T qp = Gp*pp + Pp + 0.5f*rp*(vnp*vnp + v1p*v1p + v2p*v2p);
+        const T q_deltam = chi_m*(qm + (ss - vnm)*(rm*ss + qdiv<FAST>(pm, sm - vnm))) - qm;
+        const T q_deltap = chi_p*(qp + (ss - vnp)*(rp*ss + qdiv<FAST>(pp, sp - vnp))) - qp;
+        const T fm = vnm*(qm + pm);
+        const T fp = vnp*(qp + pp);
         fe = wl * (fm + s_minus*q_deltam) + wr * (fp + s_pluss*q_deltap);
     }
     o.f[FR] = hllc_rho(rm, rp);
@@ -227,8 +272,8 @@ __device__ __forceinline__ FaceFlux riemann(const Prim& L, const Prim& Rt)
     o.f[FP] = hllc_rho(Pm, Pp);
     // _extraterm_hllc_vel (GPUonly.cuh:1035-1041)
     {
-        const Real chim1 = chi_m - 1.0f;
-        const Real chip1 = chi_p - 1.0f;
+        const T chim1 = chi_m - 1.0f;
+        const T chip1 = chi_p - 1.0f;
         o.vel = wl*(vnm + s_minus*chim1) + wr*(vnp + s_pluss*chip1);
     }
     return o;

@@ -31,7 +31,13 @@ EXPORTS = [
     "mpcf_get_stream", "mpcf_set_bc", "mpcf_upload", "mpcf_download", "mpcf_load_ghosts",
     "mpcf_max_sos", "mpcf_convection", "mpcf_update", "mpcf_process_all", "mpcf_step",
     "mpcf_last_dt", "mpcf_synchronize", "mpcf_query", "mpcf_test_weno", "mpcf_test_face",
+    "mpcf_timing_enable", "mpcf_timing_read", "mpcf_ic_shock_tube", "mpcf_ic_shock_bubble",
+    "mpcf_ic_bubble_cloud", "mpcf_ic_bubble_collapse", "mpcf_flop_count", "mpcf_host_weno",
+    "mpcf_host_face",
 ]
+
+# kernel classes of mpcf_timing_read
+KERNEL_CLASSES = ("ghosts", "max_sos", "dt", "sweep_x", "sweep_y", "sweep_z", "update", "other")
 
 _LIB = None
 
@@ -70,6 +76,16 @@ def lib():
         L.mpcf_query.argtypes = [P, P]
         L.mpcf_test_weno.argtypes = [I, P, P, P, C.c_uint]
         L.mpcf_test_face.argtypes = [I, I, P, P, C.c_uint]
+        L.mpcf_flop_count.argtypes = [C.c_uint, P]
+        L.mpcf_host_weno.argtypes = [I, P, P, P, C.c_uint]
+        L.mpcf_host_face.argtypes = [I, I, P, P, C.c_uint]
+        L.mpcf_timing_enable.argtypes = [P, I]
+        L.mpcf_timing_read.argtypes = [P, P, P]
+        U64 = C.c_ulonglong
+        L.mpcf_ic_shock_tube.argtypes = [I, I, I, I, D, U64, P]
+        L.mpcf_ic_shock_bubble.argtypes = [I, I, I, U64, P]
+        L.mpcf_ic_bubble_cloud.argtypes = [I, I, I, I, U64, P, P]
+        L.mpcf_ic_bubble_collapse.argtypes = [I, I, I, D, D, D, D, D, P]
         for name in EXPORTS:
             if name not in ("mpcf_last_error", "mpcf_version", "mpcf_get_stream"):
                 getattr(L, name).restype = I
@@ -186,10 +202,45 @@ class Block:
     def stream(self):
         return lib().mpcf_get_stream(self._h)
 
+    def timing(self, enable=True):
+        _check(lib().mpcf_timing_enable(self._h, 1 if enable else 0))
+
+    def timing_read(self):
+        """{kernel class: (total ms, launches)} since the last read."""
+        ms = (C.c_double * 8)()
+        cnt = (C.c_longlong * 8)()
+        _check(lib().mpcf_timing_read(self._h, C.cast(ms, C.c_void_p), C.cast(cnt, C.c_void_p)))
+        return {k: (ms[i], cnt[i]) for i, k in enumerate(KERNEL_CLASSES)}
+
     def query(self):
         out = (C.c_longlong * 8)()
         _check(lib().mpcf_query(self._h, C.cast(out, C.c_void_p)))
         return list(out)
+
+
+def flop_count(fast=False):
+    out = (C.c_double * 16)()
+    _check(lib().mpcf_flop_count(FLAG_FAST if fast else 0, C.cast(out, C.c_void_p)))
+    keys = ["sweep_x", "sweep_y", "sweep_z", "cell_stage", "sos_cell", "cell_step",
+            "add", "mul", "div", "sqrt", "dadd", "dmul", "ddiv", "minmax", "cvt"]
+    return {k: out[i] for i, k in enumerate(keys)}
+
+
+def host_weno(in5, fast=False):
+    a = np.ascontiguousarray(in5, dtype=np.float32)
+    n = a.shape[0]
+    m = np.empty(n, np.float32)
+    p = np.empty(n, np.float32)
+    _check(lib().mpcf_host_weno(n, _ptr(a), _ptr(m), _ptr(p), FLAG_FAST if fast else 0))
+    return m, p
+
+
+def host_face(st, d, fast=False):
+    s = np.ascontiguousarray(st, dtype=np.float32)
+    n = s.shape[0]
+    out = np.empty((n, 12), np.float32)
+    _check(lib().mpcf_host_face(n, d, _ptr(s), _ptr(out), FLAG_FAST if fast else 0))
+    return out
 
 
 def test_weno(in5, fast=False):
@@ -207,3 +258,32 @@ def test_face(st, d, fast=False):
     out = np.empty((n, 12), np.float32)
     _check(lib().mpcf_test_face(n, d, _ptr(s), _ptr(out), FLAG_FAST if fast else 0))
     return out
+
+
+def _new_fields(nx, ny, nz):
+    return np.empty((NVAR, nz, ny, nx), dtype=np.float32)
+
+
+def ic_shock_tube(nx, ny, nz, axis=0, noise=1e-3, seed=0):
+    f = _new_fields(nx, ny, nz)
+    _check(lib().mpcf_ic_shock_tube(nx, ny, nz, axis, float(noise), seed, _fields(f)))
+    return f
+
+
+def ic_shock_bubble(nx, ny, nz, seed=0):
+    f = _new_fields(nx, ny, nz)
+    _check(lib().mpcf_ic_shock_bubble(nx, ny, nz, seed, _fields(f)))
+    return f
+
+
+def ic_bubble_cloud(nx, ny, nz, nbubbles=50, seed=0):
+    f = _new_fields(nx, ny, nz)
+    b = np.zeros((nbubbles, 4))
+    _check(lib().mpcf_ic_bubble_cloud(nx, ny, nz, nbubbles, seed, _ptr(b), _fields(f)))
+    return f, b
+
+
+def ic_bubble_collapse(nx, ny, nz, radius, pratio, centre=(0.5, 0.5, 0.5)):
+    f = _new_fields(nx, ny, nz)
+    _check(lib().mpcf_ic_bubble_collapse(nx, ny, nz, float(radius), float(pratio), *map(float, centre), _fields(f)))
+    return f

@@ -121,6 +121,55 @@ int mpcf_synchronize(mpcf_ctx* ctx);
  * stride in elements (low 31 bits); out[7] = device bytes held by the context. */
 int mpcf_query(mpcf_ctx* ctx, long long out[8]);
 
+/* Per-kernel timing.  While enabled, every kernel the context launches is
+ * bracketed by HIP events on the context stream (the reference's
+ * tCUDA_START/tCUDA_STOP timers, source/GPU/GPU.h:25-35).  mpcf_timing_read
+ * synchronizes and returns, per kernel class, the summed milliseconds and the
+ * number of launches since the last read:
+ *   0 ghost fill, 1 max speed of sound, 2 dt, 3 x sweep, 4 y sweep,
+ *   5 z sweep (+ register update), 6 update, 7 other. */
+int mpcf_timing_enable(mpcf_ctx* ctx, int enable);
+int mpcf_timing_read(mpcf_ctx* ctx, double ms[8], long long count[8]);
+
+/* Initial conditions (host arrays, seven fields as above; h = 1/max(NX,NY,NZ),
+ * cell centres h (i + 1/2)).  Material states blended like set_IC of
+ * source/Sim/Sim_SICCloudMPI.cpp:249-276 with tanh-smoothed indicators;
+ * air gamma 1.4, pc 0; water gamma 6.59, pc 4096.
+ *   shock_tube:      air (rho 1, p 1) | water (rho 1000, p 0.1) across 0.5 on
+ *                    `axis`, tanh over 2 cells, rho noise +-`noise` relative
+ *                    (the Sod-type tube of source/Sim/Sim_SodMPI.cpp:19-95).
+ *   shock_bubble:    air bubble r 0.1 at the centre in water (p 1), post-shock
+ *                    water (p 1e3, u 10) for x < 0.3, tanh over 1.5 cells,
+ *                    p noise +-1e-4 (source/Sim/Sim_2DSBIMPI.cpp:86-233, in 3D).
+ *   bubble_cloud:    `nbubbles` air bubbles (p 1), radii lognormal(ln 0.02, 0.3),
+ *                    centres uniform in a ball of radius 0.3 about the centre,
+ *                    overlaps rejected, in water at p 100; bubbles (optional)
+ *                    receives n x 4 doubles x,y,z,r (Sim_SICCloudMPI.cpp:43-357).
+ *   bubble_collapse: one air bubble (p 1) of the given radius and centre in
+ *                    water at p = pratio. */
+int mpcf_ic_shock_tube(int nx, int ny, int nz, int axis, double noise, unsigned long long seed,
+                       float* const fields[7]);
+int mpcf_ic_shock_bubble(int nx, int ny, int nz, unsigned long long seed, float* const fields[7]);
+int mpcf_ic_bubble_cloud(int nx, int ny, int nz, int nbubbles, unsigned long long seed, double* bubbles,
+                         float* const fields[7]);
+int mpcf_ic_bubble_collapse(int nx, int ny, int nz, double radius, double pratio, double cx, double cy,
+                            double cz, float* const fields[7]);
+
+/* Operation count of the scheme as implemented here, from a host run of the
+ * same per-cell code with a counting scalar (binary32/64 add, sub, mul, div,
+ * sqrt = 1 FLOP each; min/max and conversions listed apart):
+ *   out[0..2] FLOPs per cell of the x, y, z sweep; out[3] per cell-stage;
+ *   out[4] max speed of sound per cell; out[5] per cell-step (3 stages + SOS);
+ *   out[6..14] per cell-stage: fp32 add/sub, mul, div, sqrt, fp64 add, mul,
+ *   div, min/max, conversions.  flags: MPCF_FLAG_FAST selects that build. */
+int mpcf_flop_count(unsigned flags, double out[16]);
+
+/* The per-cell scheme compiled for the host in binary32 (same arguments as
+ * the device hooks below); lets the restructured scheme be checked against a
+ * CPU reference without a GPU. */
+int mpcf_host_weno(int n, const float* in5, float* minus, float* plus, unsigned flags);
+int mpcf_host_face(int n, int dir, const float* st, float* out, unsigned flags);
+
 /* Device-function test hooks (the counterpart of GPU::TestKernel,
  * source/GPU/GPUkernels.cu:1354-1382).  Host arrays in and out; synchronous.
  *   mpcf_test_weno: in5 = n x 5 cell values (j-2..j+2); minus[i], plus[i] =
