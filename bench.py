@@ -10,7 +10,9 @@ ghost fill + x/y/z sweeps + register update).  Inputs are resident before
 the timed region; nothing crosses PCIe inside it.
 
 value = N^3 cells x 3 stages x K steps / wall time of the K steps
-(RHS cell-updates/s, whole job).  Per-kernel durations come from HIP events
+(RHS cell-updates/s, whole job), for the FMA-contracted build (default) or the
+bit-exact build (--mode parity); the other build is timed too and reported
+under "other_build".  Per-kernel durations come from HIP events
 recorded around every launch on the context's stream (mpcf_timing_*); the
 roofline line is for the dominant kernel (the most expensive direction sweep)
 with FLOPs from the library's own operation count of the scheme.
@@ -33,11 +35,16 @@ sys.path.insert(0, os.path.join(ROOT, "cubism-jr-gpu-mpcf_amd"))
 PEAK_FP32_TFLOPS = 157.3      # MI355X dense FP32 vector, published
 PEAK_HBM_GBS = 8000.0         # MI355X HBM3E, published
 BYTES_PER_CELL_STAGE = 112.0  # 7 fields x 4 B x (read u, read R, write R, write u)
-BYTES_PER_CELL_DIR = {        # algorithmic HBM bytes per cell of each sweep as built here
-    "sweep_x": 28 + 40,                 # read u, write 10 accumulators
-    "sweep_y": 28 + 40 + 40,            # read u, read + write accumulators
-    "sweep_z": 28 + 40 + 28 + 28 + 28,  # read u, accumulators, R; write R, u'
+# algorithmic HBM bytes per cell of each sweep kernel as built here
+KERNEL_BYTES_PER_CELL = {
+    "k_sweep_xy": 28 + 40,                 # read u, write 10 accumulators
+    "k_sweep_x": 28 + 40,                  # read u, write 10 accumulators
+    "k_sweep_yz<1>": 28 + 40 + 40,         # read u, read + write accumulators
+    "k_sweep_yz<2>": 28 + 40 + 28 + 28 + 28,  # read u, accumulators, R; write R, u'
 }
+# measured HBM bytes per cell of each kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE
+# at 512^3, calibrated on the pack copy kernel), written by the profiling run
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_512.json")
 
 
 def parse():
@@ -46,7 +53,9 @@ def parse():
     p.add_argument("--steps", type=int, default=5)
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--n", type=int, default=512)
-    p.add_argument("--mode", choices=("parity", "fast"), default="parity")
+    p.add_argument("--mode", choices=("parity", "fast"), default="fast")
+    p.add_argument("--no-parity-line", action="store_true",
+                   help="skip the extra timing of the bit-exact build reported under 'parity_build'")
     p.add_argument("--cfl", type=float, default=0.3)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-n", type=int, default=160)
@@ -96,45 +105,67 @@ def main():
     t_ic = time.perf_counter()
     f, bubbles = pympcf.ic_bubble_cloud(n, n, n, 50, 0)
     t_ic = time.perf_counter() - t_ic
-    log(f"initial condition {n}^3 built in {t_ic:.1f} s; uploading")
-    blk = pympcf.Block(n, n, n, 1.0 / n, fast=fast, bc=[pympcf.BC_ABSORBING] * 6)
-    blk.upload(pympcf.DATA, f)
-    blk.upload(pympcf.TMP, np.zeros_like(f))
+    log(f"initial condition {n}^3 built in {t_ic:.1f} s")
+
+    def timed(fast_build, steps, warmup):
+        blk = pympcf.Block(n, n, n, 1.0 / n, fast=fast_build, bc=[pympcf.BC_ABSORBING] * 6)
+        blk.upload(pympcf.DATA, f)
+        blk.upload(pympcf.TMP, np.zeros_like(f))
+        for _ in range(warmup):
+            blk.step(a.cfl, sync=False)
+        blk.synchronize()
+        log(f"{warmup} warm-up steps done; timing {steps} steps ({'fast' if fast_build else 'parity'} build)")
+        blk.timing(True)
+        blk.timing_read()                          # clear
+        blk.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            blk.step(a.cfl, sync=False)
+        blk.synchronize()
+        wall = time.perf_counter() - t0
+        ktimes = blk.timing_read()
+        blk.timing(False)
+        dt, sos = blk.last_dt()
+        fused = blk.fused_xy
+        blk.close()
+        if not (np.isfinite(dt) and dt > 0):
+            raise RuntimeError(f"non-finite dt after the timed steps: {dt}")
+        return wall, ktimes, dt, sos, fused
+
+    wall, ktimes, dt, sos, fused = timed(fast, a.steps, a.warmup)
+    other = None
+    if not a.no_parity_line:
+        w2, k2, _, _, _ = timed(not fast, a.steps, 1)
+        other = {"build": "fast" if not fast else "parity", "value": n ** 3 * 3 * a.steps / w2,
+                 "ms_per_step": w2 / a.steps * 1e3,
+                 "kernels": {k: {"avg_ms": v[0] / v[1]} for k, v in k2.items() if v[1]}}
     del f
-
-    for _ in range(a.warmup):
-        blk.step(a.cfl, sync=False)
-    blk.synchronize()
-    log(f"{a.warmup} warm-up steps done; timing {a.steps} steps ({a.mode})")
-
-    blk.timing(True)
-    blk.timing_read()                          # clear
-    blk.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        blk.step(a.cfl, sync=False)
-    blk.synchronize()
-    wall = time.perf_counter() - t0
-    ktimes = blk.timing_read()
-    blk.timing(False)
-    dt, sos = blk.last_dt()
-    if not (np.isfinite(dt) and dt > 0):
-        raise RuntimeError(f"non-finite dt after the timed steps: {dt}")
 
     cells = n ** 3
     value = cells * 3 * a.steps / wall
     fl = pympcf.flop_count(fast)
-    # dominant kernel: the direction sweep with the largest total time
-    sweeps = {k: ktimes[k] for k in ("sweep_x", "sweep_y", "sweep_z")}
+    # dominant kernel: the sweep kernel with the largest total time; the fused
+    # x+y sweep is timed as "sweep_x" and does the x and the y work
+    kflops = {"sweep_x": fl["sweep_x"] + (fl["sweep_y"] if fused else 0), "sweep_y": fl["sweep_y"],
+              "sweep_z": fl["sweep_z"]}
+    kname = {"sweep_x": "k_sweep_xy" if fused else "k_sweep_x", "sweep_y": "k_sweep_yz<1>",
+             "sweep_z": "k_sweep_yz<2>"}
+    sweeps = {k: ktimes[k] for k in ("sweep_x", "sweep_y", "sweep_z") if ktimes[k][1]}
     dom = max(sweeps, key=lambda k: sweeps[k][0])
     dom_ms, dom_cnt = sweeps[dom]
     dom_avg_s = dom_ms / dom_cnt / 1e3
-    dom_flops = fl[dom] * cells
+    dom_flops = kflops[dom] * cells
     achieved = dom_flops / dom_avg_s / 1e12
     stage_ms = sum(v[0] for v in sweeps.values()) / (3 * a.steps)
     hbm_stage = cells * BYTES_PER_CELL_STAGE / (stage_ms / 1e3) / 1e9
 
     kernels = {k: {"avg_ms": (v[0] / v[1] if v[1] else 0.0), "launches": v[1]} for k, v in ktimes.items() if v[1]}
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        tr = json.load(open(TRAFFIC_FILE)).get(a.mode, {}).get(kname[dom])
+        if tr is not None:
+            traffic = tr * cells
+    alg_bytes = KERNEL_BYTES_PER_CELL[kname[dom]] * cells
     out = {
         "metric": "RHS cell-updates/s at 512^3 FP32",
         "value": value,
@@ -151,23 +182,28 @@ def main():
         "config": {
             "workload": f"bubble-cloud {n}^3 (50 air bubbles in water, p 100/1), LSRK3 CFL {a.cfl}, "
                         f"transmissive BCs, binary32 SoA resident in HBM",
-            "n": n, "mode": a.mode, "parallelism": "single GPU",
+            "build": ("FMA-contracted binary32 with hardware rcp/sqrt; within the bit-exact scheme's own "
+                      "1-ulp noise floor (tests/test_gpu_fast.py)") if fast else "bit-exact reference arithmetic",
+            "n": n, "mode": a.mode, "xy": "fused" if fused else "split", "parallelism": "single GPU",
         },
         "roofline": {
             "bound": "valu",
-            "kernel": dom,
+            "kernel": kname[dom],
             "achieved": achieved,
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS,
-            "traffic": None,
-            "flop_per_cell": fl[dom],
+            "traffic": traffic,
+            "algorithmic_bytes": alg_bytes,
+            "kernel_hbm_GBs": alg_bytes / dom_avg_s / 1e9,
+            "flop_per_cell": kflops[dom],
             "avg_launch_ms": dom_ms / dom_cnt,
             "hbm_stage_GBs": hbm_stage,
             "hbm_stage_frac": hbm_stage / PEAK_HBM_GBS,
         },
         "kernels": kernels,
         "flop_per_cell_stage": fl["cell_stage"],
+        "other_build": other,
         "ic_seconds": t_ic,
         "last_dt": dt,
         "last_sos": sos,
@@ -176,7 +212,6 @@ def main():
     if not a.no_cpu:
         log(f"CPU baseline: oracle on a {a.cpu_n}^3 sample")
         out["cpu_baseline"] = cpu_baseline(a.cpu_n)
-    blk.close()
     print(json.dumps(out))
 
 

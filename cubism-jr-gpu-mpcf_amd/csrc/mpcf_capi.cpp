@@ -67,6 +67,7 @@ const Launchers kFast = {
     mpcf::fast::launch_update, mpcf::fast::launch_pack, mpcf::fast::launch_test_weno,
     mpcf::fast::launch_test_face};
 
+// query helper used by mpcf_query: which x/y variant runs
 // LSRK3 coefficients (LSRK3_IntegratorMPI.cpp:46,51,56), passed as binary32
 // like GPUlab::process_all's Real arguments (GPUlab.h:258)
 const double kA[3] = {0.0, -17. / 32, -32. / 27};
@@ -149,8 +150,18 @@ void run_rhs(mpcf_ctx* c, const float* u, float* uo, float a, float b, float dti
     mpcf::SweepParams p;
     p.u = u; p.R = c->R; p.acc = c->acc; p.uo = uo;
     p.a = a; p.b = b; p.dtinvh = dtinvh; p.st = st;
-    { Timed t(c, K_SWEEP_X); c->L->sweep(c->g, 0, p, stage, c->stream); }
-    { Timed t(c, K_SWEEP_Y); c->L->sweep(c->g, 1, p, stage, c->stream); }
+    // the fused x+y sweep wins when the arithmetic is cheap (FAST); the split
+    // pair wins when it is VALU-bound (bit-exact build), unless forced
+    const bool split = (c->flags & MPCF_FLAG_SPLIT_XY) ? true
+                     : (c->flags & MPCF_FLAG_FUSED_XY) ? false
+                     : !(c->flags & MPCF_FLAG_FAST);
+    if (split) {
+        { Timed t(c, K_SWEEP_X); c->L->sweep(c->g, 0, p, stage, c->stream); }
+        { Timed t(c, K_SWEEP_Y); c->L->sweep(c->g, 1, p, stage, c->stream); }
+    } else {
+        Timed t(c, K_SWEEP_X);
+        c->L->sweep(c->g, 3, p, stage, c->stream);
+    }
     { Timed t(c, K_SWEEP_Z); c->L->sweep(c->g, 2, p, stage, c->stream); }
 }
 
@@ -166,7 +177,10 @@ extern "C" int mpcf_create(mpcf_ctx** out, int nx, int ny, int nz, double h, uns
     *out = nullptr;
     if (nx < 5 || ny < 5 || nz < 5) return fail(MPCF_EINVAL, "mpcf_create: extents must be >= 5");
     if (!(h > 0) || !std::isfinite(h)) return fail(MPCF_EINVAL, "mpcf_create: h must be positive and finite");
-    if (flags & ~(unsigned)(MPCF_FLAG_FAST | MPCF_FLAG_SOS_CPU)) return fail(MPCF_EINVAL, "mpcf_create: unknown flag");
+    if (flags & ~(unsigned)(MPCF_FLAG_FAST | MPCF_FLAG_SOS_CPU | MPCF_FLAG_SPLIT_XY | MPCF_FLAG_FUSED_XY))
+        return fail(MPCF_EINVAL, "mpcf_create: unknown flag");
+    if ((flags & MPCF_FLAG_SPLIT_XY) && (flags & MPCF_FLAG_FUSED_XY))
+        return fail(MPCF_EINVAL, "mpcf_create: SPLIT_XY and FUSED_XY are exclusive");
     const Geom g = mpcf::make_geom(nx, ny, nz);
     if ((int64_t)g.px * g.py * g.pz >= (int64_t(1) << 31))
         return fail(MPCF_EINVAL, "mpcf_create: block too large for 32-bit element indices");

@@ -11,8 +11,10 @@
 // separate x/y ghost buffers and z-halo slices (source/Types.h:37-43,
 // source/GPU/GPUhousehold.cu:105-134): ghosts are filled on the device and
 // every sweep reads its six-point stencil with the same index arithmetic.
-// XOFF = 4 puts the first interior cell of a row on a 16-byte boundary; px is
-// a multiple of 32 elements (128 bytes, one L2 line) so rows start aligned.
+// XOFF = 32 puts the first interior cell of every row on a 128-byte L2 line
+// boundary (px is a multiple of 32 elements), so a wave's 64 lanes along x
+// read exactly two whole lines per field and neighbouring blocks, which run
+// on different XCDs with separate L2s, never fetch the same line.
 // Only the face-adjacent ghost slabs are ever written or read (the stencils
 // are one-directional), so edge and corner padding stays unused.
 #pragma once
@@ -29,7 +31,7 @@
 
 namespace mpcf {
 
-enum { NVAR = 7, NACC = 10, GHOST = 3, XOFF = 4 };
+enum { NVAR = 7, NACC = 10, GHOST = 3, XOFF = 32 };
 enum Field { FR = 0, FU, FV, FW, FE, FG, FP };
 // accumulator slots of the direction sweeps: 7 flux differences, then the
 // cell sums of the reconstructed G and P and the face-velocity difference

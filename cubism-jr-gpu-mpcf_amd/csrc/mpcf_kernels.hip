@@ -64,10 +64,6 @@ __device__ __forceinline__ void emit_load(const StageArgs& A, const int e, EmitI
 #pragma unroll
         for (int q = 0; q < NVAR; ++q) in.R[q] = A.R[q*fs + e];
     }
-    if (MODE == MODE_STAGE) {
-#pragma unroll
-        for (int q = 0; q < NVAR; ++q) in.u[q] = A.u[q*fs + e];
-    }
 }
 
 // write the contribution of one cell.  F = flux at the cell's right face,
@@ -166,12 +162,21 @@ void k_sweep_yz(const Geom g, const StageArgs A, const int chunk)
     const int e0 = D == 1 ? g.idx(x, 0, t) : g.idx(x, t, 0);
     const Real dtinvh = stage_dtinvh(A);
     const int64_t fs = A.fs;
+    constexpr bool kRaw = MODE == MODE_STAGE;   // the update needs the raw u of the emitted cell
 
     March<D, MODE> m;
+    // raw conserved values of cells k-1 .. k+2 (kept for the register update)
+    Real r0[NVAR], r1[NVAR], r2[NVAR], r3[NVAR];
+#pragma unroll
+    for (int q = 0; q < NVAR; ++q) {
+        r1[q] = A.u[q*fs + e0 + (c0 - 2) * s];
+        r2[q] = A.u[q*fs + e0 + (c0 - 1) * s];
+        r3[q] = A.u[q*fs + e0 + (c0    ) * s];
+    }
     m.w1 = load_prim(A.u, fs, e0 + (c0 - 3) * s);
-    m.w2 = load_prim(A.u, fs, e0 + (c0 - 2) * s);
-    m.w3 = load_prim(A.u, fs, e0 + (c0 - 1) * s);
-    m.w4 = load_prim(A.u, fs, e0 + (c0    ) * s);
+    m.w2 = to_prim<Real, kFast>(r1[0], r1[1], r1[2], r1[3], r1[4], r1[5], r1[6]);
+    m.w3 = to_prim<Real, kFast>(r2[0], r2[1], r2[2], r2[3], r2[4], r2[5], r2[6]);
+    m.w4 = to_prim<Real, kFast>(r3[0], r3[1], r3[2], r3[3], r3[4], r3[5], r3[6]);
     // software pipeline: raw values of the next incoming cell and the emit
     // operands of the next step are loaded one step ahead
     Real nxt[NVAR];
@@ -179,12 +184,19 @@ void k_sweep_yz(const Geom g, const StageArgs A, const int chunk)
     for (int q = 0; q < NVAR; ++q) nxt[q] = A.u[q*fs + e0 + (c0 + 1) * s];
     EmitIn<MODE> ein, ein_next;
     for (int k = c0 - 1; k <= c1; ++k) {
+        // rotate the raw ring: r0 = cell k-1, r3 = cell k+2
+#pragma unroll
+        for (int q = 0; q < NVAR; ++q) { r0[q] = r1[q]; r1[q] = r2[q]; r2[q] = r3[q]; r3[q] = nxt[q]; }
         const Prim q = to_prim<Real, kFast>(nxt[0], nxt[1], nxt[2], nxt[3], nxt[4], nxt[5], nxt[6]);  // cell k+2
         if (k + 3 <= c1 + 2) {
 #pragma unroll
             for (int f = 0; f < NVAR; ++f) nxt[f] = A.u[f*fs + e0 + (k + 3) * s];
         }
         if (k >= c0 && k < c1) emit_load<MODE>(A, e0 + k * s, ein_next);   // emitted at step k+1
+        if (kRaw) {
+#pragma unroll
+            for (int f = 0; f < NVAR; ++f) ein.u[f] = r0[f];                 // cell k-1
+        }
         m.push(q);
         m.step(A, dtinvh, k, c0, e0 + (k - 1) * s, ein);
         ein = ein_next;
@@ -298,6 +310,107 @@ void k_sweep_x(const Geom g, const StageArgs A, const int chunk)
         }
     }
     (void)active;
+}
+
+// ---------------------------------------------------------------------------
+// fused x + y sweep: lanes along x, marching y.  A wave covers 64 consecutive
+// x-cells starting 3 cells left of its tile and writes the 58 cells whose
+// x-stencils lie inside the wave (lanes 3..60); neighbouring tiles overlap by
+// six cells.  The x-direction of row k-1 is computed from the rolling window's
+// row k-1 with wave-shifted copies (DPP wave_shr/wave_shl) for the five-point
+// windows, the left face state and the right face flux; the y-direction is
+// the same register march as the y sweep.  The accumulators leave this kernel
+// already summed in the x-then-y order of the reference (GPUkernels.cu:1111),
+// so the x sweep's accumulator round trip disappears.
+// ---------------------------------------------------------------------------
+enum { XY_TILE = 58 };
+
+// value of lane l-1 / lane l+1; the wave's edge lanes keep their own value
+__device__ __forceinline__ Real from_left(const Real v)
+{
+    const int b = __builtin_bit_cast(int, v);
+    return __builtin_bit_cast(Real, __builtin_amdgcn_update_dpp(b, b, 0x138, 0xf, 0xf, false));   // wave_shr:1
+}
+__device__ __forceinline__ Real from_right(const Real v)
+{
+    const int b = __builtin_bit_cast(int, v);
+    return __builtin_bit_cast(Real, __builtin_amdgcn_update_dpp(b, b, 0x130, 0xf, 0xf, false));   // wave_shl:1
+}
+__device__ __forceinline__ Prim prim_from_left(const Prim& q)
+{
+    Prim o;
+    o.r = from_left(q.r); o.u = from_left(q.u); o.v = from_left(q.v); o.w = from_left(q.w);
+    o.p = from_left(q.p); o.G = from_left(q.G); o.P = from_left(q.P);
+    return o;
+}
+__device__ __forceinline__ Prim prim_from_right(const Prim& q)
+{
+    Prim o;
+    o.r = from_right(q.r); o.u = from_right(q.u); o.v = from_right(q.v); o.w = from_right(q.w);
+    o.p = from_right(q.p); o.G = from_right(q.G); o.P = from_right(q.P);
+    return o;
+}
+
+__global__ __launch_bounds__(256)
+void k_sweep_xy(const Geom g, const StageArgs A, const int chunk)
+{
+    const int lane = threadIdx.x;
+    const int z = blockIdx.y * 4 + threadIdx.y;
+    if (z >= g.nz) return;                        // uniform per wave; no barriers below
+    const int x = blockIdx.x * XY_TILE + lane - 3;
+    const int xc = min(x, g.nx + 2);              // lanes past the band read its last cell
+    const bool store = lane >= 3 && lane <= 60 && x < g.nx;
+    const int c0 = blockIdx.z * chunk;
+    const int c1 = min(c0 + chunk, g.ny);
+    const int s = g.px;
+    const int e0 = g.idx(xc, 0, z);
+    const int64_t fs = A.fs;
+    Real* __restrict__ acc = A.acc;
+
+    March<1, MODE_ACC> m;
+    m.w1 = load_prim(A.u, fs, e0 + (c0 - 3) * s);
+    m.w2 = load_prim(A.u, fs, e0 + (c0 - 2) * s);
+    m.w3 = load_prim(A.u, fs, e0 + (c0 - 1) * s);
+    m.w4 = load_prim(A.u, fs, e0 + (c0    ) * s);
+    Real nxt[NVAR];
+#pragma unroll
+    for (int q = 0; q < NVAR; ++q) nxt[q] = A.u[q*fs + e0 + (c0 + 1) * s];
+    for (int k = c0 - 1; k <= c1; ++k) {
+        const Prim q = to_prim<Real, kFast>(nxt[0], nxt[1], nxt[2], nxt[3], nxt[4], nxt[5], nxt[6]);  // row k+2
+        if (k + 3 <= c1 + 2) {
+#pragma unroll
+            for (int f = 0; f < NVAR; ++f) nxt[f] = A.u[f*fs + e0 + (k + 3) * s];
+        }
+        m.push(q);
+        const CellRec ry = reconstruct<Real, kFast>(m.w0, m.w1, m.w2, m.w3, m.w4);   // row k, y-direction
+        if (k >= c0) {
+            const FaceFlux Fy = riemann<1, Real, kFast>(m.pm, ry.p);                // face k-1/2
+            if (k >= c0 + 1) {
+                // x-direction of row k-1 (window slot w1)
+                const Prim B = prim_from_left(m.w1), D = prim_from_right(m.w1);
+                const Prim Aa = prim_from_left(B), E = prim_from_right(D);
+                const CellRec rx = reconstruct<Real, kFast>(Aa, B, m.w1, D, E);
+                const Prim Lm = prim_from_left(rx.m);
+                const FaceFlux Fxl = riemann<0, Real, kFast>(Lm, rx.p);            // face x-1/2
+                FaceFlux Fxr;                                                        // face x+1/2
+#pragma unroll
+                for (int f = 0; f < NVAR; ++f) Fxr.f[f] = from_right(Fxl.f[f]);
+                Fxr.vel = from_right(Fxl.vel);
+                if (store) {
+                    const int e = e0 + (k - 1) * s;
+                    const Real cGy = m.pGp + m.pm.G, cPy = m.pPp + m.pm.P;
+#pragma unroll
+                    for (int f = 0; f < NVAR; ++f)
+                        acc[f*fs + e] = ((Fxr.f[f] - Fxl.f[f]) + Fy.f[f]) - m.Fl.f[f];
+                    acc[A_SUMG*fs + e] = (rx.p.G + rx.m.G) + cGy;
+                    acc[A_SUMP*fs + e] = (rx.p.P + rx.m.P) + cPy;
+                    acc[A_DIVU*fs + e] = (Fxr.vel - Fxl.vel) + (Fy.vel - m.Fl.vel);
+                }
+            }
+            m.Fl = Fy;
+        }
+        m.pm = ry.m; m.pGp = ry.p.G; m.pPp = ry.p.P;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -492,15 +605,20 @@ void launch_dt(StepState* st, const double cfl, const double h, const double dt_
     hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, s, st, cfl, h, dt_max);
 }
 
-// one direction sweep.  dir 0 assigns the accumulators, dir 1 accumulates,
-// dir 2 finishes the register update (stage == 0: R only, as Convection;
-// stage == 1: R and the new solution, as process_all).
+// one direction sweep.  dir 0 (x) assigns the accumulators, dir 1 (y)
+// accumulates, dir 3 does x and y fused (assigns), dir 2 (z) finishes the
+// register update (stage == 0: R only, as Convection; stage == 1: R and the
+// new solution, as process_all).
 void launch_sweep(const Geom& g, const int dir, const SweepParams& p, const int stage, hipStream_t s)
 {
     StageArgs A;
     A.u = p.u; A.R = p.R; A.acc = p.acc; A.uo = p.uo; A.fs = g.fstride;
     A.a = p.a; A.b = p.b; A.dtinvh = p.dtinvh; A.st = p.st;
-    if (dir == 0) {
+    if (dir == 3) {
+        const int chunk = pick_chunk(g.ny, (int64_t)g.nx * g.nz, 1);
+        dim3 grid((g.nx + XY_TILE - 1) / XY_TILE, (g.nz + 3) / 4, (g.ny + chunk - 1) / chunk);
+        hipLaunchKernelGGL(k_sweep_xy, grid, dim3(64, 4), 0, s, g, A, chunk);
+    } else if (dir == 0) {
         const int npen = g.ny * g.nz;
         const int chunk = pick_chunk(g.nx, npen, 1);
         dim3 grid((npen + 63) / 64, (g.nx + chunk - 1) / chunk);

@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(_HERE, "libmpcf_hip.so")
 
 OK, EINVAL, EHIP, ENOMEM, ENUMERIC = 0, 1, 2, 3, 4
 BC_ABSORBING, BC_REFLECTING = 0, 1
-FLAG_FAST, FLAG_SOS_CPU = 1, 2
+FLAG_FAST, FLAG_SOS_CPU, FLAG_SPLIT_XY, FLAG_FUSED_XY = 1, 2, 4, 8
 DATA, TMP = 0, 1
 NVAR = 7
 
@@ -38,6 +38,7 @@ EXPORTS = [
 
 # kernel classes of mpcf_timing_read
 KERNEL_CLASSES = ("ghosts", "max_sos", "dt", "sweep_x", "sweep_y", "sweep_z", "update", "other")
+# with the fused x+y sweep, "sweep_x" times that kernel and "sweep_y" is unused
 
 _LIB = None
 
@@ -112,9 +113,15 @@ def _fields(arr):
 class Block:
     """One resident node block (an mpcf_ctx)."""
 
-    def __init__(self, nx, ny, nz, h, fast=False, sos_cpu=False, bc=None):
+    def __init__(self, nx, ny, nz, h, fast=False, sos_cpu=False, bc=None, split_xy=None):
+        """split_xy: None = the library's default for the build, True/False = force."""
         self.shape = (nz, ny, nx)
         flags = (FLAG_FAST if fast else 0) | (FLAG_SOS_CPU if sos_cpu else 0)
+        if split_xy is True:
+            flags |= FLAG_SPLIT_XY
+        elif split_xy is False:
+            flags |= FLAG_FUSED_XY
+        self.fused_xy = (split_xy is False) or (split_xy is None and fast)
         h_ = C.c_void_p()
         _check(lib().mpcf_create(C.byref(h_), nx, ny, nz, float(h), flags))
         self._h = h_

@@ -43,8 +43,11 @@ enum {
 /* context flags */
 enum {
     MPCF_FLAG_FAST = 1u,     /* FMA-contracted kernels (tolerance parity) instead of the bit-exact build */
-    MPCF_FLAG_SOS_CPU = 2u   /* speed of sound with p recovered in binary64 as MaxSpeedOfSound_CPP
+    MPCF_FLAG_SOS_CPU = 2u,  /* speed of sound with p recovered in binary64 as MaxSpeedOfSound_CPP
                                 (source/MaxSpeedOfSound.cpp:43) instead of the GPU form (GPUkernels.cu:1206) */
+    MPCF_FLAG_SPLIT_XY = 4u, /* run x and y as two sweeps ...                                          */
+    MPCF_FLAG_FUSED_XY = 8u  /* ... or as one fused x+y sweep (same results either way).  Default: fused
+                                with MPCF_FLAG_FAST, split otherwise (the faster one for each build).  */
 };
 
 /* which host array set */
@@ -126,8 +129,9 @@ int mpcf_query(mpcf_ctx* ctx, long long out[8]);
  * tCUDA_START/tCUDA_STOP timers, source/GPU/GPU.h:25-35).  mpcf_timing_read
  * synchronizes and returns, per kernel class, the summed milliseconds and the
  * number of launches since the last read:
- *   0 ghost fill, 1 max speed of sound, 2 dt, 3 x sweep, 4 y sweep,
- *   5 z sweep (+ register update), 6 update, 7 other. */
+ *   0 ghost fill, 1 max speed of sound, 2 dt, 3 x sweep (or the fused x+y
+ *   sweep), 4 y sweep (split variant only), 5 z sweep (+ register update),
+ *   6 update, 7 other. */
 int mpcf_timing_enable(mpcf_ctx* ctx, int enable);
 int mpcf_timing_read(mpcf_ctx* ctx, double ms[8], long long count[8]);
 

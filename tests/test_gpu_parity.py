@@ -77,18 +77,20 @@ def test_max_sos_bitexact(mpcf, oracle, cpu_form):
     assert np.float32(got) == np.float32(ref), (got, ref)
 
 
-BCS = [[0] * 6, [1] * 6, [0, 1, 1, 0, 1, 0]]
-SHAPES = [(20, 24, 28), (33, 17, 9), (7, 66, 12)]
+# the last shape spans three fused x+y tiles (58 cells each) with a partial one
+BCS = [[0] * 6, [1] * 6, [0, 1, 1, 0, 1, 0], [1, 0, 0, 1, 0, 1]]
+SHAPES = [(20, 24, 28), (33, 17, 9), (7, 66, 12), (131, 9, 6)]
 
 
+@pytest.mark.parametrize("split", [False, True], ids=["fused_xy", "split_xy"])
 @pytest.mark.parametrize("shape,bc", list(zip(SHAPES, BCS)))
-def test_ghosts_and_convection_bitexact(mpcf, oracle, shape, bc):
+def test_ghosts_and_convection_bitexact(mpcf, oracle, shape, bc, split):
     nx, ny, nz = shape
     s = states.random_smooth(nx, ny, nz, seed=nx + ny)
     R0 = states.random_smooth(nx, ny, nz, seed=99, two_phase=False) * np.float32(1e-3)
     a, dtinvh = np.float32(-17.0 / 32), np.float32(0.0123)
-    say("convection", shape, "bc", bc)
-    with mpcf.Block(nx, ny, nz, 1.0 / max(shape), bc=bc) as b:
+    say("convection", shape, "bc", bc, "split_xy", split)
+    with mpcf.Block(nx, ny, nz, 1.0 / max(shape), bc=bc, split_xy=split) as b:
         b.data = s
         b.tmp = R0
         b.load_ghosts()
@@ -133,16 +135,16 @@ def test_process_all_matches_convection_update(mpcf, oracle):
     assert_equal(R1, ob.tmp, "R vs oracle")
 
 
-@pytest.mark.parametrize("cpu_sos", [False, True])
-def test_lsrk3_steps_bitexact(mpcf, oracle, cpu_sos):
+@pytest.mark.parametrize("cpu_sos,split", [(False, False), (True, False), (False, True)])
+def test_lsrk3_steps_bitexact(mpcf, oracle, cpu_sos, split):
     nx, ny, nz = 28, 24, 20
     h = 1.0 / 28
     bc = [0, 1, 0, 0, 1, 0]
     s = states.random_smooth(nx, ny, nz, seed=11)
-    say("lsrk3 3 steps", (nx, ny, nz), "cpu_sos", cpu_sos)
+    say("lsrk3 3 steps", (nx, ny, nz), "cpu_sos", cpu_sos, "split_xy", split)
     ob = oracle.Block(nx, ny, nz)
     ob.data = s
-    with mpcf.Block(nx, ny, nz, h, sos_cpu=cpu_sos, bc=bc) as b:
+    with mpcf.Block(nx, ny, nz, h, sos_cpu=cpu_sos, bc=bc, split_xy=split) as b:
         b.data = s
         for it in range(3):
             dt = b.step(0.3)
