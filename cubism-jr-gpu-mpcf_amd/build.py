@@ -6,7 +6,8 @@ Objects:
   parity.o  mpcf_kernels.hip, MPCF_FAST=0, -ffp-contract=off  (bit-exact scheme)
   fast.o    mpcf_kernels.hip, MPCF_FAST=1, -ffp-contract=fast (FMA-contracted)
   capi.o    the C-ABI (include/mpcf_hip.h)
-  facades.o the C++ mirror of the reference's kernel facades and integrator
+  ic.o      initial conditions;  flops.o  host instantiations of the scheme
+and the C++ driver mpcf_run (csrc/mpcf_run.cpp over csrc/mpcf.hpp).
 All HIP code is compiled with packed-FP32 instructions disabled
 (-target-feature -packed-fp32-ops).
 """
@@ -58,7 +59,6 @@ def build(verbose: bool = False) -> str:
         ("parity.o", "mpcf_kernels.hip", ["-DMPCF_FAST=0", "-ffp-contract=off"]),
         ("fast.o", "mpcf_kernels.hip", ["-DMPCF_FAST=1", "-ffp-contract=fast"]),
         ("capi.o", "mpcf_capi.cpp", ["-x", "hip"]),
-        ("facades.o", "facades.cpp", ["-x", "hip"]),
         ("ic.o", "ic.cpp", ["-x", "hip"]),
         ("flops.o", "flops.cpp", ["-x", "hip", "-ffp-contract=off"]),
     ]
@@ -76,6 +76,12 @@ def build(verbose: bool = False) -> str:
     tmp = LIB + ".tmp"
     _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
     os.replace(tmp, LIB)
+    # the C++ driver over the C-ABI (mpcf.hpp), next to the library
+    exe = os.path.join(PKG, "mpcf_run")
+    _run(["g++", "-O2", "-std=c++17", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+          os.path.join(CSRC, "mpcf_run.cpp"), "-o", exe + ".tmp", "-L", PKG, "-lmpcf_hip",
+          "-L/opt/rocm/lib", "-Wl,-rpath,$ORIGIN", "-Wl,-rpath,/opt/rocm/lib"])
+    os.replace(exe + ".tmp", exe)
     return LIB
 
 
