@@ -18,8 +18,13 @@ roofline line is for the dominant kernel (the most expensive direction sweep)
 with FLOPs from the library's own operation count of the scheme.
 
 cpu_baseline: the CPU oracle (the restated reference scheme, OpenMP over
-z-slices) timed on this host on a bounded sample (a 128^3 block of the same
+z-slices) timed on this host on a bounded sample (a 160^3 block of the same
 workload, one LSRK3 step), scaled to the same unit.
+
+ensemble / serving (the metric's second half: p50/p99 against the plain
+solver): 64 uneven bubble-collapse cases batched vs one at a time, and a
+Poisson request stream served with continuous batching vs first-come
+first-served one at a time (cubism-jr-gpu-mpcf_amd/ensemble.py).
 """
 from __future__ import annotations
 
@@ -59,6 +64,13 @@ def parse():
     p.add_argument("--cfl", type=float, default=0.3)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--cpu-n", type=int, default=160)
+    p.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble comparison")
+    p.add_argument("--ensemble-cases", type=int, default=64)
+    p.add_argument("--ensemble-steps", type=int, default=100)
+    p.add_argument("--no-serving", action="store_true", help="skip the served-stream comparison")
+    p.add_argument("--serving-requests", type=int, default=48)
+    p.add_argument("--serving-load", type=float, default=0.7,
+                   help="offered load as a fraction of the batched ensemble throughput")
     return p.parse_args()
 
 
@@ -83,6 +95,49 @@ def cpu_baseline(n: int):
             "sample": f"one LSRK3 step (3 stages + max-SOS) of a {n}^3 bubble-cloud block, "
                       f"oracle (restated reference scheme, binary32, OpenMP) after one warm-up step; "
                       f"{el:.2f} s"}
+
+
+def ensemble_leg(a):
+    """64 bubble-collapse cases (edges {32..128}), 100 steps each: batched
+    (one launch per kernel kind, graph-captured steps, autotuned march
+    length) against the plain one-case-at-a-time solver."""
+    import ensemble as E
+    cases = E.ensemble_cases(a.ensemble_cases, a.ensemble_steps, seed=0)
+    work = sum(c.work for c in cases)
+    tb, mb, _ = E.run_batched(cases, cfl=a.cfl, fast=True, graph=True, autotune=True)
+    log(f"ensemble batched: {mb:.2f} s")
+    tp, mp, _ = E.run_plain(cases, cfl=a.cfl, fast=True)
+    log(f"ensemble plain: {mp:.2f} s")
+    return {
+        "workload": f"{len(cases)} bubble-collapse cases, edges in {{32,48,64,96,128}}, {a.ensemble_steps} LSRK3 steps each, FP32",
+        "cell_updates": work,
+        "batched": dict(cell_updates_per_s=work / mb, makespan_s=mb, **E.latency_stats(list(tb.values()))),
+        "plain": dict(cell_updates_per_s=work / mp, makespan_s=mp, **E.latency_stats(list(tp.values()))),
+    }
+
+
+def serving_leg(a, batched_rate):
+    """Poisson request stream (edges 32..192 step 16, 50..500 steps) offered at
+    a fixed rate, served with continuous batching and by the plain sequential
+    solver; time-to-solution = completion - arrival."""
+    import ensemble as E
+    probe = E.stream_cases(a.serving_requests, seed=1, rate=1.0)
+    mean_work = sum(c.work for c in probe) / len(probe)
+    rate = a.serving_load * batched_rate / mean_work
+    reqs = E.stream_cases(a.serving_requests, seed=1, rate=rate)
+    work = sum(c.work for c in reqs)
+    ts, ms, info = E.serve(reqs, cfl=a.cfl, fast=True, graph=True)
+    log(f"serving continuous batching: {ms:.2f} s")
+    tp, mp = E.serve_plain(reqs, cfl=a.cfl, fast=True)
+    log(f"serving plain: {mp:.2f} s")
+    return {
+        "workload": f"{len(reqs)} requests, Poisson at {rate:.2f}/s ({a.serving_load:.0%} of the batched ensemble "
+                    f"throughput), edges 32..192 step 16, 50..500 steps, bubble collapse, FP32",
+        "rate_per_s": rate,
+        "continuous_batching": dict(cell_updates_per_s=work / ms, makespan_s=ms, **E.latency_stats(list(ts.values())),
+                                    **info),
+        "plain_fifo": dict(cell_updates_per_s=work / mp, makespan_s=mp, **E.latency_stats(list(tp.values()))),
+    }
 
 
 def log(*msg):
@@ -209,6 +264,10 @@ def main():
         "last_sos": sos,
     }
     log(f"timed region {wall:.3f} s; {value:.3e} cell-updates/s")
+    if not a.no_ensemble:
+        out["ensemble"] = ensemble_leg(a)
+        if not a.no_serving:
+            out["serving"] = serving_leg(a, out["ensemble"]["batched"]["cell_updates_per_s"])
     if not a.no_cpu:
         log(f"CPU baseline: oracle on a {a.cpu_n}^3 sample")
         out["cpu_baseline"] = cpu_baseline(a.cpu_n)

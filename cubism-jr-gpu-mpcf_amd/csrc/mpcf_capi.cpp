@@ -56,16 +56,24 @@ struct Launchers
     decltype(&mpcf::parity::launch_pack) pack;
     decltype(&mpcf::parity::launch_test_weno) test_weno;
     decltype(&mpcf::parity::launch_test_face) test_face;
+    decltype(&mpcf::parity::batch_sos) b_sos;
+    decltype(&mpcf::parity::batch_dt) b_dt;
+    decltype(&mpcf::parity::batch_ghosts) b_ghosts;
+    decltype(&mpcf::parity::batch_xy) b_xy;
+    decltype(&mpcf::parity::batch_z) b_z;
+    decltype(&mpcf::parity::batch_flip) b_flip;
 };
 
 const Launchers kParity = {
     mpcf::parity::launch_ghosts, mpcf::parity::launch_sos, mpcf::parity::launch_dt, mpcf::parity::launch_sweep,
     mpcf::parity::launch_update, mpcf::parity::launch_pack, mpcf::parity::launch_test_weno,
-    mpcf::parity::launch_test_face};
+    mpcf::parity::launch_test_face, mpcf::parity::batch_sos, mpcf::parity::batch_dt, mpcf::parity::batch_ghosts,
+    mpcf::parity::batch_xy, mpcf::parity::batch_z, mpcf::parity::batch_flip};
 const Launchers kFast = {
     mpcf::fast::launch_ghosts, mpcf::fast::launch_sos, mpcf::fast::launch_dt, mpcf::fast::launch_sweep,
     mpcf::fast::launch_update, mpcf::fast::launch_pack, mpcf::fast::launch_test_weno,
-    mpcf::fast::launch_test_face};
+    mpcf::fast::launch_test_face, mpcf::fast::batch_sos, mpcf::fast::batch_dt, mpcf::fast::batch_ghosts,
+    mpcf::fast::batch_xy, mpcf::fast::batch_z, mpcf::fast::batch_flip};
 
 // query helper used by mpcf_query: which x/y variant runs
 // LSRK3 coefficients (LSRK3_IntegratorMPI.cpp:46,51,56), passed as binary32
@@ -455,4 +463,409 @@ extern "C" int mpcf_test_face(int n, int dir, const float* st, float* out, unsig
     LAUNCH_CHECK("test_face");
     HIP_TRY(hipMemcpy(out, o.p, sizeof(float) * 12 * (size_t)n, hipMemcpyDeviceToHost));
     return MPCF_OK;
+}
+
+// ===========================================================================
+// Batched ensembles (mpcf_batch_*): many independent cases of uneven sizes
+// advanced together, one launch per kernel kind per stage, per-case dt on the
+// device, a whole step optionally captured once as a hipGraph and replayed.
+// Each case is a full context (its own resident block); the batch owns the
+// device tables that the batch kernels read.
+// ===========================================================================
+#include <algorithm>
+#include <map>
+
+struct mpcf_batch
+{
+    unsigned flags;
+    const Launchers* L;
+    hipStream_t stream;
+    std::vector<mpcf_ctx*> cases;
+    std::vector<int> ids;
+    std::vector<int> chunk_xy, chunk_z;      // per case; 0 = default
+    int next_id = 1;
+    // device tables
+    mpcf::CaseDesc* d_cases = nullptr;
+    int* d_cur = nullptr;
+    int* d_off = nullptr;                    // 4 x (cap + 1)
+    int cap = 0;
+    bool dirty = true;
+    bool cur_on_device = false;              // d_cur holds the truth (after steps)
+    int tot_xy = 0, tot_z = 0, tot_ghost = 0, tot_sos = 0;
+    mpcf::BatchDev dev{};
+    // graph of one step
+    bool use_graph = false;
+    bool graph_valid = false;
+    double g_cfl = 0, g_dtmax = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    long long captures = 0;
+    long long steps = 0;
+    // autotuned march length by case extent
+    std::map<int, int> tuned_len;
+};
+
+namespace {
+
+void batch_drop_graph(mpcf_batch* b)
+{
+    if (b->exec) hipGraphExecDestroy(b->exec);
+    if (b->graph) hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_valid = false;
+}
+
+// bring the device-side buffer indices back into the contexts
+int batch_pull_cur(mpcf_batch* b)
+{
+    if (!b->cur_on_device || b->cases.empty()) return MPCF_OK;
+    std::vector<int> cur(b->cases.size());
+    HIP_TRY(hipMemcpyAsync(cur.data(), b->d_cur, cur.size() * sizeof(int), hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (size_t i = 0; i < cur.size(); ++i) b->cases[i]->cur = cur[i];
+    b->cur_on_device = false;
+    return MPCF_OK;
+}
+
+int march_len(const mpcf_batch* b, int i, int nd, int which)
+{
+    const int forced = which == 0 ? b->chunk_xy[i] : b->chunk_z[i];
+    if (forced > 0) return std::min(forced, nd);
+    auto it = b->tuned_len.find(nd);
+    if (it != b->tuned_len.end()) return std::min(it->second, nd);
+    return std::min(nd, 64);     // enough parallelism comes from the other cases
+}
+
+int batch_rebuild(mpcf_batch* b)
+{
+    const int rc = batch_pull_cur(b);
+    if (rc != MPCF_OK) return rc;
+    const int n = (int)b->cases.size();
+    if (n > b->cap) {
+        if (b->d_cases) hipFree(b->d_cases);
+        if (b->d_cur) hipFree(b->d_cur);
+        if (b->d_off) hipFree(b->d_off);
+        b->d_cases = nullptr; b->d_cur = nullptr; b->d_off = nullptr;
+        const int cap = std::max(16, 2 * n);
+        hipError_t e = hipMalloc((void**)&b->d_cases, cap * sizeof(mpcf::CaseDesc));
+        if (e == hipSuccess) e = hipMalloc((void**)&b->d_cur, cap * sizeof(int));
+        if (e == hipSuccess) e = hipMalloc((void**)&b->d_off, 4 * (cap + 1) * sizeof(int));
+        if (e != hipSuccess) return fail(MPCF_ENOMEM, std::string("batch tables: ") + hipGetErrorString(e));
+        b->cap = cap;
+    }
+    std::vector<mpcf::CaseDesc> desc(n);
+    std::vector<int> cur(n), off(4 * (b->cap + 1), 0);
+    int* oxy = &off[0];
+    int* oz = &off[b->cap + 1];
+    int* og = &off[2 * (b->cap + 1)];
+    int* os = &off[3 * (b->cap + 1)];
+    for (int i = 0; i < n; ++i) {
+        mpcf_ctx* c = b->cases[i];
+        mpcf::CaseDesc& d = desc[i];
+        d.g = c->g;
+        d.u[0] = c->u[0]; d.u[1] = c->u[1]; d.R = c->R; d.acc = c->acc; d.st = c->st; d.h = c->h;
+        for (int f = 0; f < 6; ++f) d.bc[f] = c->bc[f];
+        d.chunk_xy = march_len(b, i, c->g.ny, 0);
+        d.chunk_z = march_len(b, i, c->g.nz, 1);
+        int dims[3];
+        mpcf::parity::xy_grid(c->g, d.chunk_xy, dims);
+        d.nbx_xy = dims[0]; d.nby_xy = dims[1];
+        oxy[i + 1] = oxy[i] + dims[0] * dims[1] * dims[2];
+        mpcf::parity::z_grid(c->g, d.chunk_z, dims);
+        d.nbx_z = dims[0]; d.nby_z = dims[1];
+        oz[i + 1] = oz[i] + dims[0] * dims[1] * dims[2];
+        int tmax = std::max(c->g.ny * c->g.nz, std::max(c->g.nx * c->g.nz, c->g.nx * c->g.ny));
+        d.nb_ghost = (tmax + 255) / 256;
+        og[i + 1] = og[i] + 6 * d.nb_ghost;
+        d.nb_sos = std::min(2048, (c->g.ny * c->g.nz + 3) / 4);
+        os[i + 1] = os[i] + d.nb_sos;
+        cur[i] = c->cur;
+    }
+    if (n > 0) {
+        HIP_TRY(hipMemcpyAsync(b->d_cases, desc.data(), n * sizeof(mpcf::CaseDesc), hipMemcpyHostToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(b->d_cur, cur.data(), n * sizeof(int), hipMemcpyHostToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(b->d_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
+        HIP_TRY(hipStreamSynchronize(b->stream));
+    }
+    b->tot_xy = oxy[n]; b->tot_z = oz[n]; b->tot_ghost = og[n]; b->tot_sos = os[n];
+    b->dev.cases = b->d_cases; b->dev.cur = b->d_cur; b->dev.n = n;
+    b->dev.off_xy = b->d_off; b->dev.off_z = b->d_off + (b->cap + 1);
+    b->dev.off_ghost = b->d_off + 2 * (b->cap + 1); b->dev.off_sos = b->d_off + 3 * (b->cap + 1);
+    b->dirty = false;
+    batch_drop_graph(b);
+    return MPCF_OK;
+}
+
+void batch_enqueue_step(mpcf_batch* b, double cfl, double dt_max)
+{
+    const int cpu_form = (b->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0;
+    b->L->b_sos(b->dev, b->tot_sos, cpu_form, b->stream);
+    b->L->b_dt(b->dev, cfl, dt_max, b->stream);
+    for (int s = 0; s < 3; ++s) {
+        b->L->b_ghosts(b->dev, b->tot_ghost, b->stream);
+        b->L->b_xy(b->dev, b->tot_xy, (float)kA[s], (float)kB[s], b->stream);
+        b->L->b_z(b->dev, b->tot_z, (float)kA[s], (float)kB[s], b->stream);
+        b->L->b_flip(b->dev, b->stream);
+    }
+}
+
+int find_case_index(const mpcf_batch* b, int id)
+{
+    for (size_t i = 0; i < b->ids.size(); ++i)
+        if (b->ids[i] == id) return (int)i;
+    return -1;
+}
+
+} // namespace
+
+extern "C" int mpcf_case_bytes(int nx, int ny, int nz, long long* bytes)
+{
+    if (!bytes || nx < 5 || ny < 5 || nz < 5) return fail(MPCF_EINVAL, "mpcf_case_bytes: bad argument");
+    const Geom g = mpcf::make_geom(nx, ny, nz);
+    *bytes = (long long)((7 + 7 + 7 + mpcf::NACC) * (size_t)g.fstride * sizeof(float) + 256);
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_batch_create(mpcf_batch** out, unsigned flags)
+{
+    if (!out) return fail(MPCF_EINVAL, "mpcf_batch_create: out is NULL");
+    *out = nullptr;
+    if (flags & ~(unsigned)(MPCF_FLAG_FAST | MPCF_FLAG_SOS_CPU)) return fail(MPCF_EINVAL, "mpcf_batch_create: unknown flag");
+    mpcf_batch* b = new mpcf_batch();
+    b->flags = flags;
+    b->L = (flags & MPCF_FLAG_FAST) ? &kFast : &kParity;
+    hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) { delete b; return fail(MPCF_EHIP, std::string("mpcf_batch_create: ") + hipGetErrorString(e)); }
+    *out = b;
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_batch_destroy(mpcf_batch* b)
+{
+    if (!b) return MPCF_OK;
+    hipStreamSynchronize(b->stream);
+    batch_drop_graph(b);
+    for (mpcf_ctx* c : b->cases) mpcf_destroy(c);
+    if (b->d_cases) hipFree(b->d_cases);
+    if (b->d_cur) hipFree(b->d_cur);
+    if (b->d_off) hipFree(b->d_off);
+    hipStreamDestroy(b->stream);
+    delete b;
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_batch_add(mpcf_batch* b, int nx, int ny, int nz, double h, const int bc[6],
+                              const float* const fields[7], int* case_id)
+{
+    if (!b || !fields || !case_id) return fail(MPCF_EINVAL, "mpcf_batch_add: NULL argument");
+    int rc = batch_pull_cur(b);
+    if (rc != MPCF_OK) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    mpcf_ctx* c = nullptr;
+    rc = mpcf_create(&c, nx, ny, nz, h, b->flags & (MPCF_FLAG_FAST | MPCF_FLAG_SOS_CPU));
+    if (rc != MPCF_OK) return rc;
+    if (bc) {
+        rc = mpcf_set_bc(c, bc);
+        if (rc != MPCF_OK) { mpcf_destroy(c); return rc; }
+    }
+    rc = mpcf_upload(c, MPCF_DATA, fields);
+    if (rc != MPCF_OK) { mpcf_destroy(c); return rc; }
+    b->cases.push_back(c);
+    b->ids.push_back(b->next_id);
+    b->chunk_xy.push_back(0);
+    b->chunk_z.push_back(0);
+    *case_id = b->next_id++;
+    b->dirty = true;
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_batch_remove(mpcf_batch* b, int case_id)
+{
+    if (!b) return fail(MPCF_EINVAL, "mpcf_batch_remove: NULL batch");
+    const int i = find_case_index(b, case_id);
+    if (i < 0) return fail(MPCF_EINVAL, "mpcf_batch_remove: unknown case id");
+    int rc = batch_pull_cur(b);
+    if (rc != MPCF_OK) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    mpcf_destroy(b->cases[i]);
+    b->cases.erase(b->cases.begin() + i);
+    b->ids.erase(b->ids.begin() + i);
+    b->chunk_xy.erase(b->chunk_xy.begin() + i);
+    b->chunk_z.erase(b->chunk_z.begin() + i);
+    b->dirty = true;
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_batch_download(mpcf_batch* b, int case_id, int which, float* const fields[7])
+{
+    if (!b) return fail(MPCF_EINVAL, "mpcf_batch_download: NULL batch");
+    const int i = find_case_index(b, case_id);
+    if (i < 0) return fail(MPCF_EINVAL, "mpcf_batch_download: unknown case id");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    const int rc = batch_pull_cur(b);
+    if (rc != MPCF_OK) return rc;
+    return mpcf_download(b->cases[i], which, fields);
+}
+
+extern "C" int mpcf_batch_set_march(mpcf_batch* b, int case_id, int len_xy, int len_z)
+{
+    if (!b || len_xy < 0 || len_z < 0) return fail(MPCF_EINVAL, "mpcf_batch_set_march: bad argument");
+    const int i = find_case_index(b, case_id);
+    if (i < 0) return fail(MPCF_EINVAL, "mpcf_batch_set_march: unknown case id");
+    b->chunk_xy[i] = len_xy;
+    b->chunk_z[i] = len_z;
+    b->dirty = true;
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_batch_use_graph(mpcf_batch* b, int enable)
+{
+    if (!b) return fail(MPCF_EINVAL, "mpcf_batch_use_graph: NULL batch");
+    b->use_graph = enable != 0;
+    if (!b->use_graph) batch_drop_graph(b);
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_batch_step(mpcf_batch* b, double cfl, double dt_max)
+{
+    if (!b) return fail(MPCF_EINVAL, "mpcf_batch_step: NULL batch");
+    if (!(cfl > 0) || !std::isfinite(cfl) || !(dt_max > 0)) return fail(MPCF_EINVAL, "mpcf_batch_step: bad cfl/dt_max");
+    if (b->dirty) {
+        const int rc = batch_rebuild(b);
+        if (rc != MPCF_OK) return rc;
+    }
+    if (b->cases.empty()) return MPCF_OK;
+    if (b->use_graph) {
+        if (!b->graph_valid || b->g_cfl != cfl || b->g_dtmax != dt_max) {
+            batch_drop_graph(b);
+            HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+            batch_enqueue_step(b, cfl, dt_max);
+            hipGraph_t g = nullptr;
+            HIP_TRY(hipStreamEndCapture(b->stream, &g));
+            b->graph = g;
+            HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+            b->graph_valid = true;
+            b->g_cfl = cfl;
+            b->g_dtmax = dt_max;
+            ++b->captures;
+        }
+        HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+    } else {
+        batch_enqueue_step(b, cfl, dt_max);
+        LAUNCH_CHECK("batch step");
+    }
+    b->cur_on_device = true;
+    ++b->steps;
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_batch_last_dt(mpcf_batch* b, int case_id, double* dt, float* sos)
+{
+    if (!b) return fail(MPCF_EINVAL, "mpcf_batch_last_dt: NULL batch");
+    const int i = find_case_index(b, case_id);
+    if (i < 0) return fail(MPCF_EINVAL, "mpcf_batch_last_dt: unknown case id");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return mpcf_last_dt(b->cases[i], dt, sos);
+}
+
+extern "C" int mpcf_batch_synchronize(mpcf_batch* b)
+{
+    if (!b) return fail(MPCF_EINVAL, "mpcf_batch_synchronize: NULL batch");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return MPCF_OK;
+}
+
+extern "C" void* mpcf_batch_get_stream(mpcf_batch* b) { return b ? (void*)b->stream : nullptr; }
+
+extern "C" int mpcf_batch_query(mpcf_batch* b, long long out[6])
+{
+    if (!b || !out) return fail(MPCF_EINVAL, "mpcf_batch_query: NULL argument");
+    long long cells = 0, bytes = 0;
+    for (mpcf_ctx* c : b->cases) {
+        cells += (long long)c->g.nx * c->g.ny * c->g.nz;
+        bytes += (long long)c->bytes;
+    }
+    out[0] = (long long)b->cases.size();
+    out[1] = cells;
+    out[2] = bytes;
+    out[3] = b->captures;
+    out[4] = b->steps;
+    out[5] = b->tuned_len.size();
+    return MPCF_OK;
+}
+
+// Autotuner of the march length (how many cells one thread marches in the
+// x+y and z sweeps before handing over to the next chunk): for each candidate
+// length, time `steps` batch steps of the live cases from a saved state and
+// keep the fastest; the state is restored afterwards, so tuning does not
+// advance the cases.  Candidates: 16, 32, 64, 128 and whole columns.
+extern "C" int mpcf_batch_autotune(mpcf_batch* b, int steps, double cfl, int* best_len)
+{
+    if (!b || steps < 1 || !(cfl > 0)) return fail(MPCF_EINVAL, "mpcf_batch_autotune: bad argument");
+    if (b->cases.empty()) return MPCF_OK;
+    int rc = batch_pull_cur(b);
+    if (rc != MPCF_OK) return rc;
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    // save u (current buffer) and R of every case
+    std::vector<void*> save;
+    for (mpcf_ctx* c : b->cases) {
+        void* p = nullptr;
+        const size_t nb = 14 * (size_t)c->g.fstride * sizeof(float);
+        hipError_t e = hipMalloc(&p, nb);
+        if (e != hipSuccess) {
+            for (void* q : save) hipFree(q);
+            return fail(MPCF_ENOMEM, "mpcf_batch_autotune: no room to save the state");
+        }
+        HIP_TRY(hipMemcpyAsync(p, c->u[c->cur], 7 * c->g.fstride * sizeof(float), hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync((float*)p + 7 * c->g.fstride, c->R, 7 * c->g.fstride * sizeof(float),
+                               hipMemcpyDeviceToDevice, b->stream));
+        save.push_back(p);
+    }
+    std::vector<int> cur0(b->cases.size());
+    for (size_t i = 0; i < b->cases.size(); ++i) cur0[i] = b->cases[i]->cur;
+    const int cand[] = {16, 32, 64, 128, 1 << 20};
+    const bool graph = b->use_graph;
+    b->use_graph = false;
+    float best = 1e30f;
+    int bestl = 64;
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    for (int l : cand) {
+        b->tuned_len.clear();
+        for (mpcf_ctx* c : b->cases) { b->tuned_len[c->g.ny] = l; b->tuned_len[c->g.nz] = l; }
+        b->dirty = true;
+        rc = mpcf_batch_step(b, cfl, 1e30);        // warm-up with this shape
+        if (rc == MPCF_OK) {
+            hipEventRecord(e0, b->stream);
+            for (int k = 0; k < steps && rc == MPCF_OK; ++k) rc = mpcf_batch_step(b, cfl, 1e30);
+            hipEventRecord(e1, b->stream);
+            hipEventSynchronize(e1);
+            float ms = 0;
+            hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best) { best = ms; bestl = l; }
+        }
+        // restore
+        rc = batch_pull_cur(b);
+        for (size_t i = 0; i < b->cases.size(); ++i) {
+            mpcf_ctx* c = b->cases[i];
+            c->cur = cur0[i];
+            hipMemcpyAsync(c->u[c->cur], save[i], 7 * c->g.fstride * sizeof(float), hipMemcpyDeviceToDevice, b->stream);
+            hipMemcpyAsync(c->R, (float*)save[i] + 7 * c->g.fstride, 7 * c->g.fstride * sizeof(float),
+                           hipMemcpyDeviceToDevice, b->stream);
+        }
+        hipStreamSynchronize(b->stream);
+        if (rc != MPCF_OK) break;
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    for (void* q : save) hipFree(q);
+    b->tuned_len.clear();
+    for (mpcf_ctx* c : b->cases) { b->tuned_len[c->g.ny] = bestl; b->tuned_len[c->g.nz] = bestl; }
+    b->dirty = true;
+    b->use_graph = graph;
+    b->steps = 0;
+    if (best_len) *best_len = bestl;
+    return rc;
 }

@@ -144,19 +144,22 @@ __device__ __forceinline__ Real stage_dtinvh(const StageArgs& A)
 }
 
 // ---------------------------------------------------------------------------
-// y / z sweeps: block (64, 4); lanes along x, threadIdx.y + blockIdx.y*4 along
-// the remaining transverse axis, blockIdx.z = chunk of the marched axis.
+// y / z sweeps: block of 256 lanes over the flattened (x, transverse) columns,
+// blockIdx.y = chunk of the marched axis.
 // ---------------------------------------------------------------------------
 template <int D, int MODE>
-__global__ __launch_bounds__(256)
-void k_sweep_yz(const Geom g, const StageArgs A, const int chunk)
+__device__ __forceinline__ void sweep_yz_body(const Geom& g, const StageArgs& A, const int chunk,
+                                              const int bcol, const int bz)
 {
-    const int x = blockIdx.x * 64 + threadIdx.x;
-    const int t = blockIdx.y * 4 + threadIdx.y;
+    // columns (x, t) flattened x-fastest, 256 per block: consecutive lanes are
+    // consecutive x (coalesced) and no lane idles when nx is not a multiple of 64
     const int nt = D == 1 ? g.nz : g.ny;
     const int nd = D == 1 ? g.ny : g.nz;
-    if (x >= g.nx || t >= nt) return;   // no barriers below
-    const int c0 = blockIdx.z * chunk;
+    const int col = bcol * 256 + threadIdx.x;
+    if (col >= g.nx * nt) return;   // no barriers below
+    const int x = col % g.nx;
+    const int t = col / g.nx;
+    const int c0 = bz * chunk;
     const int c1 = min(c0 + chunk, nd);
     const int s = g.stride(D);
     const int e0 = D == 1 ? g.idx(x, 0, t) : g.idx(x, t, 0);
@@ -201,6 +204,13 @@ void k_sweep_yz(const Geom g, const StageArgs A, const int chunk)
         m.step(A, dtinvh, k, c0, e0 + (k - 1) * s, ein);
         ein = ein_next;
     }
+}
+
+template <int D, int MODE>
+__global__ __launch_bounds__(256)
+void k_sweep_yz(const Geom g, const StageArgs A, const int chunk)
+{
+    sweep_yz_body<D, MODE>(g, A, chunk, blockIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -351,16 +361,23 @@ __device__ __forceinline__ Prim prim_from_right(const Prim& q)
     return o;
 }
 
-__global__ __launch_bounds__(256)
-void k_sweep_xy(const Geom g, const StageArgs A, const int chunk)
+__device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A, const int chunk,
+                                              const int bw, const int bz)
 {
+    // positions q along the padded rows (x = -3 .. nx+2) of all z-planes,
+    // flattened; wave w covers q = 58 w .. 58 w + 63 and stores lanes 3..60,
+    // so consecutive waves overlap by six positions and ghost positions are
+    // computed but never stored
     const int lane = threadIdx.x;
-    const int z = blockIdx.y * 4 + threadIdx.y;
-    if (z >= g.nz) return;                        // uniform per wave; no barriers below
-    const int x = blockIdx.x * XY_TILE + lane - 3;
-    const int xc = min(x, g.nx + 2);              // lanes past the band read its last cell
-    const bool store = lane >= 3 && lane <= 60 && x < g.nx;
-    const int c0 = blockIdx.z * chunk;
+    const int rowlen = g.nx + 6;
+    const int q = (bw * 4 + threadIdx.y) * XY_TILE + lane;
+    int z = q / rowlen;
+    const int x = q - z * rowlen - 3;
+    const bool store = lane >= 3 && lane <= 60 && x >= 0 && x < g.nx && z < g.nz;
+    if ((bw * 4 + threadIdx.y) * XY_TILE >= rowlen * g.nz) return;   // whole wave past the end
+    z = min(z, g.nz - 1);                         // lanes past the last plane read its last row
+    const int xc = x;
+    const int c0 = bz * chunk;
     const int c1 = min(c0 + chunk, g.ny);
     const int s = g.px;
     const int e0 = g.idx(xc, 0, z);
@@ -413,24 +430,29 @@ void k_sweep_xy(const Geom g, const StageArgs A, const int chunk)
     }
 }
 
+__global__ __launch_bounds__(256)
+void k_sweep_xy(const Geom g, const StageArgs A, const int chunk)
+{
+    sweep_xy_body(g, A, chunk, blockIdx.x, blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------
 // ghost fill (BoundaryConditions.h:60-109 applied to the padded band).
 // blockIdx.y = face = 2*dir + side.  kind: 0 absorbing, 1 reflecting.
 // ---------------------------------------------------------------------------
 struct BCSet { int kind[6]; };
 
-__global__ __launch_bounds__(256)
-void k_ghosts(const Geom g, Real* __restrict__ u, const int64_t fs, const BCSet bc)
+__device__ __forceinline__ void ghosts_body(const Geom& g, Real* __restrict__ u, const int64_t fs, const int* bc,
+                                            const int bx, const int face)
 {
-    const int face = blockIdx.y;
     const int dir = face >> 1, side = face & 1;
     const int na = dir == 0 ? g.ny : g.nx;
     const int nb = dir == 2 ? g.ny : g.nz;
-    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int t = bx * 256 + threadIdx.x;
     if (t >= na * nb) return;
     const int ta = t % na, tb = t / na;
     const int n = g.n(dir);
-    const int kind = bc.kind[face];
+    const int kind = bc[face];
 #pragma unroll
     for (int gl = 0; gl < 3; ++gl) {
         const int dst = side == 0 ? gl - 3 : n + gl;
@@ -447,6 +469,12 @@ void k_ghosts(const Geom g, Real* __restrict__ u, const int64_t fs, const BCSet 
     }
 }
 
+__global__ __launch_bounds__(256)
+void k_ghosts(const Geom g, Real* __restrict__ u, const int64_t fs, const BCSet bc)
+{
+    ghosts_body(g, u, fs, bc.kind, blockIdx.x, blockIdx.y);
+}
+
 // ---------------------------------------------------------------------------
 // max speed of sound: reset, reduce, finalize dt (GPUkernels.cu:1182-1222,
 // MaxSpeedOfSound.cpp:18-55, LSRK3_IntegratorMPI.cpp:37-38)
@@ -454,13 +482,12 @@ void k_ghosts(const Geom g, Real* __restrict__ u, const int64_t fs, const BCSet 
 __global__ void k_sos_reset(StepState* st) { st->sos_bits = 0u; }
 
 template <int CPU_FORM>
-__global__ __launch_bounds__(256)
-void k_sos(const Geom g, const Real* __restrict__ u, const int64_t fs, StepState* st)
+__device__ __forceinline__ void sos_body(const Geom& g, const Real* __restrict__ u, const int64_t fs, StepState* st,
+                                         const int b, const int nb, Real* wmax)
 {
-    __shared__ Real wmax[4];
     const int rows = g.ny * g.nz;
     Real sos = 0.0f;
-    for (int row = blockIdx.x * 4 + threadIdx.y; row < rows; row += gridDim.x * 4) {
+    for (int row = b * 4 + threadIdx.y; row < rows; row += nb * 4) {
         const int e0 = g.idx(0, row % g.ny, row / g.ny);
         for (int x = threadIdx.x; x < g.nx; x += 64) {
             const int e = e0 + x;
@@ -483,7 +510,15 @@ void k_sos(const Geom g, const Real* __restrict__ u, const int64_t fs, StepState
     }
 }
 
-__global__ void k_dt_finalize(StepState* st, const double cfl, const double h, const double dt_max)
+template <int CPU_FORM>
+__global__ __launch_bounds__(256)
+void k_sos(const Geom g, const Real* __restrict__ u, const int64_t fs, StepState* st)
+{
+    __shared__ Real wmax[4];
+    sos_body<CPU_FORM>(g, u, fs, st, blockIdx.x, gridDim.x, wmax);
+}
+
+__device__ __forceinline__ void dt_body(StepState* st, const double cfl, const double h, const double dt_max)
 {
     const float sos = __uint_as_float(st->sos_bits);
     st->sos = sos;
@@ -492,6 +527,11 @@ __global__ void k_dt_finalize(StepState* st, const double cfl, const double h, c
     st->dt = dt;
     st->h = h; st->cfl = cfl; st->dt_max = dt_max;
     st->dtinvh = (float)(dt/h);
+}
+
+__global__ void k_dt_finalize(StepState* st, const double cfl, const double h, const double dt_max)
+{
+    dt_body(st, cfl, h, dt_max);
 }
 
 // ---------------------------------------------------------------------------
@@ -561,6 +601,89 @@ __global__ void k_test_face(const int n, const Real* __restrict__ st, Real* __re
 }
 
 // ---------------------------------------------------------------------------
+// batched ensemble: one launch per kernel kind advances every live case.  A
+// flat block index is mapped to (case, local block) by a binary search over
+// the per-case prefix offsets; the local block runs the same body as the
+// single-case kernel, so every case gets exactly the arithmetic it gets
+// alone.  The cases' current-buffer indices live on the device and are
+// flipped after each stage, so a whole step can be captured in a graph.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int find_case(const int* __restrict__ off, const int n, const int b)
+{
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= b) lo = mid; else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ StageArgs case_args(const CaseDesc& C, const int cur, const Real a, const Real b)
+{
+    StageArgs A;
+    A.u = C.u[cur]; A.R = C.R; A.acc = C.acc; A.uo = C.u[1 - cur]; A.fs = C.g.fstride;
+    A.a = a; A.b = b; A.dtinvh = 0.0f; A.st = C.st;
+    return A;
+}
+
+__global__ void kb_sos_reset(const BatchDev B)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < B.n) B.cases[c].st->sos_bits = 0u;
+}
+
+template <int CPU_FORM>
+__global__ __launch_bounds__(256)
+void kb_sos(const BatchDev B)
+{
+    __shared__ Real wmax[4];
+    const int c = find_case(B.off_sos, B.n, blockIdx.x);
+    const CaseDesc& C = B.cases[c];
+    sos_body<CPU_FORM>(C.g, C.u[B.cur[c]], C.g.fstride, C.st, blockIdx.x - B.off_sos[c], C.nb_sos, wmax);
+}
+
+__global__ void kb_dt(const BatchDev B, const double cfl, const double dt_max)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < B.n) dt_body(B.cases[c].st, cfl, B.cases[c].h, dt_max);
+}
+
+__global__ __launch_bounds__(256)
+void kb_ghosts(const BatchDev B)
+{
+    const int c = find_case(B.off_ghost, B.n, blockIdx.x);
+    const CaseDesc& C = B.cases[c];
+    const int local = blockIdx.x - B.off_ghost[c];
+    ghosts_body(C.g, C.u[B.cur[c]], C.g.fstride, C.bc, local % C.nb_ghost, local / C.nb_ghost);
+}
+
+__global__ __launch_bounds__(256)
+void kb_sweep_xy(const BatchDev B, const Real a, const Real b)
+{
+    const int c = find_case(B.off_xy, B.n, blockIdx.x);
+    const CaseDesc& C = B.cases[c];
+    const int local = blockIdx.x - B.off_xy[c];
+    const StageArgs A = case_args(C, B.cur[c], a, b);
+    sweep_xy_body(C.g, A, C.chunk_xy, local % C.nbx_xy, local / C.nbx_xy);
+}
+
+__global__ __launch_bounds__(256)
+void kb_sweep_z(const BatchDev B, const Real a, const Real b)
+{
+    const int c = find_case(B.off_z, B.n, blockIdx.x);
+    const CaseDesc& C = B.cases[c];
+    const int local = blockIdx.x - B.off_z[c];
+    const StageArgs A = case_args(C, B.cur[c], a, b);
+    sweep_yz_body<2, MODE_STAGE>(C.g, A, C.chunk_z, local % C.nbx_z, local / C.nbx_z);
+}
+
+__global__ void kb_flip(const BatchDev B)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < B.n) B.cur[c] = 1 - B.cur[c];
+}
+
+// ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
 static int pick_chunk(const int nd, const int64_t columns, const int waves_per_col_group)
@@ -616,8 +739,9 @@ void launch_sweep(const Geom& g, const int dir, const SweepParams& p, const int 
     A.a = p.a; A.b = p.b; A.dtinvh = p.dtinvh; A.st = p.st;
     if (dir == 3) {
         const int chunk = pick_chunk(g.ny, (int64_t)g.nx * g.nz, 1);
-        dim3 grid((g.nx + XY_TILE - 1) / XY_TILE, (g.nz + 3) / 4, (g.ny + chunk - 1) / chunk);
-        hipLaunchKernelGGL(k_sweep_xy, grid, dim3(64, 4), 0, s, g, A, chunk);
+        int dims[3];
+        xy_grid(g, chunk, dims);
+        hipLaunchKernelGGL(k_sweep_xy, dim3(dims[0], dims[2]), dim3(64, 4), 0, s, g, A, chunk);
     } else if (dir == 0) {
         const int npen = g.ny * g.nz;
         const int chunk = pick_chunk(g.nx, npen, 1);
@@ -625,13 +749,15 @@ void launch_sweep(const Geom& g, const int dir, const SweepParams& p, const int 
         hipLaunchKernelGGL(k_sweep_x, grid, dim3(64), 0, s, g, A, chunk);
     } else if (dir == 1) {
         const int chunk = pick_chunk(g.ny, (int64_t)g.nx * g.nz, 1);
-        dim3 grid((g.nx + 63) / 64, (g.nz + 3) / 4, (g.ny + chunk - 1) / chunk);
-        hipLaunchKernelGGL((k_sweep_yz<1, MODE_ACC>), grid, dim3(64, 4), 0, s, g, A, chunk);
+        dim3 grid((g.nx * g.nz + 255) / 256, (g.ny + chunk - 1) / chunk);
+        hipLaunchKernelGGL((k_sweep_yz<1, MODE_ACC>), grid, dim3(256), 0, s, g, A, chunk);
     } else {
         const int chunk = pick_chunk(g.nz, (int64_t)g.nx * g.ny, 1);
-        dim3 grid((g.nx + 63) / 64, (g.ny + 3) / 4, (g.nz + chunk - 1) / chunk);
-        if (stage) hipLaunchKernelGGL((k_sweep_yz<2, MODE_STAGE>), grid, dim3(64, 4), 0, s, g, A, chunk);
-        else       hipLaunchKernelGGL((k_sweep_yz<2, MODE_CONV>), grid, dim3(64, 4), 0, s, g, A, chunk);
+        int dims[3];
+        z_grid(g, chunk, dims);
+        dim3 grid(dims[0], dims[2]);
+        if (stage) hipLaunchKernelGGL((k_sweep_yz<2, MODE_STAGE>), grid, dim3(256), 0, s, g, A, chunk);
+        else       hipLaunchKernelGGL((k_sweep_yz<2, MODE_CONV>), grid, dim3(256), 0, s, g, A, chunk);
     }
 }
 
@@ -657,6 +783,54 @@ void launch_test_face(const int n, const int dir, const Real* st, Real* out, hip
     if (dir == 0)      hipLaunchKernelGGL(k_test_face<0>, grid, block, 0, s, n, st, out);
     else if (dir == 1) hipLaunchKernelGGL(k_test_face<1>, grid, block, 0, s, n, st, out);
     else               hipLaunchKernelGGL(k_test_face<2>, grid, block, 0, s, n, st, out);
+}
+
+int default_chunk(const int nd, const long long columns) { return pick_chunk(nd, columns, 1); }
+
+// grid of the fused x+y sweep: dims[0] blocks of four waves over the padded
+// rows of all planes, dims[2] y-chunks (dims[1] = 1)
+void xy_grid(const Geom& g, const int chunk, int dims[3])
+{
+    const int waves = ((g.nx + 6) * g.nz + XY_TILE - 1) / XY_TILE;
+    dims[0] = (waves + 3) / 4; dims[1] = 1; dims[2] = (g.ny + chunk - 1) / chunk;
+}
+
+// grid of the z sweep: dims[0] blocks of 256 flattened (x, y) columns, dims[2] z-chunks
+void z_grid(const Geom& g, const int chunk, int dims[3])
+{
+    dims[0] = (g.nx * g.ny + 255) / 256; dims[1] = 1; dims[2] = (g.nz + chunk - 1) / chunk;
+}
+
+void batch_sos(const BatchDev& B, const int blocks, const int cpu_form, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_sos_reset, dim3((B.n + 63) / 64), dim3(64), 0, s, B);
+    if (cpu_form) hipLaunchKernelGGL(kb_sos<1>, dim3(blocks), dim3(64, 4), 0, s, B);
+    else          hipLaunchKernelGGL(kb_sos<0>, dim3(blocks), dim3(64, 4), 0, s, B);
+}
+
+void batch_dt(const BatchDev& B, const double cfl, const double dt_max, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_dt, dim3((B.n + 63) / 64), dim3(64), 0, s, B, cfl, dt_max);
+}
+
+void batch_ghosts(const BatchDev& B, const int blocks, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_ghosts, dim3(blocks), dim3(256), 0, s, B);
+}
+
+void batch_xy(const BatchDev& B, const int blocks, const float a, const float b, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_sweep_xy, dim3(blocks), dim3(64, 4), 0, s, B, a, b);
+}
+
+void batch_z(const BatchDev& B, const int blocks, const float a, const float b, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_sweep_z, dim3(blocks), dim3(256), 0, s, B, a, b);
+}
+
+void batch_flip(const BatchDev& B, hipStream_t s)
+{
+    hipLaunchKernelGGL(kb_flip, dim3((B.n + 63) / 64), dim3(64), 0, s, B);
 }
 
 } // namespace MPCF_NS

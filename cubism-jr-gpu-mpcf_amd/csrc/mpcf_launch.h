@@ -16,6 +16,37 @@ struct SweepParams
     float a, b, dtinvh;    // stage coefficients; dtinvh ignored when st != NULL
     const StepState* st;   // device step state carrying dt/h
 };
+
+// one case of a batched ensemble, as the batch kernels see it
+struct CaseDesc
+{
+    Geom g;
+    float* u[2];           // stage input / output buffers (which is input: BatchDev::cur)
+    float* R;
+    float* acc;
+    StepState* st;
+    double h;
+    int bc[6];
+    int chunk_xy, chunk_z; // march lengths of the x+y and z sweeps
+    int nbx_xy, nby_xy;    // x+y sweep grid: x tiles, z groups (y chunks = third dim)
+    int nbx_z, nby_z;      // z sweep grid: x blocks, y groups (z chunks = third dim)
+    int nb_ghost;          // ghost-fill blocks per face
+    int nb_sos;            // speed-of-sound blocks
+};
+
+// device tables of a batch: per-case descriptors, the current-buffer index of
+// each case (flipped on the device after every stage) and, per kernel kind,
+// the prefix offsets of the cases' blocks in the flat launch grid
+struct BatchDev
+{
+    const CaseDesc* cases;
+    int* cur;
+    int n;
+    const int* off_xy;
+    const int* off_z;
+    const int* off_ghost;
+    const int* off_sos;
+};
 }
 
 #define MPCF_DECLARE_LAUNCHERS(NS)                                                                         \
@@ -28,6 +59,15 @@ void launch_update(const Geom& g, float* u, const float* R, float b, hipStream_t
 void launch_pack(const Geom& g, float* pad, float* dense, int to_padded, int nq, hipStream_t s);           \
 void launch_test_weno(int n, const float* in5, float* m, float* p, hipStream_t s);                         \
 void launch_test_face(int n, int dir, const float* st, float* out, hipStream_t s);                         \
+void batch_sos(const BatchDev& B, int blocks, int cpu_form, hipStream_t s);                                \
+void batch_dt(const BatchDev& B, double cfl, double dt_max, hipStream_t s);                                \
+void batch_ghosts(const BatchDev& B, int blocks, hipStream_t s);                                           \
+void batch_xy(const BatchDev& B, int blocks, float a, float b, hipStream_t s);                             \
+void batch_z(const BatchDev& B, int blocks, float a, float b, hipStream_t s);                              \
+void batch_flip(const BatchDev& B, hipStream_t s);                                                         \
+void xy_grid(const Geom& g, int chunk, int dims[3]);                                                       \
+void z_grid(const Geom& g, int chunk, int dims[3]);                                                        \
+int default_chunk(int nd, long long columns);                                                              \
 } }
 
 MPCF_DECLARE_LAUNCHERS(parity)

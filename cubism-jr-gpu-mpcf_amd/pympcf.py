@@ -33,7 +33,10 @@ EXPORTS = [
     "mpcf_last_dt", "mpcf_synchronize", "mpcf_query", "mpcf_test_weno", "mpcf_test_face",
     "mpcf_timing_enable", "mpcf_timing_read", "mpcf_ic_shock_tube", "mpcf_ic_shock_bubble",
     "mpcf_ic_bubble_cloud", "mpcf_ic_bubble_collapse", "mpcf_flop_count", "mpcf_host_weno",
-    "mpcf_host_face",
+    "mpcf_host_face", "mpcf_case_bytes", "mpcf_batch_create", "mpcf_batch_destroy", "mpcf_batch_add",
+    "mpcf_batch_remove", "mpcf_batch_download", "mpcf_batch_step", "mpcf_batch_use_graph",
+    "mpcf_batch_last_dt", "mpcf_batch_synchronize", "mpcf_batch_get_stream", "mpcf_batch_set_march",
+    "mpcf_batch_autotune", "mpcf_batch_query",
 ]
 
 # kernel classes of mpcf_timing_read
@@ -87,8 +90,23 @@ def lib():
         L.mpcf_ic_shock_bubble.argtypes = [I, I, I, U64, P]
         L.mpcf_ic_bubble_cloud.argtypes = [I, I, I, I, U64, P, P]
         L.mpcf_ic_bubble_collapse.argtypes = [I, I, I, D, D, D, D, D, P]
+        L.mpcf_case_bytes.argtypes = [I, I, I, P]
+        L.mpcf_batch_create.argtypes = [C.POINTER(P), C.c_uint]
+        L.mpcf_batch_destroy.argtypes = [P]
+        L.mpcf_batch_add.argtypes = [P, I, I, I, D, P, P, P]
+        L.mpcf_batch_remove.argtypes = [P, I]
+        L.mpcf_batch_download.argtypes = [P, I, I, P]
+        L.mpcf_batch_step.argtypes = [P, D, D]
+        L.mpcf_batch_use_graph.argtypes = [P, I]
+        L.mpcf_batch_last_dt.argtypes = [P, I, C.POINTER(D), C.POINTER(F)]
+        L.mpcf_batch_synchronize.argtypes = [P]
+        L.mpcf_batch_get_stream.argtypes = [P]
+        L.mpcf_batch_get_stream.restype = P
+        L.mpcf_batch_set_march.argtypes = [P, I, I, I]
+        L.mpcf_batch_autotune.argtypes = [P, I, D, P]
+        L.mpcf_batch_query.argtypes = [P, P]
         for name in EXPORTS:
-            if name not in ("mpcf_last_error", "mpcf_version", "mpcf_get_stream"):
+            if name not in ("mpcf_last_error", "mpcf_version", "mpcf_get_stream", "mpcf_batch_get_stream"):
                 getattr(L, name).restype = I
         _LIB = L
     return _LIB
@@ -294,3 +312,85 @@ def ic_bubble_collapse(nx, ny, nz, radius, pratio, centre=(0.5, 0.5, 0.5)):
     f = _new_fields(nx, ny, nz)
     _check(lib().mpcf_ic_bubble_collapse(nx, ny, nz, float(radius), float(pratio), *map(float, centre), _fields(f)))
     return f
+
+
+def case_bytes(nx, ny, nz):
+    b = C.c_longlong()
+    _check(lib().mpcf_case_bytes(nx, ny, nz, C.cast(C.byref(b), C.c_void_p)))
+    return b.value
+
+
+class Batch:
+    """A batched ensemble (mpcf_batch): cases of uneven sizes advanced together."""
+
+    def __init__(self, fast=False, sos_cpu=False, graph=False):
+        h_ = C.c_void_p()
+        _check(lib().mpcf_batch_create(C.byref(h_), (FLAG_FAST if fast else 0) | (FLAG_SOS_CPU if sos_cpu else 0)))
+        self._h = h_
+        self.shapes = {}
+        if graph:
+            self.use_graph(True)
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().mpcf_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add(self, fields, h=None, bc=None):
+        a = np.ascontiguousarray(fields, dtype=np.float32)
+        nz, ny, nx = a.shape[1:]
+        h = 1.0 / max(nx, ny, nz) if h is None else h
+        cid = C.c_int()
+        bcp = None
+        if bc is not None:
+            bca = np.ascontiguousarray(bc, dtype=np.int32)
+            bcp = _ptr(bca)
+        _check(lib().mpcf_batch_add(self._h, nx, ny, nz, float(h), bcp, _fields(a), C.cast(C.byref(cid), C.c_void_p)))
+        self.shapes[cid.value] = (nz, ny, nx)
+        return cid.value
+
+    def remove(self, cid):
+        _check(lib().mpcf_batch_remove(self._h, cid))
+        self.shapes.pop(cid, None)
+
+    def download(self, cid, which=DATA):
+        out = np.empty((NVAR,) + self.shapes[cid], dtype=np.float32)
+        _check(lib().mpcf_batch_download(self._h, cid, which, _fields(out)))
+        return out
+
+    def step(self, cfl, dt_max=1e30):
+        _check(lib().mpcf_batch_step(self._h, float(cfl), float(dt_max)))
+
+    def use_graph(self, enable=True):
+        _check(lib().mpcf_batch_use_graph(self._h, 1 if enable else 0))
+
+    def last_dt(self, cid):
+        d, s = C.c_double(), C.c_float()
+        _check(lib().mpcf_batch_last_dt(self._h, cid, C.byref(d), C.byref(s)))
+        return d.value, s.value
+
+    def synchronize(self):
+        _check(lib().mpcf_batch_synchronize(self._h))
+
+    def set_march(self, cid, len_xy, len_z):
+        _check(lib().mpcf_batch_set_march(self._h, cid, len_xy, len_z))
+
+    def autotune(self, steps=2, cfl=0.3):
+        best = C.c_int()
+        _check(lib().mpcf_batch_autotune(self._h, steps, float(cfl), C.cast(C.byref(best), C.c_void_p)))
+        return best.value
+
+    def query(self):
+        out = (C.c_longlong * 6)()
+        _check(lib().mpcf_batch_query(self._h, C.cast(out, C.c_void_p)))
+        keys = ("cases", "cells", "bytes", "captures", "steps", "tuned")
+        return dict(zip(keys, out))

@@ -159,6 +159,46 @@ int mpcf_ic_bubble_cloud(int nx, int ny, int nz, int nbubbles, unsigned long lon
 int mpcf_ic_bubble_collapse(int nx, int ny, int nz, double radius, double pratio, double cx, double cy,
                             double cz, float* const fields[7]);
 
+/* ---------------------------------------------------------------------------
+ * Batched ensembles.  No counterpart in the reference (which advances one
+ * node block per process); this is the batching part of the MI355X build.
+ * A batch holds independent cases of uneven sizes (each a full resident block
+ * with its own dt).  mpcf_batch_step advances every live case by one LSRK3
+ * step with one launch per kernel kind per stage (the cases' blocks share
+ * each grid), entirely on the device; with mpcf_batch_use_graph the step is
+ * captured once as a hipGraph and replayed (re-captured when the case set,
+ * the march lengths, cfl or dt_max change).  Cases may be added and removed
+ * between steps (continuous batching).  Each case gets the same arithmetic
+ * as a context of its own, so its results are identical alone and in a batch.
+ * --------------------------------------------------------------------------- */
+typedef struct mpcf_batch mpcf_batch;
+
+/* HBM bytes one case of this size holds (admission by footprint). */
+int mpcf_case_bytes(int nx, int ny, int nz, long long* bytes);
+
+int mpcf_batch_create(mpcf_batch** out, unsigned flags);   /* MPCF_FLAG_FAST / _SOS_CPU */
+int mpcf_batch_destroy(mpcf_batch* batch);
+/* Add a case (u uploaded, R zeroed; bc may be NULL = all absorbing); returns its id. */
+int mpcf_batch_add(mpcf_batch* batch, int nx, int ny, int nz, double h, const int bc[6],
+                   const float* const fields[7], int* case_id);
+int mpcf_batch_remove(mpcf_batch* batch, int case_id);
+int mpcf_batch_download(mpcf_batch* batch, int case_id, int which, float* const fields[7]);
+/* One LSRK3 step of every live case (asynchronous). */
+int mpcf_batch_step(mpcf_batch* batch, double cfl, double dt_max);
+int mpcf_batch_use_graph(mpcf_batch* batch, int enable);
+/* dt and max speed of sound of a case's last step (synchronizes). */
+int mpcf_batch_last_dt(mpcf_batch* batch, int case_id, double* dt, float* sos);
+int mpcf_batch_synchronize(mpcf_batch* batch);
+void* mpcf_batch_get_stream(mpcf_batch* batch);
+/* Force the march lengths of one case (0 = tuned / default). */
+int mpcf_batch_set_march(mpcf_batch* batch, int case_id, int len_xy, int len_z);
+/* Autotune the march length on the live cases: times `steps` steps per
+ * candidate from a saved state and restores it (the cases do not advance). */
+int mpcf_batch_autotune(mpcf_batch* batch, int steps, double cfl, int* best_len);
+/* out[0] live cases, [1] interior cells, [2] device bytes, [3] graph captures,
+ * [4] steps since the last autotune, [5] tuned extents. */
+int mpcf_batch_query(mpcf_batch* batch, long long out[6]);
+
 /* Operation count of the scheme as implemented here, from a host run of the
  * same per-cell code with a counting scalar (binary32/64 add, sub, mul, div,
  * sqrt = 1 FLOP each; min/max and conversions listed apart):

@@ -1,0 +1,120 @@
+"""Batched ensembles: every case gives the same result alone (its own
+context) and in a batch -- bit for bit, since the batch kernels run the same
+per-block bodies -- with eager launches, with graph replay (checked after every
+replay, with eager work and synchronisations in between), across additions
+and removals between steps (continuous batching), and after autotuning."""
+import sys
+
+import numpy as np
+import pytest
+
+import states
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-batch]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def cases(mpcf):
+    out = [
+        (states.random_smooth(20, 20, 20, seed=1), [0] * 6),
+        (mpcf.ic_shock_tube(24, 16, 20, 0, 1e-3, 2), [0, 0, 1, 1, 0, 0]),
+        (states.random_smooth(33, 17, 9, seed=3), [1] * 6),
+        (mpcf.ic_bubble_collapse(48, 48, 48, 0.2, 100.0), [0] * 6),
+        (mpcf.ic_shock_bubble(16, 16, 16, 4), [1, 0, 0, 1, 1, 0]),
+        (states.random_smooth(70, 12, 10, seed=6), [0] * 6),
+    ]
+    return out
+
+
+def alone(mpcf, s, bc, steps, fast=False):
+    nz, ny, nx = s.shape[1:]
+    with mpcf.Block(nx, ny, nz, 1.0 / max(nx, ny, nz), fast=fast, bc=bc, split_xy=False) as b:
+        b.data = s
+        dts = [b.step(0.3) for _ in range(steps)]
+        return b.data, b.tmp, dts
+
+
+@pytest.mark.parametrize("fast", [False, True], ids=["bitexact", "fast"])
+@pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
+def test_batch_equals_alone(mpcf, fast, graph):
+    cs = cases(mpcf)
+    steps = 3
+    say("batch", len(cs), "cases fast", fast, "graph", graph)
+    with mpcf.Batch(fast=fast, graph=graph) as B:
+        ids = [B.add(s, bc=bc) for s, bc in cs]
+        per_step = []
+        for k in range(steps):
+            B.step(0.3)
+            # eager work and a synchronisation between replays
+            per_step.append([B.last_dt(i)[0] for i in ids])
+            _ = B.download(ids[k % len(ids)])
+        q = B.query()
+        if graph:
+            assert q["captures"] == 1, q
+        got = [(B.download(i), B.download(i, mpcf.TMP)) for i in ids]
+    for j, (s, bc) in enumerate(cs):
+        u, R, dts = alone(mpcf, s, bc, steps, fast=fast)
+        assert [per_step[k][j] for k in range(steps)] == dts, j
+        assert np.array_equal(got[j][0], u), j
+        assert np.array_equal(got[j][1], R), j
+
+
+def test_graph_replays_match_eager_each_step(mpcf):
+    cs = cases(mpcf)[:4]
+    say("graph replay vs eager, step by step")
+    with mpcf.Batch(graph=True) as G, mpcf.Batch(graph=False) as E:
+        gi = [G.add(s, bc=bc) for s, bc in cs]
+        ei = [E.add(s, bc=bc) for s, bc in cs]
+        for k in range(4):
+            G.step(0.3)
+            E.step(0.3)
+            for a, b in zip(gi, ei):
+                assert np.array_equal(G.download(a), E.download(b)), (k, a)
+
+
+def test_continuous_batching_add_remove(mpcf):
+    """Cases join and leave between steps; each finishes with exactly the
+    state it has when run alone for its own number of steps."""
+    cs = cases(mpcf)
+    plan = [(0, 0, 4), (1, 0, 2), (2, 1, 3), (3, 2, 2), (4, 2, 4), (5, 3, 1)]   # (case, join step, steps)
+    say("continuous batching")
+    done = {}
+    with mpcf.Batch(fast=True, graph=True) as B:
+        live = {}
+        for t in range(8):
+            for c, join, n in plan:
+                if join == t:
+                    live[c] = [B.add(cs[c][0], bc=cs[c][1]), n]
+            if not live:
+                break
+            B.step(0.3)
+            for c in list(live):
+                live[c][1] -= 1
+                if live[c][1] == 0:
+                    done[c] = B.download(live[c][0])
+                    B.remove(live[c][0])
+                    del live[c]
+    assert sorted(done) == [c for c, _, _ in plan]
+    for c, _, n in plan:
+        u, _, _ = alone(mpcf, cs[c][0], cs[c][1], n, fast=True)
+        assert np.array_equal(done[c], u), c
+
+
+def test_autotune_keeps_state_and_results(mpcf):
+    cs = cases(mpcf)
+    say("autotune")
+    with mpcf.Batch(fast=True) as B:
+        ids = [B.add(s, bc=bc) for s, bc in cs]
+        best = B.autotune(steps=1)
+        assert best in (16, 32, 64, 128, 1 << 20)
+        for i, (s, _) in zip(ids, cs):
+            assert np.array_equal(B.download(i), s)     # tuning did not advance the cases
+        B.step(0.3)
+        got = [B.download(i) for i in ids]
+    for j, (s, bc) in enumerate(cs):
+        u, _, _ = alone(mpcf, s, bc, 1, fast=True)
+        assert np.array_equal(got[j], u), j
