@@ -5,6 +5,9 @@
 Objects:
   parity.o  mpcf_kernels.hip, MPCF_FAST=0, -ffp-contract=off  (bit-exact scheme)
   fast.o    mpcf_kernels.hip, MPCF_FAST=1, -ffp-contract=fast (FMA-contracted)
+  fastp.o   mpcf_kernels.hip, fast + MPCF_P64=1 (binary64 pressure recovery and reductions)
+  f64.o     mpcf_kernels.hip, MPCF_F64=1, -ffp-contract=off (the scheme in binary64)
+  reduce.o  mpcf_reduce.hip  (binary64 block comparisons and integrals)
   capi.o    the C-ABI (include/mpcf_hip.h)
   ic.o      initial conditions;  flops.o  host instantiations of the scheme
 and the C++ driver mpcf_run (csrc/mpcf_run.cpp over csrc/mpcf.hpp).
@@ -58,7 +61,11 @@ def build(verbose: bool = False) -> str:
     units = [
         ("parity.o", "mpcf_kernels.hip", ["-DMPCF_FAST=0", "-ffp-contract=off"]),
         ("fast.o", "mpcf_kernels.hip", ["-DMPCF_FAST=1", "-ffp-contract=fast"]),
+        ("fastp.o", "mpcf_kernels.hip", ["-DMPCF_FAST=1", "-DMPCF_P64=1", "-ffp-contract=fast"]),
+        ("f64.o", "mpcf_kernels.hip", ["-DMPCF_FAST=0", "-DMPCF_F64=1", "-ffp-contract=off"]),
+        ("reduce.o", "mpcf_reduce.hip", ["-ffp-contract=off"]),
         ("capi.o", "mpcf_capi.cpp", ["-x", "hip"]),
+        ("serve.o", "mpcf_serve.cpp", ["-x", "hip"]),
         ("ic.o", "ic.cpp", ["-x", "hip"]),
         ("flops.o", "flops.cpp", ["-x", "hip", "-ffp-contract=off"]),
     ]
@@ -68,7 +75,7 @@ def build(verbose: bool = False) -> str:
         if not os.path.exists(path):
             continue
         cmds.append([cc, *common_flags(), *extra, "-c", path, "-o", os.path.join(BUILD, obj)])
-    with cf.ThreadPoolExecutor(max_workers=min(4, len(cmds))) as ex:
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(cmds))) as ex:
         for r, cmd in zip(ex.map(_run, cmds), cmds):
             if verbose:
                 print(" ".join(cmd))

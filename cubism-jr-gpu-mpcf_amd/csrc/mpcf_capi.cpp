@@ -17,14 +17,18 @@
 #include <vector>
 
 #include "mpcf_hip.h"
-#include "mpcf_launch.h"
+#include "mpcf_internal.h"
+#include "mpcf_reduce.h"
 
 using mpcf::Geom;
 using mpcf::StepState;
 
-namespace {
+namespace mpcf {
+namespace detail {
 
+namespace {
 thread_local std::string g_err;
+}
 
 int fail(int code, const std::string& msg)
 {
@@ -32,81 +36,10 @@ int fail(int code, const std::string& msg)
     return code;
 }
 
-#define HIP_TRY(call)                                                                   \
-    do {                                                                                \
-        hipError_t e_ = (call);                                                         \
-        if (e_ != hipSuccess)                                                           \
-            return fail(MPCF_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
-    } while (0)
+const char* last_error() { return g_err.c_str(); }
 
-#define LAUNCH_CHECK(what)                                                                \
-    do {                                                                                  \
-        hipError_t e_ = hipGetLastError();                                                \
-        if (e_ != hipSuccess)                                                             \
-            return fail(MPCF_EHIP, std::string(what) + " launch: " + hipGetErrorString(e_)); \
-    } while (0)
-
-struct Launchers
-{
-    decltype(&mpcf::parity::launch_ghosts) ghosts;
-    decltype(&mpcf::parity::launch_sos) sos;
-    decltype(&mpcf::parity::launch_dt) dt;
-    decltype(&mpcf::parity::launch_sweep) sweep;
-    decltype(&mpcf::parity::launch_update) update;
-    decltype(&mpcf::parity::launch_pack) pack;
-    decltype(&mpcf::parity::launch_test_weno) test_weno;
-    decltype(&mpcf::parity::launch_test_face) test_face;
-    decltype(&mpcf::parity::batch_sos) b_sos;
-    decltype(&mpcf::parity::batch_dt) b_dt;
-    decltype(&mpcf::parity::batch_ghosts) b_ghosts;
-    decltype(&mpcf::parity::batch_xy) b_xy;
-    decltype(&mpcf::parity::batch_z) b_z;
-    decltype(&mpcf::parity::batch_flip) b_flip;
-};
-
-const Launchers kParity = {
-    mpcf::parity::launch_ghosts, mpcf::parity::launch_sos, mpcf::parity::launch_dt, mpcf::parity::launch_sweep,
-    mpcf::parity::launch_update, mpcf::parity::launch_pack, mpcf::parity::launch_test_weno,
-    mpcf::parity::launch_test_face, mpcf::parity::batch_sos, mpcf::parity::batch_dt, mpcf::parity::batch_ghosts,
-    mpcf::parity::batch_xy, mpcf::parity::batch_z, mpcf::parity::batch_flip};
-const Launchers kFast = {
-    mpcf::fast::launch_ghosts, mpcf::fast::launch_sos, mpcf::fast::launch_dt, mpcf::fast::launch_sweep,
-    mpcf::fast::launch_update, mpcf::fast::launch_pack, mpcf::fast::launch_test_weno,
-    mpcf::fast::launch_test_face, mpcf::fast::batch_sos, mpcf::fast::batch_dt, mpcf::fast::batch_ghosts,
-    mpcf::fast::batch_xy, mpcf::fast::batch_z, mpcf::fast::batch_flip};
-
-// query helper used by mpcf_query: which x/y variant runs
-// LSRK3 coefficients (LSRK3_IntegratorMPI.cpp:46,51,56), passed as binary32
-// like GPUlab::process_all's Real arguments (GPUlab.h:258)
-const double kA[3] = {0.0, -17. / 32, -32. / 27};
-const double kB[3] = {1. / 4, 8. / 9, 3. / 4};
-
-} // namespace
-
-struct mpcf_ctx
-{
-    Geom g;
-    double h;
-    unsigned flags;
-    int bc[6];
-    int device;
-    float* base;        // single allocation
-    float* u[2];
-    int cur;
-    float* R;
-    float* acc;
-    StepState* st;      // device
-    StepState* st_host; // pinned host mirror
-    hipStream_t stream;
-    bool own_stream;
-    size_t bytes;
-    const Launchers* L;
-    // optional per-kernel event timing (mpcf_timing_enable)
-    bool timing;
-    std::vector<hipEvent_t> ev_pool;
-    struct Mark { int kind; hipEvent_t b, e; };
-    std::vector<Mark> marks;
-};
+}  // namespace detail
+}  // namespace mpcf
 
 namespace {
 
@@ -153,7 +86,7 @@ struct Timed
 };
 
 // R <- a R - dtinvh L(u) (stage == 0), or that and u_out <- u + b R (stage == 1)
-void run_rhs(mpcf_ctx* c, const float* u, float* uo, float a, float b, float dtinvh, const StepState* st, int stage)
+void run_rhs(mpcf_ctx* c, const void* u, void* uo, double a, double b, double dtinvh, const StepState* st, int stage)
 {
     mpcf::SweepParams p;
     p.u = u; p.R = c->R; p.acc = c->acc; p.uo = uo;
@@ -175,7 +108,7 @@ void run_rhs(mpcf_ctx* c, const float* u, float* uo, float a, float b, float dti
 
 } // namespace
 
-extern "C" const char* mpcf_last_error(void) { return g_err.c_str(); }
+extern "C" const char* mpcf_last_error(void) { return mpcf::detail::last_error(); }
 
 extern "C" const char* mpcf_version(void) { return "mpcf-mi355x 0.1 (gfx950)"; }
 
@@ -185,10 +118,12 @@ extern "C" int mpcf_create(mpcf_ctx** out, int nx, int ny, int nz, double h, uns
     *out = nullptr;
     if (nx < 5 || ny < 5 || nz < 5) return fail(MPCF_EINVAL, "mpcf_create: extents must be >= 5");
     if (!(h > 0) || !std::isfinite(h)) return fail(MPCF_EINVAL, "mpcf_create: h must be positive and finite");
-    if (flags & ~(unsigned)(MPCF_FLAG_FAST | MPCF_FLAG_SOS_CPU | MPCF_FLAG_SPLIT_XY | MPCF_FLAG_FUSED_XY))
-        return fail(MPCF_EINVAL, "mpcf_create: unknown flag");
+    if (flags & ~kAllFlags) return fail(MPCF_EINVAL, "mpcf_create: unknown flag");
     if ((flags & MPCF_FLAG_SPLIT_XY) && (flags & MPCF_FLAG_FUSED_XY))
         return fail(MPCF_EINVAL, "mpcf_create: SPLIT_XY and FUSED_XY are exclusive");
+    if ((flags & MPCF_FLAG_FP64) && (flags & (MPCF_FLAG_FAST | MPCF_FLAG_P64)))
+        return fail(MPCF_EINVAL, "mpcf_create: FP64 excludes FAST and P64");
+    if (flags & MPCF_FLAG_P64) flags |= MPCF_FLAG_FAST;   // P64 is the FAST arithmetic plus binary64 steps
     const Geom g = mpcf::make_geom(nx, ny, nz);
     if ((int64_t)g.px * g.py * g.pz >= (int64_t(1) << 31))
         return fail(MPCF_EINVAL, "mpcf_create: block too large for 32-bit element indices");
@@ -196,12 +131,13 @@ extern "C" int mpcf_create(mpcf_ctx** out, int nx, int ny, int nz, double h, uns
     mpcf_ctx* c = new mpcf_ctx();
     c->g = g; c->h = h; c->flags = flags; c->cur = 0;
     for (int i = 0; i < 6; ++i) c->bc[i] = MPCF_BC_ABSORBING;
-    c->L = (flags & MPCF_FLAG_FAST) ? &kFast : &kParity;
+    c->L = pick_launchers(flags);
+    c->esize = elem_bytes(flags);
     hipError_t e = hipGetDevice(&c->device);
     if (e != hipSuccess) { delete c; return fail(MPCF_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e)); }
 
     const size_t nfields = 7 + 7 + 7 + mpcf::NACC;
-    c->bytes = nfields * (size_t)g.fstride * sizeof(float) + 256;
+    c->bytes = nfields * (size_t)g.fstride * c->esize + 256;
     e = hipMalloc((void**)&c->base, c->bytes);
     if (e != hipSuccess) {
         delete c;
@@ -216,11 +152,13 @@ extern "C" int mpcf_create(mpcf_ctx** out, int nx, int ny, int nz, double h, uns
         return fail(MPCF_EHIP, std::string("mpcf_create: ") + hipGetErrorString(e));
     }
     c->own_stream = true;
+    const size_t fb = (size_t)g.fstride * c->esize;   // bytes per field (a multiple of 256)
     c->u[0] = c->base;
-    c->u[1] = c->base + 7 * g.fstride;
-    c->R = c->base + 14 * g.fstride;
-    c->acc = c->base + 21 * g.fstride;
-    c->st = (StepState*)(c->base + nfields * (size_t)g.fstride);
+    c->u[1] = c->base + 7 * fb;
+    c->R = c->base + 14 * fb;
+    c->acc = c->base + 21 * fb;
+    c->st = (StepState*)(c->base + nfields * fb);
+    static_assert(sizeof(StepState) <= 256, "step state must fit the reserved tail");
     std::memset(c->st_host, 0, sizeof(StepState));
     *out = c;
     return MPCF_OK;
@@ -266,36 +204,58 @@ extern "C" int mpcf_set_bc(mpcf_ctx* c, const int bc[6])
     return MPCF_OK;
 }
 
+namespace {
+
+// host fields <-> the padded device layout, one field at a time through the
+// accumulator area (free between stages), converting between the host type
+// (binary32 or binary64) and the storage type on the device
+int transfer(mpcf_ctx* c, const int which, void* const* fields, const int host_f64, const bool to_device,
+             const char* what)
+{
+    if (!c || !fields) return fail(MPCF_EINVAL, std::string(what) + ": NULL argument");
+    if (which != MPCF_DATA && which != MPCF_TMP) return fail(MPCF_EINVAL, std::string(what) + ": which must be DATA or TMP");
+    for (int q = 0; q < 7; ++q)
+        if (!fields[q]) return fail(MPCF_EINVAL, std::string(what) + ": NULL field pointer");
+    const size_t n = (size_t)c->g.nx * c->g.ny * c->g.nz;
+    const size_t hb = host_f64 ? sizeof(double) : sizeof(float);
+    const size_t fb = (size_t)c->g.fstride * c->esize;
+    char* blk = (char*)(which == MPCF_DATA ? c->u[c->cur] : c->R);
+    for (int q = 0; q < 7; ++q) {
+        void* dq = blk + q * fb;
+        if (to_device) {
+            HIP_TRY(hipMemcpyAsync(c->acc, fields[q], n * hb, hipMemcpyHostToDevice, c->stream));
+            c->L->pack(c->g, dq, c->acc, 1, 1, host_f64, c->stream);
+            LAUNCH_CHECK("pack");
+        } else {
+            c->L->pack(c->g, dq, c->acc, 0, 1, host_f64, c->stream);
+            LAUNCH_CHECK("unpack");
+            HIP_TRY(hipMemcpyAsync(fields[q], c->acc, n * hb, hipMemcpyDeviceToHost, c->stream));
+        }
+        HIP_TRY(hipStreamSynchronize(c->stream));   // the staging area is reused by the next field
+    }
+    return MPCF_OK;
+}
+
+} // namespace
+
 extern "C" int mpcf_upload(mpcf_ctx* c, int which, const float* const fields[7])
 {
-    if (!c || !fields) return fail(MPCF_EINVAL, "mpcf_upload: NULL argument");
-    if (which != MPCF_DATA && which != MPCF_TMP) return fail(MPCF_EINVAL, "mpcf_upload: which must be DATA or TMP");
-    for (int q = 0; q < 7; ++q)
-        if (!fields[q]) return fail(MPCF_EINVAL, "mpcf_upload: NULL field pointer");
-    const size_t n = (size_t)c->g.nx * c->g.ny * c->g.nz;
-    for (int q = 0; q < 7; ++q)
-        HIP_TRY(hipMemcpyAsync(c->acc + q * n, fields[q], n * sizeof(float), hipMemcpyHostToDevice, c->stream));
-    float* dst = which == MPCF_DATA ? c->u[c->cur] : c->R;
-    c->L->pack(c->g, dst, c->acc, 1, 7, c->stream);
-    LAUNCH_CHECK("pack");
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return MPCF_OK;
+    return transfer(c, which, (void* const*)fields, 0, true, "mpcf_upload");
 }
 
 extern "C" int mpcf_download(mpcf_ctx* c, int which, float* const fields[7])
 {
-    if (!c || !fields) return fail(MPCF_EINVAL, "mpcf_download: NULL argument");
-    if (which != MPCF_DATA && which != MPCF_TMP) return fail(MPCF_EINVAL, "mpcf_download: which must be DATA or TMP");
-    for (int q = 0; q < 7; ++q)
-        if (!fields[q]) return fail(MPCF_EINVAL, "mpcf_download: NULL field pointer");
-    const size_t n = (size_t)c->g.nx * c->g.ny * c->g.nz;
-    float* src = which == MPCF_DATA ? c->u[c->cur] : c->R;
-    c->L->pack(c->g, src, c->acc, 0, 7, c->stream);
-    LAUNCH_CHECK("unpack");
-    for (int q = 0; q < 7; ++q)
-        HIP_TRY(hipMemcpyAsync(fields[q], c->acc + q * n, n * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    HIP_TRY(hipStreamSynchronize(c->stream));
-    return MPCF_OK;
+    return transfer(c, which, (void* const*)fields, 0, false, "mpcf_download");
+}
+
+extern "C" int mpcf_upload_f64(mpcf_ctx* c, int which, const double* const fields[7])
+{
+    return transfer(c, which, (void* const*)fields, 1, true, "mpcf_upload_f64");
+}
+
+extern "C" int mpcf_download_f64(mpcf_ctx* c, int which, double* const fields[7])
+{
+    return transfer(c, which, (void* const*)fields, 1, false, "mpcf_download_f64");
 }
 
 extern "C" int mpcf_load_ghosts(mpcf_ctx* c)
@@ -311,19 +271,26 @@ extern "C" int mpcf_max_sos(mpcf_ctx* c, float* sos)
     if (!c || !sos) return fail(MPCF_EINVAL, "mpcf_max_sos: NULL argument");
     { Timed t(c, K_SOS); c->L->sos(c->g, c->u[c->cur], c->st, (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0, c->stream); }
     LAUNCH_CHECK("sos");
-    unsigned int bits = 0;
-    HIP_TRY(hipMemcpyAsync(&bits, &c->st->sos_bits, sizeof(bits), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipMemcpyAsync(c->st_host, c->st, sizeof(StepState), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    float f;
-    std::memcpy(&f, &bits, sizeof(f));
-    *sos = f;
+    if (c->flags & (MPCF_FLAG_FP64 | MPCF_FLAG_P64)) {
+        double d;
+        const unsigned long long bits = c->st_host->sos_bits_d;
+        std::memcpy(&d, &bits, sizeof(d));
+        *sos = (float)d;
+    } else {
+        float f;
+        const unsigned int bits = c->st_host->sos_bits;
+        std::memcpy(&f, &bits, sizeof(f));
+        *sos = f;
+    }
     return MPCF_OK;
 }
 
 extern "C" int mpcf_convection(mpcf_ctx* c, float a, float dtinvh)
 {
     if (!c) return fail(MPCF_EINVAL, "mpcf_convection: NULL context");
-    run_rhs(c, c->u[c->cur], nullptr, a, 0.0f, dtinvh, nullptr, 0);
+    run_rhs(c, c->u[c->cur], nullptr, a, 0.0, dtinvh, nullptr, 0);
     LAUNCH_CHECK("convection");
     return MPCF_OK;
 }
@@ -355,7 +322,7 @@ extern "C" int mpcf_step(mpcf_ctx* c, double cfl, double dt_max, double* dt)
     LAUNCH_CHECK("sos/dt");
     for (int s = 0; s < 3; ++s) {
         { Timed t(c, K_GHOSTS); c->L->ghosts(c->g, c->u[c->cur], c->bc, c->stream); }
-        run_rhs(c, c->u[c->cur], c->u[1 - c->cur], (float)kA[s], (float)kB[s], 0.0f, c->st, 1);
+        run_rhs(c, c->u[c->cur], c->u[1 - c->cur], kA[s], kB[s], 0.0, c->st, 1);
         LAUNCH_CHECK("stage");
         c->cur = 1 - c->cur;
     }
@@ -378,6 +345,52 @@ extern "C" int mpcf_last_dt(mpcf_ctx* c, double* dt, float* sos)
     HIP_TRY(hipStreamSynchronize(c->stream));
     if (dt) *dt = c->st_host->dt;
     if (sos) *sos = c->st_host->sos;
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_last_sos64(mpcf_ctx* c, double* sos)
+{
+    if (!c || !sos) return fail(MPCF_EINVAL, "mpcf_last_sos64: NULL argument");
+    HIP_TRY(hipMemcpyAsync(c->st_host, c->st, sizeof(StepState), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    *sos = c->st_host->sos_d;
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_compare(mpcf_ctx* a, mpcf_ctx* b, int which, double out[28])
+{
+    if (!a || !b || !out) return fail(MPCF_EINVAL, "mpcf_compare: NULL argument");
+    if (which != MPCF_DATA && which != MPCF_TMP) return fail(MPCF_EINVAL, "mpcf_compare: which must be DATA or TMP");
+    if (a->g.nx != b->g.nx || a->g.ny != b->g.ny || a->g.nz != b->g.nz)
+        return fail(MPCF_EINVAL, "mpcf_compare: blocks differ in extent");
+    if (a->device != b->device) return fail(MPCF_EINVAL, "mpcf_compare: blocks live on different devices");
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    const void* ua = which == MPCF_DATA ? a->u[a->cur] : a->R;
+    const void* ub = which == MPCF_DATA ? b->u[b->cur] : b->R;
+    // the accumulator area of `a` holds the partials; the result follows them
+    double* scratch = (double*)a->acc;
+    double* res = scratch + mpcf::reduce::scratch_doubles(a->g);
+    mpcf::reduce::diff(a->g, ua, a->esize == 8, a->g.fstride, ub, b->esize == 8, b->g.fstride, scratch, res, a->stream);
+    LAUNCH_CHECK("compare");
+    HIP_TRY(hipMemcpyAsync(out, res, 28 * sizeof(double), hipMemcpyDeviceToHost, a->stream));
+    HIP_TRY(hipStreamSynchronize(a->stream));
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_integrals(mpcf_ctx* c, int which, double out[7])
+{
+    if (!c || !out) return fail(MPCF_EINVAL, "mpcf_integrals: NULL argument");
+    if (which != MPCF_DATA && which != MPCF_TMP) return fail(MPCF_EINVAL, "mpcf_integrals: which must be DATA or TMP");
+    const void* u = which == MPCF_DATA ? c->u[c->cur] : c->R;
+    double* scratch = (double*)c->acc;
+    double* res = scratch + mpcf::reduce::scratch_doubles(c->g);
+    mpcf::reduce::sums(c->g, u, c->esize == 8, c->g.fstride, scratch, res, c->stream);
+    LAUNCH_CHECK("integrals");
+    double sums[7];
+    HIP_TRY(hipMemcpyAsync(sums, res, sizeof(sums), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    const double vol = c->h * c->h * c->h;
+    for (int q = 0; q < 7; ++q) out[q] = sums[q] * vol;
     return MPCF_OK;
 }
 
@@ -437,13 +450,14 @@ extern "C" int mpcf_test_weno(int n, const float* in5, float* minus, float* plus
 {
     if (n < 0 || (n > 0 && (!in5 || !minus || !plus))) return fail(MPCF_EINVAL, "mpcf_test_weno: bad argument");
     if (n == 0) return MPCF_OK;
-    const Launchers* L = (flags & MPCF_FLAG_FAST) ? &kFast : &kParity;
+    if (flags & MPCF_FLAG_FP64) return fail(MPCF_EINVAL, "mpcf_test_weno: binary32 arrays only");
+    const Launchers* L = pick_launchers(flags);
     DevBuf a, m, p;
     HIP_TRY(hipMalloc(&a.p, sizeof(float) * 5 * (size_t)n));
     HIP_TRY(hipMalloc(&m.p, sizeof(float) * (size_t)n));
     HIP_TRY(hipMalloc(&p.p, sizeof(float) * (size_t)n));
     HIP_TRY(hipMemcpy(a.p, in5, sizeof(float) * 5 * (size_t)n, hipMemcpyHostToDevice));
-    L->test_weno(n, (const float*)a.p, (float*)m.p, (float*)p.p, nullptr);
+    L->test_weno(n, a.p, m.p, p.p, nullptr);
     LAUNCH_CHECK("test_weno");
     HIP_TRY(hipMemcpy(minus, m.p, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
     HIP_TRY(hipMemcpy(plus, p.p, sizeof(float) * (size_t)n, hipMemcpyDeviceToHost));
@@ -454,12 +468,13 @@ extern "C" int mpcf_test_face(int n, int dir, const float* st, float* out, unsig
 {
     if (n < 0 || dir < 0 || dir > 2 || (n > 0 && (!st || !out))) return fail(MPCF_EINVAL, "mpcf_test_face: bad argument");
     if (n == 0) return MPCF_OK;
-    const Launchers* L = (flags & MPCF_FLAG_FAST) ? &kFast : &kParity;
+    if (flags & MPCF_FLAG_FP64) return fail(MPCF_EINVAL, "mpcf_test_face: binary32 arrays only");
+    const Launchers* L = pick_launchers(flags);
     DevBuf a, o;
     HIP_TRY(hipMalloc(&a.p, sizeof(float) * 42 * (size_t)n));
     HIP_TRY(hipMalloc(&o.p, sizeof(float) * 12 * (size_t)n));
     HIP_TRY(hipMemcpy(a.p, st, sizeof(float) * 42 * (size_t)n, hipMemcpyHostToDevice));
-    L->test_face(n, dir, (const float*)a.p, (float*)o.p, nullptr);
+    L->test_face(n, dir, a.p, o.p, nullptr);
     LAUNCH_CHECK("test_face");
     HIP_TRY(hipMemcpy(out, o.p, sizeof(float) * 12 * (size_t)n, hipMemcpyDeviceToHost));
     return MPCF_OK;
@@ -531,10 +546,8 @@ int batch_pull_cur(mpcf_batch* b)
 int march_len(const mpcf_batch* b, int i, int nd, int which)
 {
     const int forced = which == 0 ? b->chunk_xy[i] : b->chunk_z[i];
-    if (forced > 0) return std::min(forced, nd);
     auto it = b->tuned_len.find(nd);
-    if (it != b->tuned_len.end()) return std::min(it->second, nd);
-    return std::min(nd, 64);     // enough parallelism comes from the other cases
+    return mpcf::detail::batch_march(nd, forced, it != b->tuned_len.end() ? it->second : 0);
 }
 
 int batch_rebuild(mpcf_batch* b)
@@ -562,24 +575,12 @@ int batch_rebuild(mpcf_batch* b)
     int* os = &off[3 * (b->cap + 1)];
     for (int i = 0; i < n; ++i) {
         mpcf_ctx* c = b->cases[i];
-        mpcf::CaseDesc& d = desc[i];
-        d.g = c->g;
-        d.u[0] = c->u[0]; d.u[1] = c->u[1]; d.R = c->R; d.acc = c->acc; d.st = c->st; d.h = c->h;
-        for (int f = 0; f < 6; ++f) d.bc[f] = c->bc[f];
-        d.chunk_xy = march_len(b, i, c->g.ny, 0);
-        d.chunk_z = march_len(b, i, c->g.nz, 1);
-        int dims[3];
-        mpcf::parity::xy_grid(c->g, d.chunk_xy, dims);
-        d.nbx_xy = dims[0]; d.nby_xy = dims[1];
-        oxy[i + 1] = oxy[i] + dims[0] * dims[1] * dims[2];
-        mpcf::parity::z_grid(c->g, d.chunk_z, dims);
-        d.nbx_z = dims[0]; d.nby_z = dims[1];
-        oz[i + 1] = oz[i] + dims[0] * dims[1] * dims[2];
-        int tmax = std::max(c->g.ny * c->g.nz, std::max(c->g.nx * c->g.nz, c->g.nx * c->g.ny));
-        d.nb_ghost = (tmax + 255) / 256;
-        og[i + 1] = og[i] + 6 * d.nb_ghost;
-        d.nb_sos = std::min(2048, (c->g.ny * c->g.nz + 3) / 4);
-        os[i + 1] = os[i] + d.nb_sos;
+        int blocks[4];
+        mpcf::detail::case_desc(c, c->cur, march_len(b, i, c->g.ny, 0), march_len(b, i, c->g.nz, 1), desc[i], blocks);
+        oxy[i + 1] = oxy[i] + blocks[0];
+        oz[i + 1] = oz[i] + blocks[1];
+        og[i + 1] = og[i] + blocks[2];
+        os[i + 1] = os[i] + blocks[3];
         cur[i] = c->cur;
     }
     if (n > 0) {
@@ -604,8 +605,8 @@ void batch_enqueue_step(mpcf_batch* b, double cfl, double dt_max)
     b->L->b_dt(b->dev, cfl, dt_max, b->stream);
     for (int s = 0; s < 3; ++s) {
         b->L->b_ghosts(b->dev, b->tot_ghost, b->stream);
-        b->L->b_xy(b->dev, b->tot_xy, (float)kA[s], (float)kB[s], b->stream);
-        b->L->b_z(b->dev, b->tot_z, (float)kA[s], (float)kB[s], b->stream);
+        b->L->b_xy(b->dev, b->tot_xy, kA[s], kB[s], b->stream);
+        b->L->b_z(b->dev, b->tot_z, kA[s], kB[s], b->stream);
         b->L->b_flip(b->dev, b->stream);
     }
 }
@@ -631,10 +632,14 @@ extern "C" int mpcf_batch_create(mpcf_batch** out, unsigned flags)
 {
     if (!out) return fail(MPCF_EINVAL, "mpcf_batch_create: out is NULL");
     *out = nullptr;
-    if (flags & ~(unsigned)(MPCF_FLAG_FAST | MPCF_FLAG_SOS_CPU)) return fail(MPCF_EINVAL, "mpcf_batch_create: unknown flag");
+    if (flags & ~(unsigned)(MPCF_FLAG_FAST | MPCF_FLAG_SOS_CPU | MPCF_FLAG_P64 | MPCF_FLAG_FP64))
+        return fail(MPCF_EINVAL, "mpcf_batch_create: unknown flag");
+    if ((flags & MPCF_FLAG_FP64) && (flags & (MPCF_FLAG_FAST | MPCF_FLAG_P64)))
+        return fail(MPCF_EINVAL, "mpcf_batch_create: FP64 excludes FAST and P64");
+    if (flags & MPCF_FLAG_P64) flags |= MPCF_FLAG_FAST;
     mpcf_batch* b = new mpcf_batch();
     b->flags = flags;
-    b->L = (flags & MPCF_FLAG_FAST) ? &kFast : &kParity;
+    b->L = pick_launchers(flags);
     hipError_t e = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking);
     if (e != hipSuccess) { delete b; return fail(MPCF_EHIP, std::string("mpcf_batch_create: ") + hipGetErrorString(e)); }
     *out = b;
@@ -663,7 +668,7 @@ extern "C" int mpcf_batch_add(mpcf_batch* b, int nx, int ny, int nz, double h, c
     if (rc != MPCF_OK) return rc;
     HIP_TRY(hipStreamSynchronize(b->stream));
     mpcf_ctx* c = nullptr;
-    rc = mpcf_create(&c, nx, ny, nz, h, b->flags & (MPCF_FLAG_FAST | MPCF_FLAG_SOS_CPU));
+    rc = mpcf_create(&c, nx, ny, nz, h, b->flags);
     if (rc != MPCF_OK) return rc;
     if (bc) {
         rc = mpcf_set_bc(c, bc);
@@ -811,15 +816,14 @@ extern "C" int mpcf_batch_autotune(mpcf_batch* b, int steps, double cfl, int* be
     std::vector<void*> save;
     for (mpcf_ctx* c : b->cases) {
         void* p = nullptr;
-        const size_t nb = 14 * (size_t)c->g.fstride * sizeof(float);
-        hipError_t e = hipMalloc(&p, nb);
+        const size_t reg = 7 * (size_t)c->g.fstride * c->esize;   // one 7-field register
+        hipError_t e = hipMalloc(&p, 2 * reg);
         if (e != hipSuccess) {
             for (void* q : save) hipFree(q);
             return fail(MPCF_ENOMEM, "mpcf_batch_autotune: no room to save the state");
         }
-        HIP_TRY(hipMemcpyAsync(p, c->u[c->cur], 7 * c->g.fstride * sizeof(float), hipMemcpyDeviceToDevice, b->stream));
-        HIP_TRY(hipMemcpyAsync((float*)p + 7 * c->g.fstride, c->R, 7 * c->g.fstride * sizeof(float),
-                               hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(p, c->u[c->cur], reg, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync((char*)p + reg, c->R, reg, hipMemcpyDeviceToDevice, b->stream));
         save.push_back(p);
     }
     std::vector<int> cur0(b->cases.size());
@@ -851,9 +855,9 @@ extern "C" int mpcf_batch_autotune(mpcf_batch* b, int steps, double cfl, int* be
         for (size_t i = 0; i < b->cases.size(); ++i) {
             mpcf_ctx* c = b->cases[i];
             c->cur = cur0[i];
-            hipMemcpyAsync(c->u[c->cur], save[i], 7 * c->g.fstride * sizeof(float), hipMemcpyDeviceToDevice, b->stream);
-            hipMemcpyAsync(c->R, (float*)save[i] + 7 * c->g.fstride, 7 * c->g.fstride * sizeof(float),
-                           hipMemcpyDeviceToDevice, b->stream);
+            const size_t reg = 7 * (size_t)c->g.fstride * c->esize;
+            hipMemcpyAsync(c->u[c->cur], save[i], reg, hipMemcpyDeviceToDevice, b->stream);
+            hipMemcpyAsync(c->R, (char*)save[i] + reg, reg, hipMemcpyDeviceToDevice, b->stream);
         }
         hipStreamSynchronize(b->stream);
         if (rc != MPCF_OK) break;

@@ -62,7 +62,9 @@ inline Geom make_geom(int nx, int ny, int nz)
 }
 
 // device-resident step state: the max speed of sound of the last reduction,
-// the time step derived from it and the binary32 dt/h every stage reads
+// the time step derived from it and the dt/h every stage reads.  Builds whose
+// speed-of-sound reduction runs in binary64 (the binary64 scheme and the
+// selective-binary64 build) use the *_d members; the others the binary32 ones.
 struct StepState
 {
     double dt;            // binary64, LSRK3_IntegratorMPI.cpp:37-38
@@ -70,7 +72,10 @@ struct StepState
     float  dtinvh;        // (float)(dt/h), GPUlab.h:258
     float  sos;           // max speed of sound as binary32
     unsigned int sos_bits;// atomicMax target (non-negative floats order as uints)
-    int    pad;
+    int    sos_is_d;      // 1 when the last reduction ran in binary64
+    double dtinvh_d;      // dt/h in binary64
+    double sos_d;         // max speed of sound as binary64
+    unsigned long long sos_bits_d;   // atomicMax target (non-negative doubles order as u64)
 };
 
 } // namespace mpcf

@@ -1,0 +1,185 @@
+// mpcf_internal.h -- library-internal definitions shared by the C-ABI
+// translation units (mpcf_capi.cpp, mpcf_serve.cpp): error reporting, the
+// launcher tables of the arithmetic builds, the context struct.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+
+#include "mpcf_hip.h"
+#include "mpcf_launch.h"
+
+using mpcf::Geom;
+using mpcf::StepState;
+
+namespace mpcf {
+namespace detail {
+
+// records msg as this thread's last error (mpcf_last_error) and returns code
+int fail(int code, const std::string& msg);
+const char* last_error();
+
+}  // namespace detail
+}  // namespace mpcf
+
+using mpcf::detail::fail;
+
+namespace mpcf {
+namespace detail {
+
+#define HIP_TRY(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess)                                                           \
+            return fail(MPCF_EHIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define LAUNCH_CHECK(what)                                                                \
+    do {                                                                                  \
+        hipError_t e_ = hipGetLastError();                                                \
+        if (e_ != hipSuccess)                                                             \
+            return fail(MPCF_EHIP, std::string(what) + " launch: " + hipGetErrorString(e_)); \
+    } while (0)
+
+struct Launchers
+{
+    decltype(&mpcf::parity::launch_ghosts) ghosts;
+    decltype(&mpcf::parity::launch_sos) sos;
+    decltype(&mpcf::parity::launch_dt) dt;
+    decltype(&mpcf::parity::launch_sweep) sweep;
+    decltype(&mpcf::parity::launch_update) update;
+    decltype(&mpcf::parity::launch_pack) pack;
+    decltype(&mpcf::parity::launch_test_weno) test_weno;
+    decltype(&mpcf::parity::launch_test_face) test_face;
+    decltype(&mpcf::parity::batch_sos) b_sos;
+    decltype(&mpcf::parity::batch_dt) b_dt;
+    decltype(&mpcf::parity::batch_ghosts) b_ghosts;
+    decltype(&mpcf::parity::batch_xy) b_xy;
+    decltype(&mpcf::parity::batch_z) b_z;
+    decltype(&mpcf::parity::batch_flip) b_flip;
+};
+
+inline const Launchers kParity = {
+    mpcf::parity::launch_ghosts, mpcf::parity::launch_sos, mpcf::parity::launch_dt, mpcf::parity::launch_sweep,
+    mpcf::parity::launch_update, mpcf::parity::launch_pack, mpcf::parity::launch_test_weno,
+    mpcf::parity::launch_test_face, mpcf::parity::batch_sos, mpcf::parity::batch_dt, mpcf::parity::batch_ghosts,
+    mpcf::parity::batch_xy, mpcf::parity::batch_z, mpcf::parity::batch_flip};
+inline const Launchers kFast = {
+    mpcf::fast::launch_ghosts, mpcf::fast::launch_sos, mpcf::fast::launch_dt, mpcf::fast::launch_sweep,
+    mpcf::fast::launch_update, mpcf::fast::launch_pack, mpcf::fast::launch_test_weno,
+    mpcf::fast::launch_test_face, mpcf::fast::batch_sos, mpcf::fast::batch_dt, mpcf::fast::batch_ghosts,
+    mpcf::fast::batch_xy, mpcf::fast::batch_z, mpcf::fast::batch_flip};
+inline const Launchers kFastP = {
+    mpcf::fastp::launch_ghosts, mpcf::fastp::launch_sos, mpcf::fastp::launch_dt, mpcf::fastp::launch_sweep,
+    mpcf::fastp::launch_update, mpcf::fastp::launch_pack, mpcf::fastp::launch_test_weno,
+    mpcf::fastp::launch_test_face, mpcf::fastp::batch_sos, mpcf::fastp::batch_dt, mpcf::fastp::batch_ghosts,
+    mpcf::fastp::batch_xy, mpcf::fastp::batch_z, mpcf::fastp::batch_flip};
+inline const Launchers kF64 = {
+    mpcf::f64::launch_ghosts, mpcf::f64::launch_sos, mpcf::f64::launch_dt, mpcf::f64::launch_sweep,
+    mpcf::f64::launch_update, mpcf::f64::launch_pack, mpcf::f64::launch_test_weno,
+    mpcf::f64::launch_test_face, mpcf::f64::batch_sos, mpcf::f64::batch_dt, mpcf::f64::batch_ghosts,
+    mpcf::f64::batch_xy, mpcf::f64::batch_z, mpcf::f64::batch_flip};
+
+inline const unsigned kAllFlags = MPCF_FLAG_FAST | MPCF_FLAG_SOS_CPU | MPCF_FLAG_SPLIT_XY | MPCF_FLAG_FUSED_XY |
+                           MPCF_FLAG_P64 | MPCF_FLAG_FP64;
+
+// arithmetic build selected by the flags
+inline const Launchers* pick_launchers(const unsigned flags)
+{
+    if (flags & MPCF_FLAG_FP64) return &kF64;
+    if (flags & MPCF_FLAG_P64) return &kFastP;
+    return (flags & MPCF_FLAG_FAST) ? &kFast : &kParity;
+}
+
+// storage bytes per element of a context with these flags
+inline size_t elem_bytes(const unsigned flags) { return (flags & MPCF_FLAG_FP64) ? sizeof(double) : sizeof(float); }
+
+// query helper used by mpcf_query: which x/y variant runs
+// LSRK3 coefficients (LSRK3_IntegratorMPI.cpp:46,51,56), rounded to the
+// storage type by the launchers like GPUlab::process_all's Real arguments
+// (GPUlab.h:258): binary32, or binary64 in the binary64 build
+inline const double kA[3] = {0.0, -17. / 32, -32. / 27};
+inline const double kB[3] = {1. / 4, 8. / 9, 3. / 4};
+
+}  // namespace detail
+}  // namespace mpcf
+
+using mpcf::detail::Launchers;
+using mpcf::detail::pick_launchers;
+using mpcf::detail::elem_bytes;
+using mpcf::detail::kA;
+using mpcf::detail::kB;
+using mpcf::detail::kAllFlags;
+using mpcf::detail::kParity;
+using mpcf::detail::kFast;
+
+// one resident node block
+struct mpcf_ctx
+{
+    Geom g;
+    double h;
+    unsigned flags;
+    int bc[6];
+    int device;
+    size_t esize;       // bytes per stored element (4, or 8 with MPCF_FLAG_FP64)
+    char* base;         // single allocation
+    void* u[2];
+    int cur;
+    void* R;
+    void* acc;
+    StepState* st;      // device
+    StepState* st_host; // pinned host mirror
+    hipStream_t stream;
+    bool own_stream;
+    size_t bytes;
+    const Launchers* L;
+    // optional per-kernel event timing (mpcf_timing_enable)
+    bool timing;
+    std::vector<hipEvent_t> ev_pool;
+    struct Mark { int kind; hipEvent_t b, e; };
+    std::vector<Mark> marks;
+};
+
+namespace mpcf {
+namespace detail {
+
+// march length of the batch kernels for an axis of extent nd: `forced` if
+// positive, else the autotuned length for that extent, else min(nd, 64)
+// (enough parallelism comes from the other cases)
+inline int batch_march(const int nd, const int forced, const int tuned)
+{
+    if (forced > 0) return forced < nd ? forced : nd;
+    if (tuned > 0) return tuned < nd ? tuned : nd;
+    return nd < 64 ? nd : 64;
+}
+
+// descriptor of one case for the batch kernels, and its block counts in the
+// flat grids: blocks[0] x+y sweep, [1] z sweep, [2] ghost fill, [3] speed of sound
+inline void case_desc(const mpcf_ctx* c, const int cur, const int len_xy, const int len_z, CaseDesc& d, int blocks[4])
+{
+    d.g = c->g;
+    d.u[0] = c->u[0]; d.u[1] = c->u[1]; d.R = c->R; d.acc = c->acc; d.st = c->st; d.h = c->h;
+    for (int f = 0; f < 6; ++f) d.bc[f] = c->bc[f];
+    d.chunk_xy = len_xy;
+    d.chunk_z = len_z;
+    int dims[3];
+    parity::xy_grid(c->g, d.chunk_xy, dims);
+    d.nbx_xy = dims[0]; d.nby_xy = dims[1];
+    blocks[0] = dims[0] * dims[1] * dims[2];
+    parity::z_grid(c->g, d.chunk_z, dims);
+    d.nbx_z = dims[0]; d.nby_z = dims[1];
+    blocks[1] = dims[0] * dims[1] * dims[2];
+    int tmax = c->g.ny * c->g.nz;
+    if (c->g.nx * c->g.nz > tmax) tmax = c->g.nx * c->g.nz;
+    if (c->g.nx * c->g.ny > tmax) tmax = c->g.nx * c->g.ny;
+    d.nb_ghost = (tmax + 255) / 256;
+    blocks[2] = 6 * d.nb_ghost;
+    const int rows4 = (c->g.ny * c->g.nz + 3) / 4;
+    d.nb_sos = rows4 < 2048 ? rows4 : 2048;
+    blocks[3] = d.nb_sos;
+    (void)cur;
+}
+
+}  // namespace detail
+}  // namespace mpcf

@@ -1,8 +1,12 @@
 // mpcf_kernels.hip -- CDNA4 kernels of one LSRK3 stage and the dt reduction.
 //
-// Built twice (see build.py):
+// Built four times (see build.py):
 //   MPCF_FAST=0, -ffp-contract=off  -> namespace mpcf::parity  (bit-exact scheme)
 //   MPCF_FAST=1, -ffp-contract=fast -> namespace mpcf::fast
+//   MPCF_FAST=1, MPCF_P64=1         -> namespace mpcf::fastp   (fast + binary64
+//                                      pressure recovery and speed-of-sound reduction)
+//   MPCF_F64=1,  -ffp-contract=off  -> namespace mpcf::f64     (the scheme in binary64:
+//                                      storage, arithmetic and reductions)
 //
 // One RK stage (GPUlab::process_all, source/GPUlab.cpp:703-793) is
 //   ghost fill -> x sweep -> y sweep -> z sweep (+ register update)
@@ -14,10 +18,25 @@
 // Accumulators carry the partial flux sums between sweeps in the reference's
 // summation order (((((fxp-fxm)+fyp)-fym)+fzp)-fzm) (GPUkernels.cu:1111).
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include "mpcf_scheme.h"
 #include "mpcf_launch.h"
 
-#if MPCF_FAST
+#ifndef MPCF_FAST
+#define MPCF_FAST 0
+#endif
+#ifndef MPCF_P64
+#define MPCF_P64 0
+#endif
+#ifndef MPCF_F64
+#define MPCF_F64 0
+#endif
+
+#if MPCF_F64
+#define MPCF_NS f64
+#elif MPCF_P64
+#define MPCF_NS fastp
+#elif MPCF_FAST
 #define MPCF_NS fast
 #else
 #define MPCF_NS parity
@@ -25,6 +44,16 @@
 
 namespace mpcf {
 namespace MPCF_NS {
+
+#if MPCF_F64
+typedef double Real;
+#else
+typedef float Real;
+#endif
+typedef PrimT<Real> Prim;
+constexpr bool kP64 = MPCF_P64 != 0;
+// the speed-of-sound reduction and dt/h run in binary64
+constexpr bool kSosD = MPCF_P64 != 0 || MPCF_F64 != 0;
 
 typedef FaceFluxT<Real> FaceFlux;
 typedef CellRecT<Real> CellRec;
@@ -93,7 +122,7 @@ __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, cons
         const Real sG = in.acc[A_SUMG] + cG;
         const Real sP = in.acc[A_SUMP] + cP;
         const Real dU = in.acc[A_DIVU] + (F.vel - Fl.vel);
-        const Real factor6 = 1.0f / 6.0f;
+        const Real factor6 = Real(1.0f) / Real(6.0f);
 #pragma unroll
         for (int q = 0; q < NVAR; ++q) {
             const Real t = (in.acc[q] + F.f[q]) - Fl.f[q];
@@ -135,12 +164,13 @@ struct March
 
 __device__ __forceinline__ Prim load_prim(const Real* __restrict__ u, const int64_t fs, const int e)
 {
-    return to_prim<Real, kFast>(u[e], u[fs + e], u[2*fs + e], u[3*fs + e], u[4*fs + e], u[5*fs + e], u[6*fs + e]);
+    return to_prim<Real, kFast, kP64>(u[e], u[fs + e], u[2*fs + e], u[3*fs + e], u[4*fs + e], u[5*fs + e], u[6*fs + e]);
 }
 
 __device__ __forceinline__ Real stage_dtinvh(const StageArgs& A)
 {
-    return A.st ? A.st->dtinvh : A.dtinvh;
+    if (!A.st) return A.dtinvh;
+    return MPCF_F64 ? Real(A.st->dtinvh_d) : Real(A.st->dtinvh);
 }
 
 // ---------------------------------------------------------------------------
@@ -177,9 +207,9 @@ __device__ __forceinline__ void sweep_yz_body(const Geom& g, const StageArgs& A,
         r3[q] = A.u[q*fs + e0 + (c0    ) * s];
     }
     m.w1 = load_prim(A.u, fs, e0 + (c0 - 3) * s);
-    m.w2 = to_prim<Real, kFast>(r1[0], r1[1], r1[2], r1[3], r1[4], r1[5], r1[6]);
-    m.w3 = to_prim<Real, kFast>(r2[0], r2[1], r2[2], r2[3], r2[4], r2[5], r2[6]);
-    m.w4 = to_prim<Real, kFast>(r3[0], r3[1], r3[2], r3[3], r3[4], r3[5], r3[6]);
+    m.w2 = to_prim<Real, kFast, kP64>(r1[0], r1[1], r1[2], r1[3], r1[4], r1[5], r1[6]);
+    m.w3 = to_prim<Real, kFast, kP64>(r2[0], r2[1], r2[2], r2[3], r2[4], r2[5], r2[6]);
+    m.w4 = to_prim<Real, kFast, kP64>(r3[0], r3[1], r3[2], r3[3], r3[4], r3[5], r3[6]);
     // software pipeline: raw values of the next incoming cell and the emit
     // operands of the next step are loaded one step ahead
     Real nxt[NVAR];
@@ -190,7 +220,7 @@ __device__ __forceinline__ void sweep_yz_body(const Geom& g, const StageArgs& A,
         // rotate the raw ring: r0 = cell k-1, r3 = cell k+2
 #pragma unroll
         for (int q = 0; q < NVAR; ++q) { r0[q] = r1[q]; r1[q] = r2[q]; r2[q] = r3[q]; r3[q] = nxt[q]; }
-        const Prim q = to_prim<Real, kFast>(nxt[0], nxt[1], nxt[2], nxt[3], nxt[4], nxt[5], nxt[6]);  // cell k+2
+        const Prim q = to_prim<Real, kFast, kP64>(nxt[0], nxt[1], nxt[2], nxt[3], nxt[4], nxt[5], nxt[6]);  // cell k+2
         if (k + 3 <= c1 + 2) {
 #pragma unroll
             for (int f = 0; f < NVAR; ++f) nxt[f] = A.u[f*fs + e0 + (k + 3) * s];
@@ -282,7 +312,7 @@ void k_sweep_x(const Geom g, const StageArgs A, const int chunk)
         // stage the run (previous run's consumers and storers are past the barrier below)
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const Prim q = to_prim<Real, kFast>(pre[i][0], pre[i][1], pre[i][2], pre[i][3], pre[i][4], pre[i][5], pre[i][6]);
+            const Prim q = to_prim<Real, kFast, kP64>(pre[i][0], pre[i][1], pre[i][2], pre[i][3], pre[i][4], pre[i][5], pre[i][6]);
             const int pl = i * 8 + lane / SX;
             S.in[FR][pl][xl] = q.r; S.in[FU][pl][xl] = q.u; S.in[FV][pl][xl] = q.v; S.in[FW][pl][xl] = q.w;
             S.in[FE][pl][xl] = q.p; S.in[FG][pl][xl] = q.G; S.in[FP][pl][xl] = q.P;
@@ -336,16 +366,24 @@ void k_sweep_x(const Geom g, const StageArgs A, const int chunk)
 enum { XY_TILE = 58 };
 
 // value of lane l-1 / lane l+1; the wave's edge lanes keep their own value
-__device__ __forceinline__ Real from_left(const Real v)
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(const float v)
 {
     const int b = __builtin_bit_cast(int, v);
-    return __builtin_bit_cast(Real, __builtin_amdgcn_update_dpp(b, b, 0x138, 0xf, 0xf, false));   // wave_shr:1
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, CTRL, 0xf, 0xf, false));
 }
-__device__ __forceinline__ Real from_right(const Real v)
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(const double v)
 {
-    const int b = __builtin_bit_cast(int, v);
-    return __builtin_bit_cast(Real, __builtin_amdgcn_update_dpp(b, b, 0x130, 0xf, 0xf, false));   // wave_shl:1
+    // the two 32-bit halves move with the same lane shift
+    const long long b = __builtin_bit_cast(long long, v);
+    const int lo = (int)(b & 0xffffffffLL), hi = (int)(b >> 32);
+    const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
+    const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((long long)hi2 << 32) | (long long)(unsigned)lo2);
 }
+__device__ __forceinline__ Real from_left(const Real v) { return dpp_move<0x138>(v); }    // wave_shr:1
+__device__ __forceinline__ Real from_right(const Real v) { return dpp_move<0x130>(v); }   // wave_shl:1
 __device__ __forceinline__ Prim prim_from_left(const Prim& q)
 {
     Prim o;
@@ -393,7 +431,7 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
 #pragma unroll
     for (int q = 0; q < NVAR; ++q) nxt[q] = A.u[q*fs + e0 + (c0 + 1) * s];
     for (int k = c0 - 1; k <= c1; ++k) {
-        const Prim q = to_prim<Real, kFast>(nxt[0], nxt[1], nxt[2], nxt[3], nxt[4], nxt[5], nxt[6]);  // row k+2
+        const Prim q = to_prim<Real, kFast, kP64>(nxt[0], nxt[1], nxt[2], nxt[3], nxt[4], nxt[5], nxt[6]);  // row k+2
         if (k + 3 <= c1 + 2) {
 #pragma unroll
             for (int f = 0; f < NVAR; ++f) nxt[f] = A.u[f*fs + e0 + (k + 3) * s];
@@ -479,34 +517,40 @@ void k_ghosts(const Geom g, Real* __restrict__ u, const int64_t fs, const BCSet 
 // max speed of sound: reset, reduce, finalize dt (GPUkernels.cu:1182-1222,
 // MaxSpeedOfSound.cpp:18-55, LSRK3_IntegratorMPI.cpp:37-38)
 // ---------------------------------------------------------------------------
-__global__ void k_sos_reset(StepState* st) { st->sos_bits = 0u; }
+__device__ __forceinline__ void sos_reset(StepState* st) { st->sos_bits = 0u; st->sos_bits_d = 0ull; }
+
+__global__ void k_sos_reset(StepState* st) { sos_reset(st); }
+
+// scalar type of the speed-of-sound reduction
+typedef std::conditional<kSosD, double, float>::type SosT;
 
 template <int CPU_FORM>
 __device__ __forceinline__ void sos_body(const Geom& g, const Real* __restrict__ u, const int64_t fs, StepState* st,
-                                         const int b, const int nb, Real* wmax)
+                                         const int b, const int nb, SosT* wmax)
 {
     const int rows = g.ny * g.nz;
-    Real sos = 0.0f;
+    SosT sos = 0.0f;
     for (int row = b * 4 + threadIdx.y; row < rows; row += nb * 4) {
         const int e0 = g.idx(0, row % g.ny, row / g.ny);
         for (int x = threadIdx.x; x < g.nx; x += 64) {
             const int e = e0 + x;
-            const Real r = u[e], mu = u[fs + e], mv = u[2*fs + e], mw = u[3*fs + e];
-            const Real E = u[4*fs + e], G = u[5*fs + e], P = u[6*fs + e];
-            Real p;
-            if (CPU_FORM) p = (Real)((E - (mu*mu + mv*mv + mw*mw)*(0.5/r) - P)/G);
-            else          p = (E - (mu*mu + mv*mv + mw*mw)*(0.5f/r) - P) / G;
-            const Real c = sqrtf(((p + P) / G + p) / r);
-            sos = fmaxf(sos, c + fmaxf(fmaxf(fabsf(mu), fabsf(mv)), fabsf(mw)) / r);
+            const SosT r = u[e], mu = u[fs + e], mv = u[2*fs + e], mw = u[3*fs + e];
+            const SosT E = u[4*fs + e], G = u[5*fs + e], P = u[6*fs + e];
+            SosT p;
+            if (CPU_FORM) p = (SosT)((E - (mu*mu + mv*mv + mw*mw)*(0.5/r) - P)/G);
+            else          p = (E - (mu*mu + mv*mv + mw*mw)*(SosT(0.5f)/r) - P) / G;
+            const SosT c = msqrt(((p + P) / G + p) / r);
+            sos = mmax(sos, c + mmax(mmax(fabs(mu), fabs(mv)), fabs(mw)) / r);
         }
     }
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) sos = fmaxf(sos, __shfl_xor(sos, off, 64));
+    for (int off = 32; off > 0; off >>= 1) sos = mmax(sos, __shfl_xor(sos, off, 64));
     if (threadIdx.x == 0) wmax[threadIdx.y] = sos;
     __syncthreads();
     if (threadIdx.x == 0 && threadIdx.y == 0) {
-        const Real m = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
-        atomicMax(&st->sos_bits, __float_as_uint(m));
+        const SosT m = mmax(mmax(wmax[0], wmax[1]), mmax(wmax[2], wmax[3]));
+        if (kSosD) atomicMax(&st->sos_bits_d, (unsigned long long)__double_as_longlong((double)m));
+        else       atomicMax(&st->sos_bits, __float_as_uint((float)m));
     }
 }
 
@@ -514,19 +558,30 @@ template <int CPU_FORM>
 __global__ __launch_bounds__(256)
 void k_sos(const Geom g, const Real* __restrict__ u, const int64_t fs, StepState* st)
 {
-    __shared__ Real wmax[4];
+    __shared__ SosT wmax[4];
     sos_body<CPU_FORM>(g, u, fs, st, blockIdx.x, gridDim.x, wmax);
 }
 
 __device__ __forceinline__ void dt_body(StepState* st, const double cfl, const double h, const double dt_max)
 {
-    const float sos = __uint_as_float(st->sos_bits);
-    st->sos = sos;
-    double dt = cfl*h/sos;
+    double dt;
+    if (kSosD) {
+        const double sos = __longlong_as_double((long long)st->sos_bits_d);
+        st->sos_d = sos;
+        st->sos = (float)sos;
+        dt = cfl*h/sos;
+    } else {
+        const float sos = __uint_as_float(st->sos_bits);
+        st->sos = sos;
+        st->sos_d = sos;
+        dt = cfl*h/sos;
+    }
+    st->sos_is_d = kSosD ? 1 : 0;
     dt = dt_max < dt ? dt_max : dt;
     st->dt = dt;
     st->h = h; st->cfl = cfl; st->dt_max = dt_max;
     st->dtinvh = (float)(dt/h);
+    st->dtinvh_d = dt/h;
 }
 
 __global__ void k_dt_finalize(StepState* st, const double cfl, const double h, const double dt_max)
@@ -551,10 +606,11 @@ void k_update(const Geom g, Real* __restrict__ u, const Real* __restrict__ R, co
     }
 }
 
-// dense reference SoA (NX*NY*NZ per field, Types.h:31) <-> padded layout
-template <int TO_PADDED>
+// dense reference SoA (NX*NY*NZ per field, Types.h:31) <-> padded layout;
+// the dense side may be binary32 or binary64 (converted on the way)
+template <int TO_PADDED, typename Dense>
 __global__ __launch_bounds__(256)
-void k_pack(const Geom g, Real* __restrict__ pad, Real* __restrict__ dense, const int64_t fs, const int nq)
+void k_pack(const Geom g, Real* __restrict__ pad, Dense* __restrict__ dense, const int64_t fs, const int nq)
 {
     const int rows = g.ny * g.nz;
     const int64_t n = (int64_t)g.nx * g.ny * g.nz;
@@ -564,8 +620,8 @@ void k_pack(const Geom g, Real* __restrict__ pad, Real* __restrict__ dense, cons
         const int64_t d0 = (int64_t)g.nx * row;
         for (int x = threadIdx.x; x < g.nx; x += 64)
             for (int q = 0; q < nq; ++q) {
-                if (TO_PADDED) pad[q*fs + e0 + x] = dense[q*n + d0 + x];
-                else           dense[q*n + d0 + x] = pad[q*fs + e0 + x];
+                if (TO_PADDED) pad[q*fs + e0 + x] = (Real)dense[q*n + d0 + x];
+                else           dense[q*n + d0 + x] = (Dense)pad[q*fs + e0 + x];
             }
     }
 }
@@ -591,7 +647,7 @@ __global__ void k_test_face(const int n, const Real* __restrict__ st, Real* __re
     const Real* s = st + 42*(int64_t)i;
     Prim c[6];
     for (int k = 0; k < 6; ++k)
-        c[k] = to_prim<Real, kFast>(s[0*6+k], s[1*6+k], s[2*6+k], s[3*6+k], s[4*6+k], s[5*6+k], s[6*6+k]);
+        c[k] = to_prim<Real, kFast, kP64>(s[0*6+k], s[1*6+k], s[2*6+k], s[3*6+k], s[4*6+k], s[5*6+k], s[6*6+k]);
     const CellRec left = reconstruct<Real, kFast>(c[0], c[1], c[2], c[3], c[4]);   // cell i-1
     const CellRec right = reconstruct<Real, kFast>(c[1], c[2], c[3], c[4], c[5]);  // cell i
     const FaceFlux F = riemann<D, Real, kFast>(left.m, right.p);
@@ -621,25 +677,26 @@ __device__ __forceinline__ int find_case(const int* __restrict__ off, const int 
 __device__ __forceinline__ StageArgs case_args(const CaseDesc& C, const int cur, const Real a, const Real b)
 {
     StageArgs A;
-    A.u = C.u[cur]; A.R = C.R; A.acc = C.acc; A.uo = C.u[1 - cur]; A.fs = C.g.fstride;
-    A.a = a; A.b = b; A.dtinvh = 0.0f; A.st = C.st;
+    A.u = (const Real*)C.u[cur]; A.R = (Real*)C.R; A.acc = (Real*)C.acc; A.uo = (Real*)C.u[1 - cur];
+    A.fs = C.g.fstride;
+    A.a = a; A.b = b; A.dtinvh = Real(0.0f); A.st = C.st;
     return A;
 }
 
 __global__ void kb_sos_reset(const BatchDev B)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < B.n) B.cases[c].st->sos_bits = 0u;
+    if (c < B.n) sos_reset(B.cases[c].st);
 }
 
 template <int CPU_FORM>
 __global__ __launch_bounds__(256)
 void kb_sos(const BatchDev B)
 {
-    __shared__ Real wmax[4];
+    __shared__ SosT wmax[4];
     const int c = find_case(B.off_sos, B.n, blockIdx.x);
     const CaseDesc& C = B.cases[c];
-    sos_body<CPU_FORM>(C.g, C.u[B.cur[c]], C.g.fstride, C.st, blockIdx.x - B.off_sos[c], C.nb_sos, wmax);
+    sos_body<CPU_FORM>(C.g, (const Real*)C.u[B.cur[c]], C.g.fstride, C.st, blockIdx.x - B.off_sos[c], C.nb_sos, wmax);
 }
 
 __global__ void kb_dt(const BatchDev B, const double cfl, const double dt_max)
@@ -654,7 +711,7 @@ void kb_ghosts(const BatchDev B)
     const int c = find_case(B.off_ghost, B.n, blockIdx.x);
     const CaseDesc& C = B.cases[c];
     const int local = blockIdx.x - B.off_ghost[c];
-    ghosts_body(C.g, C.u[B.cur[c]], C.g.fstride, C.bc, local % C.nb_ghost, local / C.nb_ghost);
+    ghosts_body(C.g, (Real*)C.u[B.cur[c]], C.g.fstride, C.bc, local % C.nb_ghost, local / C.nb_ghost);
 }
 
 __global__ __launch_bounds__(256)
@@ -705,18 +762,19 @@ static int rows_grid(const Geom& g)
     return blocks < 8192 ? blocks : 8192;
 }
 
-void launch_ghosts(const Geom& g, Real* u, const int bc[6], hipStream_t s)
+void launch_ghosts(const Geom& g, void* uv, const int bc[6], hipStream_t s)
 {
     BCSet b;
     for (int i = 0; i < 6; ++i) b.kind[i] = bc[i];
     int tmax = g.ny * g.nz;
     if (g.nx * g.nz > tmax) tmax = g.nx * g.nz;
     if (g.nx * g.ny > tmax) tmax = g.nx * g.ny;
-    hipLaunchKernelGGL(k_ghosts, dim3((tmax + 255) / 256, 6), dim3(256), 0, s, g, u, g.fstride, b);
+    hipLaunchKernelGGL(k_ghosts, dim3((tmax + 255) / 256, 6), dim3(256), 0, s, g, (Real*)uv, g.fstride, b);
 }
 
-void launch_sos(const Geom& g, const Real* u, StepState* st, const int cpu_form, hipStream_t s)
+void launch_sos(const Geom& g, const void* uv, StepState* st, const int cpu_form, hipStream_t s)
 {
+    const Real* u = (const Real*)uv;
     hipLaunchKernelGGL(k_sos_reset, dim3(1), dim3(1), 0, s, st);
     const int nb = rows_grid(g) < 2048 ? rows_grid(g) : 2048;
     if (cpu_form) hipLaunchKernelGGL(k_sos<1>, dim3(nb), dim3(64, 4), 0, s, g, u, g.fstride, st);
@@ -735,8 +793,8 @@ void launch_dt(StepState* st, const double cfl, const double h, const double dt_
 void launch_sweep(const Geom& g, const int dir, const SweepParams& p, const int stage, hipStream_t s)
 {
     StageArgs A;
-    A.u = p.u; A.R = p.R; A.acc = p.acc; A.uo = p.uo; A.fs = g.fstride;
-    A.a = p.a; A.b = p.b; A.dtinvh = p.dtinvh; A.st = p.st;
+    A.u = (const Real*)p.u; A.R = (Real*)p.R; A.acc = (Real*)p.acc; A.uo = (Real*)p.uo; A.fs = g.fstride;
+    A.a = (Real)p.a; A.b = (Real)p.b; A.dtinvh = (Real)p.dtinvh; A.st = p.st;
     if (dir == 3) {
         const int chunk = pick_chunk(g.ny, (int64_t)g.nx * g.nz, 1);
         int dims[3];
@@ -761,24 +819,37 @@ void launch_sweep(const Geom& g, const int dir, const SweepParams& p, const int 
     }
 }
 
-void launch_update(const Geom& g, Real* u, const Real* R, const Real b, hipStream_t s)
+void launch_update(const Geom& g, void* u, const void* R, const double b, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_update, dim3(rows_grid(g)), dim3(64, 4), 0, s, g, u, R, g.fstride, b);
+    hipLaunchKernelGGL(k_update, dim3(rows_grid(g)), dim3(64, 4), 0, s, g, (Real*)u, (const Real*)R, g.fstride, (Real)b);
 }
 
-void launch_pack(const Geom& g, Real* pad, Real* dense, const int to_padded, const int nq, hipStream_t s)
+void launch_pack(const Geom& g, void* pad, void* dense, const int to_padded, const int nq, const int dense_f64,
+                 hipStream_t s)
 {
-    if (to_padded) hipLaunchKernelGGL(k_pack<1>, dim3(rows_grid(g)), dim3(64, 4), 0, s, g, pad, dense, g.fstride, nq);
-    else           hipLaunchKernelGGL(k_pack<0>, dim3(rows_grid(g)), dim3(64, 4), 0, s, g, pad, dense, g.fstride, nq);
+    Real* P = (Real*)pad;
+    const dim3 grid(rows_grid(g)), block(64, 4);
+    if (dense_f64) {
+        if (to_padded) hipLaunchKernelGGL((k_pack<1, double>), grid, block, 0, s, g, P, (double*)dense, g.fstride, nq);
+        else           hipLaunchKernelGGL((k_pack<0, double>), grid, block, 0, s, g, P, (double*)dense, g.fstride, nq);
+    } else {
+        if (to_padded) hipLaunchKernelGGL((k_pack<1, float>), grid, block, 0, s, g, P, (float*)dense, g.fstride, nq);
+        else           hipLaunchKernelGGL((k_pack<0, float>), grid, block, 0, s, g, P, (float*)dense, g.fstride, nq);
+    }
 }
 
-void launch_test_weno(const int n, const Real* in5, Real* m, Real* p, hipStream_t s)
+void launch_test_weno(const int n, const void* in5v, void* mv, void* pv, hipStream_t s)
 {
+    const Real* in5 = (const Real*)in5v;
+    Real* m = (Real*)mv;
+    Real* p = (Real*)pv;
     hipLaunchKernelGGL(k_test_weno, dim3((n + 255) / 256), dim3(256), 0, s, n, in5, m, p);
 }
 
-void launch_test_face(const int n, const int dir, const Real* st, Real* out, hipStream_t s)
+void launch_test_face(const int n, const int dir, const void* stv, void* outv, hipStream_t s)
 {
+    const Real* st = (const Real*)stv;
+    Real* out = (Real*)outv;
     dim3 grid((n + 127) / 128), block(128);
     if (dir == 0)      hipLaunchKernelGGL(k_test_face<0>, grid, block, 0, s, n, st, out);
     else if (dir == 1) hipLaunchKernelGGL(k_test_face<1>, grid, block, 0, s, n, st, out);
@@ -818,14 +889,14 @@ void batch_ghosts(const BatchDev& B, const int blocks, hipStream_t s)
     hipLaunchKernelGGL(kb_ghosts, dim3(blocks), dim3(256), 0, s, B);
 }
 
-void batch_xy(const BatchDev& B, const int blocks, const float a, const float b, hipStream_t s)
+void batch_xy(const BatchDev& B, const int blocks, const double a, const double b, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_sweep_xy, dim3(blocks), dim3(64, 4), 0, s, B, a, b);
+    hipLaunchKernelGGL(kb_sweep_xy, dim3(blocks), dim3(64, 4), 0, s, B, (Real)a, (Real)b);
 }
 
-void batch_z(const BatchDev& B, const int blocks, const float a, const float b, hipStream_t s)
+void batch_z(const BatchDev& B, const int blocks, const double a, const double b, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_sweep_z, dim3(blocks), dim3(256), 0, s, B, a, b);
+    hipLaunchKernelGGL(kb_sweep_z, dim3(blocks), dim3(256), 0, s, B, (Real)a, (Real)b);
 }
 
 void batch_flip(const BatchDev& B, hipStream_t s)

@@ -1,5 +1,8 @@
-// mpcf_launch.h -- host launchers exported by the two builds of
-// mpcf_kernels.hip (namespace parity: bit-exact scheme; fast: FMA-contracted).
+// mpcf_launch.h -- host launchers exported by the four builds of
+// mpcf_kernels.hip (namespace parity: bit-exact scheme; fast: FMA-contracted;
+// fastp: fast with binary64 pressure recovery and speed-of-sound reduction;
+// f64: the scheme in binary64).  Field pointers are untyped: they point to
+// the build's storage type (binary32, or binary64 for f64).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -9,11 +12,12 @@ namespace mpcf {
 // arguments of one direction sweep of a stage
 struct SweepParams
 {
-    const float* u;        // stage input solution (ghosts filled)
-    float* R;              // LSRK3 register, updated in place by the z sweep
-    float* acc;            // accumulators (10 fields)
-    float* uo;             // stage output solution (stage == 1)
-    float a, b, dtinvh;    // stage coefficients; dtinvh ignored when st != NULL
+    const void* u;         // stage input solution (ghosts filled)
+    void* R;               // LSRK3 register, updated in place by the z sweep
+    void* acc;             // accumulators (10 fields)
+    void* uo;              // stage output solution (stage == 1)
+    double a, b, dtinvh;   // stage coefficients, rounded to the storage type by
+                           // the launcher; dtinvh ignored when st != NULL
     const StepState* st;   // device step state carrying dt/h
 };
 
@@ -21,9 +25,9 @@ struct SweepParams
 struct CaseDesc
 {
     Geom g;
-    float* u[2];           // stage input / output buffers (which is input: BatchDev::cur)
-    float* R;
-    float* acc;
+    void* u[2];            // stage input / output buffers (which is input: BatchDev::cur)
+    void* R;
+    void* acc;
     StepState* st;
     double h;
     int bc[6];
@@ -51,19 +55,19 @@ struct BatchDev
 
 #define MPCF_DECLARE_LAUNCHERS(NS)                                                                         \
 namespace mpcf { namespace NS {                                                                            \
-void launch_ghosts(const Geom& g, float* u, const int bc[6], hipStream_t s);                               \
-void launch_sos(const Geom& g, const float* u, StepState* st, int cpu_form, hipStream_t s);                 \
+void launch_ghosts(const Geom& g, void* u, const int bc[6], hipStream_t s);                                \
+void launch_sos(const Geom& g, const void* u, StepState* st, int cpu_form, hipStream_t s);                  \
 void launch_dt(StepState* st, double cfl, double h, double dt_max, hipStream_t s);                         \
 void launch_sweep(const Geom& g, int dir, const SweepParams& p, int stage, hipStream_t s);                \
-void launch_update(const Geom& g, float* u, const float* R, float b, hipStream_t s);                       \
-void launch_pack(const Geom& g, float* pad, float* dense, int to_padded, int nq, hipStream_t s);           \
-void launch_test_weno(int n, const float* in5, float* m, float* p, hipStream_t s);                         \
-void launch_test_face(int n, int dir, const float* st, float* out, hipStream_t s);                         \
+void launch_update(const Geom& g, void* u, const void* R, double b, hipStream_t s);                        \
+void launch_pack(const Geom& g, void* pad, void* dense, int to_padded, int nq, int dense_f64, hipStream_t s); \
+void launch_test_weno(int n, const void* in5, void* m, void* p, hipStream_t s);                            \
+void launch_test_face(int n, int dir, const void* st, void* out, hipStream_t s);                           \
 void batch_sos(const BatchDev& B, int blocks, int cpu_form, hipStream_t s);                                \
 void batch_dt(const BatchDev& B, double cfl, double dt_max, hipStream_t s);                                \
 void batch_ghosts(const BatchDev& B, int blocks, hipStream_t s);                                           \
-void batch_xy(const BatchDev& B, int blocks, float a, float b, hipStream_t s);                             \
-void batch_z(const BatchDev& B, int blocks, float a, float b, hipStream_t s);                              \
+void batch_xy(const BatchDev& B, int blocks, double a, double b, hipStream_t s);                           \
+void batch_z(const BatchDev& B, int blocks, double a, double b, hipStream_t s);                            \
 void batch_flip(const BatchDev& B, hipStream_t s);                                                         \
 void xy_grid(const Geom& g, int chunk, int dims[3]);                                                       \
 void z_grid(const Geom& g, int chunk, int dims[3]);                                                        \
@@ -72,3 +76,5 @@ int default_chunk(int nd, long long columns);                                   
 
 MPCF_DECLARE_LAUNCHERS(parity)
 MPCF_DECLARE_LAUNCHERS(fast)
+MPCF_DECLARE_LAUNCHERS(fastp)
+MPCF_DECLARE_LAUNCHERS(f64)

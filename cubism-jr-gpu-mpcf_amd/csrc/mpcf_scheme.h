@@ -40,6 +40,11 @@ template <typename T> struct Wide { typedef double type; };
 MPCF_HD float mmin(float a, float b) { return fminf(a, b); }
 MPCF_HD float mmax(float a, float b) { return fmaxf(a, b); }
 MPCF_HD float msqrt(float a) { return sqrtf(a); }
+MPCF_HD double mmin(double a, double b) { return fmin(a, b); }
+MPCF_HD double mmax(double a, double b) { return fmax(a, b); }
+MPCF_HD double msqrt(double a) { return sqrt(a); }
+MPCF_HD double mrcp(double a) { return 1.0 / a; }
+MPCF_HD double msqrt_fast(double a) { return sqrt(a); }
 // approximate reciprocal / square root of the FAST build: the hardware's
 // v_rcp_f32 / v_sqrt_f32 (1 ulp) instead of the correctly rounded sequences
 MPCF_HD float mrcp(float a)
@@ -68,12 +73,28 @@ struct PrimT { T r, u, v, w, p, G, P; };
 typedef PrimT<Real> Prim;
 
 // primitive recovery (GPUkernels.cu:209-244: u=ru/r, ..., p=(E-0.5 r|u|^2-P)/G)
-template <typename T, bool FAST = false>
+//
+// P64 (selective binary64, binary32 storage): the pressure is recovered in
+// binary64 from the stored binary32 values and rounded once.  In liquid cells
+// E and P are both close to gamma*pc/(gamma-1) (4828.79 for water) while
+// G*p is a few units, so E - KE - P cancels all but the last few bits of a
+// binary32 evaluation; in binary64 that cancellation is exact and p carries
+// only the storage error of E.
+template <typename T, bool FAST = false, bool P64 = false>
 MPCF_HD PrimT<T> to_prim(T r, T ru, T rv, T rw, T E, T G, T P)
 {
     PrimT<T> q;
     q.r = r;
-    if (FAST) {
+    if constexpr (P64) {
+        const double rd = r;
+        const double ird = 1.0 / rd;
+        const double ud = ru * ird, vd = rv * ird, wd = rw * ird;
+        q.u = T(ud);
+        q.v = T(vd);
+        q.w = T(wd);
+        const double ke = 0.5 * (double(ru)*ud + double(rv)*vd + double(rw)*wd);
+        q.p = T(((double(E) - ke) - double(P)) / double(G));
+    } else if (FAST) {
         const T ir = mrcp(r);
         q.u = ru * ir;
         q.v = rv * ir;

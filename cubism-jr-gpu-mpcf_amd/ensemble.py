@@ -7,10 +7,14 @@ ensemble_cases(): the batched-ensemble workload -- `ncases` independent
 run_batched() / run_plain(): advance a case list together (one launch per
     kernel kind per stage, optional hipGraph) or one case at a time with its
     own context (the plain solver); both report per-case time-to-solution.
-serve(): continuous batching of a Poisson request stream -- requests are
-    admitted while their HBM footprint fits the budget, join the live batch
-    between steps, leave it when done; reports p50/p99 time-to-solution and
-    aggregate cell-updates/s.  serve_plain() is the sequential FIFO baseline.
+serve(): continuous batching of a Poisson request stream driven from Python
+    over the batch API -- requests are admitted while their HBM footprint
+    fits the budget, join the live batch between steps, leave it when done.
+serve_native(): the same stream through the library's serving engine
+    (mpcf_server_*): submission returns at once, an engine thread admits,
+    steps and retires requests without synchronizing the device.
+Both report p50/p99 time-to-solution and aggregate cell-updates/s;
+serve_plain() is the sequential one-case-at-a-time baseline.
 """
 from __future__ import annotations
 
@@ -197,3 +201,37 @@ def serve_plain(requests, cfl=0.3, fast=True):
             _ = b.data
         tts[r.ident] = time.perf_counter() - t0 - r.arrival
     return tts, time.perf_counter() - t0
+
+
+def default_budget(fraction=0.8):
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    free_b, total_b = ctypes.c_size_t(), ctypes.c_size_t()
+    hip.hipMemGetInfo(ctypes.byref(free_b), ctypes.byref(total_b))
+    return int(fraction * free_b.value)
+
+
+def serve_native(requests, cfl=0.3, fast=True, policy="fifo_fill", fill_cells=0, hbm_budget=None, max_live=256,
+                 graph=True, keep_results=False):
+    """Submit each request at its arrival time to an mpcf_server and collect
+    completion times.  Returns per-request time-to-solution, the makespan,
+    the server's counters and (optionally) the results by request ident."""
+    if hbm_budget is None:
+        hbm_budget = default_budget()
+    pending = sorted(requests, key=lambda r: r.arrival)
+    with pympcf.Server(hbm_budget, policy=policy, fill_cells=fill_cells, max_live=max_live, cfl=cfl,
+                       fast=fast, graph=graph) as S:
+        t0 = pympcf.server_clock()
+        by_id = {}
+        for r in pending:
+            now = pympcf.server_clock() - t0
+            if r.arrival > now:
+                time.sleep(r.arrival - now)
+            rid = S.submit(r.fields(), r.steps, h=1.0 / r.n)
+            by_id[rid] = r
+        S.wait()
+        makespan = pympcf.server_clock() - t0
+        stats = S.stats()
+        tts = {by_id[rid].ident: S.t_done[rid] - t0 - by_id[rid].arrival for rid in S.t_done}
+        results = {by_id[rid].ident: out for rid, out in S.results.items()} if keep_results else {}
+    return tts, makespan, stats, results

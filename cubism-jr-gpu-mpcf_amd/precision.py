@@ -1,0 +1,116 @@
+"""Precision of the binary32 builds against a binary64 run of the same scheme.
+
+    python cubism-jr-gpu-mpcf_amd/precision.py [--n 512] [--steps 200] [--bubbles 50] [--json out.json]
+
+Workload: the bubble-cloud collapse (mpcf_ic_bubble_cloud: 50 air bubbles,
+lognormal radii mu = ln 0.02, sigma = 0.3, centres in a ball of radius 0.3,
+water at p = 100 and bubbles at p = 1), absorbing boundaries, CFL 0.3.
+
+The reference run is MPCF_FLAG_FP64: storage, WENO/HLLC arithmetic and the
+speed-of-sound / dt reduction in binary64, starting from the binary32 initial
+state widened exactly.  Each binary32 variant then advances the same initial
+state with the reference's dt sequence (dt_max = dt64[k] with a CFL large
+enough that the bound is what binds), so the error measured is that of the
+scheme's arithmetic and storage, not a drift in time.  Separately, the dt each
+variant's own reduction would have chosen is compared with dt64[k].
+
+Variants:
+  parity : the bit-exact build (the reference's binary32 arithmetic)
+  fast   : FMA-contracted binary32 arithmetic
+  p64    : fast + binary64 pressure recovery p = (E - KE - P)/G and binary64
+           speed-of-sound / dt reduction (MPCF_FLAG_P64)
+Reported per field: relative L1 = sum|u32 - u64| / sum|u64| and relative
+Linf = max|u32 - u64| / max|u64| over the interior, plus the relative
+difference of the conserved integrals (mass, momenta, energy) from the
+binary64 run's, all evaluated on the device in binary64 (mpcf_compare,
+mpcf_integrals).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import pympcf
+
+FIELDS = ("rho", "rho_u", "rho_v", "rho_w", "E", "G", "P")
+VARIANTS = {
+    "parity": dict(),
+    "fast": dict(fast=True),
+    "p64": dict(p64=True),
+}
+
+
+def run(n=512, steps=200, nbubbles=50, seed=0, cfl=0.3, variants=("parity", "fast", "p64"), log=None):
+    log = log or (lambda *a: None)
+    h = 1.0 / n
+    t0 = time.perf_counter()
+    fields, bubbles = pympcf.ic_bubble_cloud(n, n, n, nbubbles, seed)
+    log(f"initial state {n}^3 with {nbubbles} bubbles in {time.perf_counter() - t0:.1f} s")
+    out = {"workload": f"bubble-cloud collapse {n}^3, {nbubbles} bubbles, {steps} LSRK3 steps, CFL {cfl}",
+           "n": n, "steps": steps, "variants": {}}
+    with pympcf.Block(n, n, n, h, fp64=True) as ref:
+        ref.data = fields
+        i0 = ref.integrals()
+        dts, sos64 = [], []
+        t0 = time.perf_counter()
+        for k in range(steps):
+            dts.append(ref.step(cfl))
+            sos64.append(ref.last_sos64())
+        t64 = time.perf_counter() - t0
+        i64 = ref.integrals()
+        log(f"binary64 run: {steps} steps in {t64:.1f} s, t_end = {sum(dts):.6e}")
+        out["fp64"] = {"seconds": t64, "ms_per_step": 1e3 * t64 / steps, "t_end": sum(dts),
+                       "integrals_start": i0, "integrals_end": i64}
+        for name in variants:
+            kw = VARIANTS[name]
+            with pympcf.Block(n, n, n, h, **kw) as b:
+                b.data = fields
+                dt_rel = 0.0
+                t0 = time.perf_counter()
+                for k in range(steps):
+                    dt = b.step(2 * cfl, dt_max=dts[k])
+                    if dt != dts[k]:
+                        raise RuntimeError(f"{name}: step {k} took dt {dt!r}, expected {dts[k]!r}")
+                    # the dt this variant's own reduction gives at CFL cfl
+                    s = b.last_sos64() if kw.get("p64") else b.last_dt()[1]
+                    own = cfl * h / s
+                    dt_rel = max(dt_rel, abs(own - dts[k]) / dts[k])
+                tv = time.perf_counter() - t0
+                cmp = b.compare(ref)
+                iv = b.integrals()
+            res = {
+                "seconds": tv,
+                "rel_l1": {f: cmp[q]["rel_l1"] for q, f in enumerate(FIELDS)},
+                "rel_linf": {f: cmp[q]["rel_linf"] for q, f in enumerate(FIELDS)},
+                "integral_rel_diff": {f: (abs(iv[q] - i64[q]) / abs(i64[q]) if i64[q] else abs(iv[q]))
+                                      for q, f in enumerate(FIELDS)},
+                "dt_own_rel_err_max": dt_rel,
+            }
+            out["variants"][name] = res
+            log(f"{name}: {tv:.1f} s; rel L1 " + " ".join(f"{f}={res['rel_l1'][f]:.2e}" for f in FIELDS))
+            log(f"{name}: rel Linf " + " ".join(f"{f}={res['rel_linf'][f]:.2e}" for f in FIELDS)
+                + f"; own-dt rel err {dt_rel:.2e}")
+    return out
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--bubbles", type=int, default=50)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--json", default=None)
+    a = ap.parse_args(argv)
+    res = run(a.n, a.steps, a.bubbles, a.seed, log=lambda *m: print("[precision]", *m, flush=True))
+    s = json.dumps(res, indent=1)
+    if a.json:
+        with open(a.json, "w") as f:
+            f.write(s)
+    else:
+        print(s)
+
+
+if __name__ == "__main__":
+    sys.exit(main())

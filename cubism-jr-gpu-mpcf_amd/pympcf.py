@@ -16,9 +16,9 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmpcf_hip.so")
 
-OK, EINVAL, EHIP, ENOMEM, ENUMERIC = 0, 1, 2, 3, 4
+OK, EINVAL, EHIP, ENOMEM, ENUMERIC, EBUSY = 0, 1, 2, 3, 4, 5
 BC_ABSORBING, BC_REFLECTING = 0, 1
-FLAG_FAST, FLAG_SOS_CPU, FLAG_SPLIT_XY, FLAG_FUSED_XY = 1, 2, 4, 8
+FLAG_FAST, FLAG_SOS_CPU, FLAG_SPLIT_XY, FLAG_FUSED_XY, FLAG_P64, FLAG_FP64 = 1, 2, 4, 8, 16, 32
 DATA, TMP = 0, 1
 NVAR = 7
 
@@ -36,8 +36,14 @@ EXPORTS = [
     "mpcf_host_face", "mpcf_case_bytes", "mpcf_batch_create", "mpcf_batch_destroy", "mpcf_batch_add",
     "mpcf_batch_remove", "mpcf_batch_download", "mpcf_batch_step", "mpcf_batch_use_graph",
     "mpcf_batch_last_dt", "mpcf_batch_synchronize", "mpcf_batch_get_stream", "mpcf_batch_set_march",
-    "mpcf_batch_autotune", "mpcf_batch_query",
+    "mpcf_batch_autotune", "mpcf_batch_query", "mpcf_upload_f64", "mpcf_download_f64",
+    "mpcf_last_sos64", "mpcf_compare", "mpcf_integrals", "mpcf_server_create", "mpcf_server_submit",
+    "mpcf_server_poll", "mpcf_server_wait", "mpcf_server_stats", "mpcf_server_clock", "mpcf_server_destroy",
 ]
+
+# serving policies and the eager-launch flag (mpcf_server_create)
+SERVE_ALL, SERVE_FIFO_FILL, SERVE_SRPT_FILL = 0, 1, 2
+FLAG_SERVE_EAGER = 64
 
 # kernel classes of mpcf_timing_read
 KERNEL_CLASSES = ("ghosts", "max_sos", "dt", "sweep_x", "sweep_y", "sweep_z", "update", "other")
@@ -105,9 +111,24 @@ def lib():
         L.mpcf_batch_set_march.argtypes = [P, I, I, I]
         L.mpcf_batch_autotune.argtypes = [P, I, D, P]
         L.mpcf_batch_query.argtypes = [P, P]
+        L.mpcf_upload_f64.argtypes = [P, I, P]
+        L.mpcf_download_f64.argtypes = [P, I, P]
+        L.mpcf_last_sos64.argtypes = [P, C.POINTER(D)]
+        L.mpcf_compare.argtypes = [P, P, I, P]
+        L.mpcf_integrals.argtypes = [P, I, P]
+        LL = C.c_longlong
+        L.mpcf_server_create.argtypes = [C.POINTER(P), C.c_uint, LL, I, LL, I, D]
+        L.mpcf_server_submit.argtypes = [P, I, I, I, D, P, I, P, P, P]
+        L.mpcf_server_poll.argtypes = [P, P, P, I, P]
+        L.mpcf_server_wait.argtypes = [P, D]
+        L.mpcf_server_stats.argtypes = [P, P]
+        L.mpcf_server_clock.argtypes = []
+        L.mpcf_server_destroy.argtypes = [P]
         for name in EXPORTS:
-            if name not in ("mpcf_last_error", "mpcf_version", "mpcf_get_stream", "mpcf_batch_get_stream"):
+            if name not in ("mpcf_last_error", "mpcf_version", "mpcf_get_stream", "mpcf_batch_get_stream",
+                            "mpcf_server_clock"):
                 getattr(L, name).restype = I
+        L.mpcf_server_clock.restype = D
         _LIB = L
     return _LIB
 
@@ -131,10 +152,15 @@ def _fields(arr):
 class Block:
     """One resident node block (an mpcf_ctx)."""
 
-    def __init__(self, nx, ny, nz, h, fast=False, sos_cpu=False, bc=None, split_xy=None):
-        """split_xy: None = the library's default for the build, True/False = force."""
+    def __init__(self, nx, ny, nz, h, fast=False, sos_cpu=False, bc=None, split_xy=None, p64=False, fp64=False):
+        """split_xy: None = the library's default for the build, True/False = force.
+        p64: binary32 storage with binary64 pressure recovery and speed-of-sound
+        reduction (FAST arithmetic otherwise); fp64: the whole scheme in binary64."""
         self.shape = (nz, ny, nx)
+        self.h = float(h)
+        fast = fast or p64
         flags = (FLAG_FAST if fast else 0) | (FLAG_SOS_CPU if sos_cpu else 0)
+        flags |= (FLAG_P64 if p64 else 0) | (FLAG_FP64 if fp64 else 0)
         if split_xy is True:
             flags |= FLAG_SPLIT_XY
         elif split_xy is False:
@@ -173,6 +199,39 @@ class Block:
         out = np.empty((NVAR,) + self.shape, dtype=np.float32)
         _check(lib().mpcf_download(self._h, which, _fields(out)))
         return out
+
+    def upload_f64(self, which, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float64)
+        assert a.shape == (NVAR,) + self.shape, a.shape
+        _check(lib().mpcf_upload_f64(self._h, which, _fields(a)))
+
+    def download_f64(self, which=DATA):
+        out = np.empty((NVAR,) + self.shape, dtype=np.float64)
+        _check(lib().mpcf_download_f64(self._h, which, _fields(out)))
+        return out
+
+    def compare(self, ref, which=DATA):
+        """Per-field {l1, linf, l1_ref, linf_ref, rel_l1, rel_linf} of self - ref (binary64)."""
+        out = (C.c_double * 28)()
+        _check(lib().mpcf_compare(self._h, ref._h, which, C.cast(out, C.c_void_p)))
+        res = []
+        for q in range(NVAR):
+            s1, m1, s2, m2 = out[4 * q:4 * q + 4]
+            res.append({"l1": s1, "linf": m1, "l1_ref": s2, "linf_ref": m2,
+                        "rel_l1": s1 / s2 if s2 else float(s1 != 0) * float("inf"),
+                        "rel_linf": m1 / m2 if m2 else float(m1 != 0) * float("inf")})
+        return res
+
+    def integrals(self, which=DATA):
+        """h^3 * sum of each field over the interior (binary64)."""
+        out = (C.c_double * NVAR)()
+        _check(lib().mpcf_integrals(self._h, which, C.cast(out, C.c_void_p)))
+        return list(out)
+
+    def last_sos64(self):
+        s = C.c_double()
+        _check(lib().mpcf_last_sos64(self._h, C.byref(s)))
+        return s.value
 
     @property
     def data(self):
@@ -277,11 +336,12 @@ def test_weno(in5, fast=False):
     return m, p
 
 
-def test_face(st, d, fast=False):
+def test_face(st, d, fast=False, p64=False):
     s = np.ascontiguousarray(st, dtype=np.float32)
     n = s.shape[0]
     out = np.empty((n, 12), np.float32)
-    _check(lib().mpcf_test_face(n, d, _ptr(s), _ptr(out), FLAG_FAST if fast else 0))
+    flags = (FLAG_FAST if fast or p64 else 0) | (FLAG_P64 if p64 else 0)
+    _check(lib().mpcf_test_face(n, d, _ptr(s), _ptr(out), flags))
     return out
 
 
@@ -323,9 +383,10 @@ def case_bytes(nx, ny, nz):
 class Batch:
     """A batched ensemble (mpcf_batch): cases of uneven sizes advanced together."""
 
-    def __init__(self, fast=False, sos_cpu=False, graph=False):
+    def __init__(self, fast=False, sos_cpu=False, graph=False, p64=False):
         h_ = C.c_void_p()
-        _check(lib().mpcf_batch_create(C.byref(h_), (FLAG_FAST if fast else 0) | (FLAG_SOS_CPU if sos_cpu else 0)))
+        flags = (FLAG_FAST if fast else 0) | (FLAG_SOS_CPU if sos_cpu else 0) | (FLAG_P64 if p64 else 0)
+        _check(lib().mpcf_batch_create(C.byref(h_), flags))
         self._h = h_
         self.shapes = {}
         if graph:
@@ -394,3 +455,87 @@ class Batch:
         _check(lib().mpcf_batch_query(self._h, C.cast(out, C.c_void_p)))
         keys = ("cases", "cells", "bytes", "captures", "steps", "tuned")
         return dict(zip(keys, out))
+
+
+def server_clock():
+    """Seconds on the clock the server timestamps completions with."""
+    return lib().mpcf_server_clock()
+
+
+class Server:
+    """Continuous-batching request server (mpcf_server): submit cases from this
+    thread, the library's engine thread advances them on the device."""
+
+    POLICIES = {"all": SERVE_ALL, "fifo_fill": SERVE_FIFO_FILL, "srpt_fill": SERVE_SRPT_FILL}
+
+    def __init__(self, hbm_bytes, policy="fifo_fill", fill_cells=0, max_live=256, cfl=0.3, fast=True,
+                 p64=False, graph=True):
+        flags = (FLAG_FAST if fast or p64 else 0) | (FLAG_P64 if p64 else 0) | (0 if graph else FLAG_SERVE_EAGER)
+        h_ = C.c_void_p()
+        _check(lib().mpcf_server_create(C.byref(h_), flags, int(hbm_bytes), self.POLICIES[policy],
+                                        int(fill_cells), int(max_live), float(cfl)))
+        self._h = h_
+        self._keep = {}    # request id -> (input, output) arrays, alive until done
+        self.results = {}  # request id -> output fields
+        self.t_done = {}
+
+    def submit(self, fields, steps, h=None, bc=None):
+        a = np.ascontiguousarray(fields, dtype=np.float32)
+        nz, ny, nx = a.shape[1:]
+        out = np.empty_like(a)
+        bcp = None
+        if bc is not None:
+            b = np.ascontiguousarray(bc, dtype=np.int32)
+            bcp = _ptr(b)
+        rid = C.c_longlong()
+        hh = h if h is not None else 1.0 / max(nx, ny, nz)
+        _check(lib().mpcf_server_submit(self._h, nx, ny, nz, float(hh), bcp, int(steps), _fields(a), _fields(out),
+                                        C.cast(C.byref(rid), C.c_void_p)))
+        self._keep[rid.value] = (a, out)
+        return rid.value
+
+    def poll(self):
+        """[(id, completion time)] completed since the last poll; results land in self.results."""
+        got = []
+        cap = 256
+        ids = (C.c_longlong * cap)()
+        ts = (C.c_double * cap)()
+        n = C.c_int()
+        while True:
+            _check(lib().mpcf_server_poll(self._h, C.cast(ids, C.c_void_p), C.cast(ts, C.c_void_p), cap,
+                                          C.cast(C.byref(n), C.c_void_p)))
+            for k in range(n.value):
+                rid = ids[k]
+                got.append((rid, ts[k]))
+                self.results[rid] = self._keep.pop(rid)[1]
+                self.t_done[rid] = ts[k]
+            if n.value < cap:
+                return got
+
+    def wait(self, timeout=-1.0):
+        _check(lib().mpcf_server_wait(self._h, float(timeout)))
+        return self.poll()
+
+    def stats(self):
+        out = (C.c_longlong * 8)()
+        _check(lib().mpcf_server_stats(self._h, C.cast(out, C.c_void_p)))
+        keys = ("submitted", "completed", "steps", "captures", "rebuilds", "peak_active", "peak_bytes",
+                "active_cells")
+        return dict(zip(keys, list(out)))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _check(lib().mpcf_server_destroy(self._h))
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -31,7 +31,8 @@ enum {
     MPCF_EINVAL = 1,   /* bad argument (null pointer, size, enum value)        */
     MPCF_EHIP = 2,     /* a HIP runtime call failed                             */
     MPCF_ENOMEM = 3,   /* device allocation failed                              */
-    MPCF_ENUMERIC = 4  /* non-finite or non-positive dt / speed of sound        */
+    MPCF_ENUMERIC = 4, /* non-finite or non-positive dt / speed of sound        */
+    MPCF_EBUSY = 5     /* a wait timed out with work still outstanding          */
 };
 
 /* boundary kinds per face (face = 2*dir + side; dir 0,1,2 = x,y,z; side 0 = low) */
@@ -46,8 +47,15 @@ enum {
     MPCF_FLAG_SOS_CPU = 2u,  /* speed of sound with p recovered in binary64 as MaxSpeedOfSound_CPP
                                 (source/MaxSpeedOfSound.cpp:43) instead of the GPU form (GPUkernels.cu:1206) */
     MPCF_FLAG_SPLIT_XY = 4u, /* run x and y as two sweeps ...                                          */
-    MPCF_FLAG_FUSED_XY = 8u  /* ... or as one fused x+y sweep (same results either way).  Default: fused
+    MPCF_FLAG_FUSED_XY = 8u, /* ... or as one fused x+y sweep (same results either way).  Default: fused
                                 with MPCF_FLAG_FAST, split otherwise (the faster one for each build).  */
+    MPCF_FLAG_P64 = 16u,     /* selective binary64 on binary32 storage: the FAST arithmetic, with the
+                                pressure recovery p = (E - KE - P)/G and the speed-of-sound / dt
+                                reduction in binary64 (implies MPCF_FLAG_FAST)                      */
+    MPCF_FLAG_FP64 = 32u     /* the whole scheme in binary64 (storage, arithmetic, reductions): the
+                                high-precision run the binary32 builds are measured against.  Host
+                                arrays stay binary32 for mpcf_upload/mpcf_download (rounded on the way
+                                out); mpcf_upload_f64/mpcf_download_f64 move binary64 arrays.       */
 };
 
 /* which host array set */
@@ -84,6 +92,12 @@ int mpcf_set_bc(mpcf_ctx* ctx, const int bc[6]);
 int mpcf_upload(mpcf_ctx* ctx, int which, const float* const fields[7]);
 int mpcf_download(mpcf_ctx* ctx, int which, float* const fields[7]);
 
+/* The same transfers with binary64 host arrays (exact for an MPCF_FLAG_FP64
+ * context; rounded to nearest on upload / exact on download otherwise).
+ * No reference counterpart: the reference stores binary32 only (NodeBlock.h:26). */
+int mpcf_upload_f64(mpcf_ctx* ctx, int which, const double* const fields[7]);
+int mpcf_download_f64(mpcf_ctx* ctx, int which, double* const fields[7]);
+
 /* Fill the three-cell ghost band of u from the boundary conditions
  * (GPUlab::load_ghosts -> _apply_bc, source/GPUlab.cpp:622-644). */
 int mpcf_load_ghosts(mpcf_ctx* ctx);
@@ -116,6 +130,23 @@ int mpcf_step(mpcf_ctx* ctx, double cfl, double dt_max, double* dt);
 
 /* dt and max speed of sound of the last mpcf_step (synchronizes). */
 int mpcf_last_dt(mpcf_ctx* ctx, double* dt, float* sos);
+
+/* Max speed of sound of the last mpcf_step before rounding to binary32: the
+ * binary64 reduction of MPCF_FLAG_P64 / MPCF_FLAG_FP64 contexts (the binary32
+ * value widened otherwise).  Synchronizes. */
+int mpcf_last_sos64(mpcf_ctx* ctx, double* sos);
+
+/* Field-wise difference of two blocks of equal extent on one device, in
+ * binary64 (any storage precision on either side): out[4q + k] for field q
+ * is k=0: sum |a-b|, 1: max |a-b|, 2: sum |b|, 3: max |b| over the interior
+ * cells.  Reproducible (fixed reduction order).  Synchronizes both.  No
+ * reference counterpart: it measures a binary32 run against a binary64 one. */
+int mpcf_compare(mpcf_ctx* a, mpcf_ctx* b, int which, double out[28]);
+
+/* Conserved integrals: out[q] = h^3 * sum over interior cells of field q
+ * (mass, momenta, total energy, and the G / P sums), accumulated in binary64
+ * in a fixed order.  Synchronizes.  The reference computes no integrals. */
+int mpcf_integrals(mpcf_ctx* ctx, int which, double out[7]);
 
 /* Block until the context's queued work is done. */
 int mpcf_synchronize(mpcf_ctx* ctx);
@@ -224,6 +255,66 @@ int mpcf_host_face(int n, int dir, const float* st, float* out, unsigned flags);
  *     the HLLC face velocity, Gm, Gp, Pm, Pp (GPUkernels.cu:203-287). */
 int mpcf_test_weno(int n, const float* in5, float* minus, float* plus, unsigned flags);
 int mpcf_test_face(int n, int dir, const float* st, float* out, unsigned flags);
+
+/* ------------------------------------------------------------------------
+ * Serving: a stream of independent requests advanced with continuous
+ * batching by an engine thread (cubism-jr-gpu-mpcf_amd/csrc/mpcf_serve.cpp).
+ * No reference counterpart -- the reference runs one case per process
+ * (source/main.cpp) -- this is the request loop of the serving workload.
+ *
+ * A request is one block (binary32 host fields in, results out), its
+ * boundary kinds and a number of LSRK3 steps at the server's CFL.  Requests
+ * are admitted in arrival order while their footprint (mpcf_case_bytes) fits
+ * the HBM budget, which the server allocates once; each engine step advances
+ * an active subset of the admitted requests with one launch per kernel kind
+ * per stage (replayed as a hipGraph while the subset is unchanged).  Each
+ * request's result equals mpcf_step run alone on it for the same number of
+ * steps, bit for bit, under every policy.
+ * ------------------------------------------------------------------------ */
+typedef struct mpcf_server mpcf_server;
+
+/* active-subset policies */
+enum {
+    MPCF_SERVE_ALL = 0,        /* every admitted request steps every time                    */
+    MPCF_SERVE_FIFO_FILL = 1,  /* oldest admitted first, added while the cells fit fill_cells */
+    MPCF_SERVE_SRPT_FILL = 2   /* least remaining work (cells x steps) first, same fill       */
+};
+
+/* server flag: launch each step eagerly instead of replaying a hipGraph */
+enum { MPCF_FLAG_SERVE_EAGER = 64u };
+
+/* Create a server on the current device: `hbm_bytes` of device memory for
+ * request state, `max_live` admitted requests at most, `fill_cells` the
+ * active-subset cell budget of the *_FILL policies (<= 0: unbounded), `cfl`
+ * for every request.  flags: MPCF_FLAG_FAST, MPCF_FLAG_SOS_CPU,
+ * MPCF_FLAG_P64, MPCF_FLAG_SERVE_EAGER. */
+int mpcf_server_create(mpcf_server** out, unsigned flags, long long hbm_bytes, int policy,
+                       long long fill_cells, int max_live, double cfl);
+
+/* Queue a request (thread-safe, returns at once).  `in` and `out` (seven
+ * NX*NY*NZ binary32 fields each) must stay valid until the request is
+ * reported done.  bc may be NULL (all absorbing). */
+int mpcf_server_submit(mpcf_server* s, int nx, int ny, int nz, double h, const int bc[6], int steps,
+                       const float* const in[7], float* const out[7], long long* id);
+
+/* Up to `cap` completed requests since the last poll: ids and completion
+ * times on the mpcf_server_clock() clock (the result is in `out` by then). */
+int mpcf_server_poll(mpcf_server* s, long long* ids, double* t_done, int cap, int* n);
+
+/* Block until every submitted request is done (timeout_s < 0: no limit;
+ * MPCF_EBUSY on timeout). */
+int mpcf_server_wait(mpcf_server* s, double timeout_s);
+
+/* out: submitted, completed, engine steps, graph captures, active-set
+ * changes, most requests stepped together, peak bytes in use, cells of the
+ * current active set. */
+int mpcf_server_stats(mpcf_server* s, long long out[8]);
+
+/* Seconds on the steady clock the server timestamps completions with. */
+double mpcf_server_clock(void);
+
+/* Finish the outstanding requests, then free everything. */
+int mpcf_server_destroy(mpcf_server* s);
 
 #ifdef __cplusplus
 }

@@ -118,3 +118,44 @@ def test_autotune_keeps_state_and_results(mpcf):
     for j, (s, bc) in enumerate(cs):
         u, _, _ = alone(mpcf, s, bc, 1, fast=True)
         assert np.array_equal(got[j], u), j
+
+
+@pytest.mark.parametrize("policy,fill,graph", [("all", 0, True), ("fifo_fill", 40000, True),
+                                               ("srpt_fill", 40000, True), ("fifo_fill", 40000, False)],
+                         ids=["all", "fifo_fill", "srpt_fill", "fifo_fill-eager"])
+def test_server_results_equal_alone(mpcf, policy, fill, graph):
+    """Requests submitted while others run, stepped in changing active subsets,
+    each return exactly the state it reaches alone."""
+    import time
+    cs = cases(mpcf)
+    steps = [3, 5, 2, 4, 1, 6]
+    say("server", policy, "fill", fill, "graph", graph)
+    with mpcf.Server(1 << 30, policy=policy, fill_cells=fill, max_live=4, fast=True, graph=graph) as S:
+        ids = []
+        for k, ((s, bc), n) in enumerate(zip(cs, steps)):
+            h = 1.0 / max(s.shape[1:])
+            ids.append(S.submit(s, n, h=h, bc=bc))
+            if k % 2:
+                time.sleep(0.002)       # later requests arrive while earlier ones run
+        S.wait(60.0)
+        st = S.stats()
+        got = [S.results[i] for i in ids]
+    assert st["completed"] == len(cs) and st["peak_active"] <= 4, st
+    for j, ((s, bc), n) in enumerate(zip(cs, steps)):
+        u, _, _ = alone(mpcf, s, bc, n, fast=True)
+        assert np.array_equal(got[j], u), j
+
+
+def test_server_budget_queues_requests(mpcf):
+    """A budget that holds one case at a time still completes every request."""
+    s = states.random_smooth(20, 20, 20, seed=1)
+    one = mpcf.case_bytes(20, 20, 20)
+    with mpcf.Server(one + 4096, policy="all", max_live=8, fast=True) as S:
+        ids = [S.submit(s, 2, h=0.05) for _ in range(3)]
+        S.wait(60.0)
+        st = S.stats()
+        outs = [S.results[i] for i in ids]
+    assert st["peak_active"] == 1 and st["completed"] == 3, st
+    u, _, _ = alone(mpcf, s, [0] * 6, 2, fast=True)
+    for o in outs:
+        assert np.array_equal(o, u)

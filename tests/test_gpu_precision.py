@@ -1,0 +1,120 @@
+"""Precision builds on the GPU.
+
+* MPCF_FLAG_FP64 (the scheme in binary64) reproduces the oracle's binary64
+  instantiation bit for bit -- RHS, dt and state after LSRK3 steps -- so the
+  binary64 runs the binary32 builds are measured against are pinned to the
+  same oracle as the bit-exact build.
+* MPCF_FLAG_P64 (binary64 pressure recovery): on liquid stencils its face
+  fluxes are closer to the binary64 fluxes than the plain FAST build's.
+* mpcf_compare / mpcf_integrals agree with numpy in binary64, and total mass
+  stays constant to round-off in a closed (reflecting) box.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import states
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-prec]", *a, flush=True)
+    sys.stdout.flush()
+
+
+CASES = [
+    ("smooth", lambda: states.random_smooth(20, 16, 12, seed=11), [0, 1, 1, 0, 0, 1]),
+    ("tube", lambda: states.shock_tube(24, 8, 10, axis=1, seed=2), [0] * 6),
+]
+
+
+@pytest.mark.parametrize("name,make,bc", CASES, ids=[c[0] for c in CASES])
+def test_fp64_steps_match_oracle_binary64(mpcf, oracle, name, make, bc):
+    s = make().astype(np.float64)
+    nz, ny, nx = s.shape[1:]
+    h = 1.0 / max(nx, ny, nz)
+    say("fp64 vs oracle<double>", name, (nx, ny, nz))
+    ref = oracle.Block(nx, ny, nz, dtype=np.float64)
+    ref.data = s
+    with mpcf.Block(nx, ny, nz, h, bc=bc, fp64=True) as b:
+        b.upload_f64(mpcf.DATA, s)
+        for k in range(3):
+            dt = b.step(0.3)
+            dt_ref, _ = ref.step(h, 0.3, 1e30, bc)
+            assert dt == dt_ref, (k, dt, dt_ref)
+        assert np.array_equal(b.download_f64(mpcf.DATA), ref.data)
+        assert np.array_equal(b.download_f64(mpcf.TMP), ref.tmp)
+
+
+def test_fp64_rhs_matches_oracle_binary64(mpcf, oracle):
+    s = states.random_smooth(18, 14, 11, seed=5).astype(np.float64)
+    R0 = (states.random_smooth(18, 14, 11, seed=6, two_phase=False) * 1e-3).astype(np.float64)
+    nz, ny, nx = s.shape[1:]
+    bc = [1, 0, 0, 1, 1, 0]
+    # convection's arguments are binary32 values (GPUlab.h:258), widened exactly
+    a, dtinvh = -17.0 / 32, float(np.float32(0.0123))
+    ref = oracle.Block(nx, ny, nz, dtype=np.float64)
+    ref.data = s
+    ref.tmp = R0
+    ref.apply_bc(bc)
+    ref.rhs(a, dtinvh)
+    with mpcf.Block(nx, ny, nz, 1.0 / nx, bc=bc, fp64=True, split_xy=False) as b:
+        b.upload_f64(mpcf.DATA, s)
+        b.upload_f64(mpcf.TMP, R0)
+        b.load_ghosts()
+        b.convection(a, dtinvh)
+        assert np.array_equal(b.download_f64(mpcf.TMP), ref.tmp)
+
+
+def test_p64_faces_closer_to_binary64(mpcf, oracle):
+    st = states.random_stencils(4000, seed=21)
+    # keep the stencils whose six cells are all liquid (P large)
+    liquid = (st[:, 6, :] > 1000).all(axis=1)
+    st = st[liquid]
+    assert len(st) > 200, len(st)
+    say("p64 faces on", len(st), "liquid stencils")
+    for d in range(3):
+        ref = oracle.face_batch(st.astype(np.float64), d, dtype=np.float64)
+        e_fast = np.abs(mpcf.test_face(st, d, fast=True).astype(np.float64) - ref)
+        e_p64 = np.abs(mpcf.test_face(st, d, p64=True).astype(np.float64) - ref)
+        # the normal-momentum and energy fluxes carry the pressure error
+        for q in (1 + d, 4):
+            assert e_p64[:, q].mean() < e_fast[:, q].mean(), (d, q, e_p64[:, q].mean(), e_fast[:, q].mean())
+
+
+def test_compare_and_integrals_match_numpy(mpcf):
+    a32 = states.random_smooth(17, 13, 9, seed=8)
+    b64 = a32.astype(np.float64) * (1 + 1e-6 * np.random.default_rng(3).standard_normal(a32.shape))
+    nz, ny, nx = a32.shape[1:]
+    h = 1.0 / 17
+    with mpcf.Block(nx, ny, nz, h) as A, mpcf.Block(nx, ny, nz, h, fp64=True) as B:
+        A.data = a32
+        B.upload_f64(mpcf.DATA, b64)
+        res = A.compare(B)
+        ints = B.integrals()
+    a = a32.astype(np.float64)
+    for q in range(7):
+        d = np.abs(a[q] - b64[q])
+        assert res[q]["linf"] == d.max()
+        assert res[q]["linf_ref"] == np.abs(b64[q]).max()
+        assert res[q]["l1"] == pytest.approx(d.sum(), rel=1e-12)
+        assert res[q]["l1_ref"] == pytest.approx(np.abs(b64[q]).sum(), rel=1e-12)
+        assert ints[q] == pytest.approx(b64[q].sum() * h ** 3, rel=1e-12, abs=1e-300)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "p64", "fp32"])
+def test_mass_conserved_in_closed_box(mpcf, prec):
+    s = states.random_smooth(24, 20, 16, seed=4)
+    nz, ny, nx = s.shape[1:]
+    kw = {"fp64": dict(fp64=True), "p64": dict(p64=True), "fp32": dict()}[prec]
+    tol = {"fp64": 1e-13, "p64": 2e-6, "fp32": 2e-6}[prec]
+    with mpcf.Block(nx, ny, nz, 1.0 / nx, bc=[1] * 6, **kw) as b:
+        b.data = s
+        m0 = b.integrals()[0]
+        for _ in range(10):
+            b.step(0.3, sync=False)
+        m1 = b.integrals()[0]
+    say(prec, "mass drift", (m1 - m0) / m0)
+    assert abs(m1 - m0) <= tol * abs(m0), (m0, m1)
