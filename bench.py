@@ -22,9 +22,11 @@ z-slices) timed on this host on a bounded sample (a 160^3 block of the same
 workload, one LSRK3 step), scaled to the same unit.
 
 ensemble / serving (the metric's second half: p50/p99 against the plain
-solver): 64 uneven bubble-collapse cases batched vs one at a time, and a
-Poisson request stream served with continuous batching vs first-come
-first-served one at a time (cubism-jr-gpu-mpcf_amd/ensemble.py).
+solver): 64 uneven bubble-collapse cases batched vs one at a time, and
+Poisson request streams served by the library's continuous-batching server
+vs first-come first-served one at a time (cubism-jr-gpu-mpcf_amd/ensemble.py).
+precision: the binary32 builds against a binary64 run of the same scheme
+(cubism-jr-gpu-mpcf_amd/precision.py) on a smaller bubble cloud.
 """
 from __future__ import annotations
 
@@ -69,7 +71,12 @@ def parse():
     p.add_argument("--ensemble-steps", type=int, default=100)
     p.add_argument("--no-serving", action="store_true", help="skip the served-stream comparison")
     p.add_argument("--serving-requests", type=int, default=48)
-    p.add_argument("--serving-load", type=float, default=0.7,
+    p.add_argument("--serving-policy", default="fifo_fill", choices=("all", "fifo_fill", "srpt_fill"))
+    p.add_argument("--serving-fill", type=int, default=4 << 20, help="active-subset cell budget of the *_fill policies")
+    p.add_argument("--no-precision", action="store_true", help="skip the binary32-vs-binary64 leg")
+    p.add_argument("--precision-n", type=int, default=128)
+    p.add_argument("--precision-steps", type=int, default=50)
+    p.add_argument("--serving-load", type=float, nargs="+", default=[0.7, 1.1],
                    help="offered load as a fraction of the batched ensemble throughput")
     return p.parse_args()
 
@@ -116,28 +123,48 @@ def ensemble_leg(a):
     }
 
 
-def serving_leg(a, batched_rate):
-    """Poisson request stream (edges 32..192 step 16, 50..500 steps) offered at
-    a fixed rate, served with continuous batching and by the plain sequential
+def serving_leg(a):
+    """Poisson request streams (edges 32..192 step 16, 50..500 steps) offered
+    at fixed fractions of the plain solver's capacity, served by the library's
+    continuous-batching server (mpcf_server) and by the plain sequential
     solver; time-to-solution = completion - arrival."""
     import ensemble as E
-    probe = E.stream_cases(a.serving_requests, seed=1, rate=1.0)
-    mean_work = sum(c.work for c in probe) / len(probe)
-    rate = a.serving_load * batched_rate / mean_work
-    reqs = E.stream_cases(a.serving_requests, seed=1, rate=rate)
-    work = sum(c.work for c in reqs)
-    ts, ms, info = E.serve(reqs, cfl=a.cfl, fast=True, graph=True)
-    log(f"serving continuous batching: {ms:.2f} s")
-    tp, mp = E.serve_plain(reqs, cfl=a.cfl, fast=True)
-    log(f"serving plain: {mp:.2f} s")
-    return {
-        "workload": f"{len(reqs)} requests, Poisson at {rate:.2f}/s ({a.serving_load:.0%} of the batched ensemble "
-                    f"throughput), edges 32..192 step 16, 50..500 steps, bubble collapse, FP32",
-        "rate_per_s": rate,
-        "continuous_batching": dict(cell_updates_per_s=work / ms, makespan_s=ms, **E.latency_stats(list(ts.values())),
-                                    **info),
-        "plain_fifo": dict(cell_updates_per_s=work / mp, makespan_s=mp, **E.latency_stats(list(tp.values()))),
-    }
+    # the plain solver's capacity on this request mix: mean service time of a
+    # sample of requests run back to back
+    probe = E.stream_cases(12, seed=2, rate=1.0)
+    _, mk, _ = E.run_plain(probe, cfl=a.cfl, fast=True)
+    plain_rate = len(probe) / mk
+    log(f"plain solver capacity on the request mix: {plain_rate:.2f} requests/s")
+    out = {"workload": f"{a.serving_requests} requests per load, edges 32..192 step 16, 50..500 steps, "
+                       f"single-bubble collapse, FP32; Poisson arrivals",
+           "plain_capacity_per_s": plain_rate, "policy": a.serving_policy, "fill_cells": a.serving_fill,
+           "loads": []}
+    for load in a.serving_load:
+        rate = load * plain_rate
+        reqs = E.stream_cases(a.serving_requests, seed=1, rate=rate)
+        work = sum(c.work for c in reqs)
+        ts, ms, stats, _ = E.serve_native(reqs, cfl=a.cfl, fast=True, policy=a.serving_policy,
+                                          fill_cells=a.serving_fill)
+        log(f"serving load {load:.2f}: continuous batching {ms:.2f} s")
+        tp, mp = E.serve_plain(reqs, cfl=a.cfl, fast=True)
+        log(f"serving load {load:.2f}: plain {mp:.2f} s")
+        out["loads"].append({
+            "offered_load": load, "rate_per_s": rate,
+            "continuous_batching": dict(cell_updates_per_s=work / ms, makespan_s=ms,
+                                        **E.latency_stats(list(ts.values())),
+                                        **{k: stats[k] for k in ("steps", "captures", "rebuilds", "peak_active")}),
+            "plain_fifo": dict(cell_updates_per_s=work / mp, makespan_s=mp, **E.latency_stats(list(tp.values()))),
+        })
+    return out
+
+
+def precision_leg(a):
+    """binary32 builds against a binary64 run of the same scheme on a smaller
+    bubble cloud (the 512^3 x 200-step report is a separate run of
+    cubism-jr-gpu-mpcf_amd/precision.py)."""
+    import precision
+    return precision.run(n=a.precision_n, steps=a.precision_steps, nbubbles=50, seed=0, cfl=a.cfl,
+                         log=lambda *m: log("precision:", *m))
 
 
 def log(*msg):
@@ -266,8 +293,10 @@ def main():
     log(f"timed region {wall:.3f} s; {value:.3e} cell-updates/s")
     if not a.no_ensemble:
         out["ensemble"] = ensemble_leg(a)
-        if not a.no_serving:
-            out["serving"] = serving_leg(a, out["ensemble"]["batched"]["cell_updates_per_s"])
+    if not a.no_serving:
+        out["serving"] = serving_leg(a)
+    if not a.no_precision:
+        out["precision"] = precision_leg(a)
     if not a.no_cpu:
         log(f"CPU baseline: oracle on a {a.cpu_n}^3 sample")
         out["cpu_baseline"] = cpu_baseline(a.cpu_n)

@@ -91,6 +91,8 @@ void run_rhs(mpcf_ctx* c, const void* u, void* uo, double a, double b, double dt
     mpcf::SweepParams p;
     p.u = u; p.R = c->R; p.acc = c->acc; p.uo = uo;
     p.a = a; p.b = b; p.dtinvh = dtinvh; p.st = st;
+    p.ghosts = stage;                       // a stage output carries its own ghosts
+    p.reflect = mpcf::detail::reflect_mask(c->bc);
     // the fused x+y sweep wins when the arithmetic is cheap (FAST); the split
     // pair wins when it is VALU-bound (bit-exact build), unless forced
     const bool split = (c->flags & MPCF_FLAG_SPLIT_XY) ? true
@@ -152,6 +154,7 @@ extern "C" int mpcf_create(mpcf_ctx** out, int nx, int ny, int nz, double h, uns
         return fail(MPCF_EHIP, std::string("mpcf_create: ") + hipGetErrorString(e));
     }
     c->own_stream = true;
+    c->ghosts_valid = false;
     const size_t fb = (size_t)g.fstride * c->esize;   // bytes per field (a multiple of 256)
     c->u[0] = c->base;
     c->u[1] = c->base + 7 * fb;
@@ -201,6 +204,7 @@ extern "C" int mpcf_set_bc(mpcf_ctx* c, const int bc[6])
         if (bc[i] != MPCF_BC_ABSORBING && bc[i] != MPCF_BC_REFLECTING)
             return fail(MPCF_EINVAL, "mpcf_set_bc: unknown boundary kind");
     for (int i = 0; i < 6; ++i) c->bc[i] = bc[i];
+    c->ghosts_valid = false;
     return MPCF_OK;
 }
 
@@ -220,6 +224,7 @@ int transfer(mpcf_ctx* c, const int which, void* const* fields, const int host_f
     const size_t hb = host_f64 ? sizeof(double) : sizeof(float);
     const size_t fb = (size_t)c->g.fstride * c->esize;
     char* blk = (char*)(which == MPCF_DATA ? c->u[c->cur] : c->R);
+    if (to_device && which == MPCF_DATA) c->ghosts_valid = false;
     for (int q = 0; q < 7; ++q) {
         void* dq = blk + q * fb;
         if (to_device) {
@@ -263,27 +268,21 @@ extern "C" int mpcf_load_ghosts(mpcf_ctx* c)
     if (!c) return fail(MPCF_EINVAL, "mpcf_load_ghosts: NULL context");
     { Timed t(c, K_GHOSTS); c->L->ghosts(c->g, c->u[c->cur], c->bc, c->stream); }
     LAUNCH_CHECK("ghosts");
+    c->ghosts_valid = true;
     return MPCF_OK;
 }
 
 extern "C" int mpcf_max_sos(mpcf_ctx* c, float* sos)
 {
     if (!c || !sos) return fail(MPCF_EINVAL, "mpcf_max_sos: NULL argument");
-    { Timed t(c, K_SOS); c->L->sos(c->g, c->u[c->cur], c->st, (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0, c->stream); }
+    {
+        Timed t(c, K_SOS);
+        c->L->sos(c->g, c->u[c->cur], c->st, (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0, 0, 0.0, c->h, 0.0, c->stream);
+    }
     LAUNCH_CHECK("sos");
     HIP_TRY(hipMemcpyAsync(c->st_host, c->st, sizeof(StepState), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (c->flags & (MPCF_FLAG_FP64 | MPCF_FLAG_P64)) {
-        double d;
-        const unsigned long long bits = c->st_host->sos_bits_d;
-        std::memcpy(&d, &bits, sizeof(d));
-        *sos = (float)d;
-    } else {
-        float f;
-        const unsigned int bits = c->st_host->sos_bits;
-        std::memcpy(&f, &bits, sizeof(f));
-        *sos = f;
-    }
+    *sos = c->st_host->sos;
     return MPCF_OK;
 }
 
@@ -300,6 +299,7 @@ extern "C" int mpcf_update(mpcf_ctx* c, float b)
     if (!c) return fail(MPCF_EINVAL, "mpcf_update: NULL context");
     { Timed t(c, K_UPDATE); c->L->update(c->g, c->u[c->cur], c->R, b, c->stream); }
     LAUNCH_CHECK("update");
+    c->ghosts_valid = false;
     return MPCF_OK;
 }
 
@@ -309,6 +309,7 @@ extern "C" int mpcf_process_all(mpcf_ctx* c, float a, float b, float dtinvh)
     run_rhs(c, c->u[c->cur], c->u[1 - c->cur], a, b, dtinvh, nullptr, 1);
     LAUNCH_CHECK("process_all");
     c->cur = 1 - c->cur;
+    c->ghosts_valid = true;
     return MPCF_OK;
 }
 
@@ -317,15 +318,20 @@ extern "C" int mpcf_step(mpcf_ctx* c, double cfl, double dt_max, double* dt)
     if (!c) return fail(MPCF_EINVAL, "mpcf_step: NULL context");
     if (!(cfl > 0) || !std::isfinite(cfl)) return fail(MPCF_EINVAL, "mpcf_step: cfl must be positive and finite");
     if (!(dt_max > 0)) return fail(MPCF_EINVAL, "mpcf_step: dt_max must be positive");
-    { Timed t(c, K_SOS); c->L->sos(c->g, c->u[c->cur], c->st, (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0, c->stream); }
-    { Timed t(c, K_DT); c->L->dt(c->st, cfl, c->h, dt_max, c->stream); }
+    {
+        Timed t(c, K_SOS);
+        c->L->sos(c->g, c->u[c->cur], c->st, (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0, 1, cfl, c->h, dt_max, c->stream);
+    }
     LAUNCH_CHECK("sos/dt");
+    // ghosts are filled here only when the block was changed from outside a
+    // stage; every stage writes the ghosts of its output
+    if (!c->ghosts_valid) { Timed t(c, K_GHOSTS); c->L->ghosts(c->g, c->u[c->cur], c->bc, c->stream); }
     for (int s = 0; s < 3; ++s) {
-        { Timed t(c, K_GHOSTS); c->L->ghosts(c->g, c->u[c->cur], c->bc, c->stream); }
         run_rhs(c, c->u[c->cur], c->u[1 - c->cur], kA[s], kB[s], 0.0, c->st, 1);
         LAUNCH_CHECK("stage");
         c->cur = 1 - c->cur;
     }
+    c->ghosts_valid = true;
     if (dt) {
         double d;
         float sos;
@@ -543,11 +549,15 @@ int batch_pull_cur(mpcf_batch* b)
     return MPCF_OK;
 }
 
-int march_len(const mpcf_batch* b, int i, int nd, int which)
+// march length of case i in the x+y (which 0) or z sweep; `columns` is the
+// marching columns of all cases in that sweep
+int march_len(const mpcf_batch* b, int i, int nd, int which, long long columns)
 {
     const int forced = which == 0 ? b->chunk_xy[i] : b->chunk_z[i];
     auto it = b->tuned_len.find(nd);
-    return mpcf::detail::batch_march(nd, forced, it != b->tuned_len.end() ? it->second : 0);
+    const int tuned = it != b->tuned_len.end() ? it->second : 0;
+    if (forced > 0 || tuned > 0) return mpcf::detail::batch_march(nd, forced, tuned);
+    return mpcf::detail::adaptive_march(nd, columns);
 }
 
 int batch_rebuild(mpcf_batch* b)
@@ -573,10 +583,16 @@ int batch_rebuild(mpcf_batch* b)
     int* oz = &off[b->cap + 1];
     int* og = &off[2 * (b->cap + 1)];
     int* os = &off[3 * (b->cap + 1)];
+    long long cols_xy = 0, cols_z = 0;
+    for (mpcf_ctx* c : b->cases) {
+        cols_xy += (long long)(c->g.nx + 6) * c->g.nz;
+        cols_z += (long long)c->g.nx * c->g.ny;
+    }
     for (int i = 0; i < n; ++i) {
         mpcf_ctx* c = b->cases[i];
         int blocks[4];
-        mpcf::detail::case_desc(c, c->cur, march_len(b, i, c->g.ny, 0), march_len(b, i, c->g.nz, 1), desc[i], blocks);
+        mpcf::detail::case_desc(c, c->cur, march_len(b, i, c->g.ny, 0, cols_xy), march_len(b, i, c->g.nz, 1, cols_z),
+                                desc[i], blocks);
         oxy[i + 1] = oxy[i] + blocks[0];
         oz[i + 1] = oz[i] + blocks[1];
         og[i + 1] = og[i] + blocks[2];
@@ -600,14 +616,13 @@ int batch_rebuild(mpcf_batch* b)
 
 void batch_enqueue_step(mpcf_batch* b, double cfl, double dt_max)
 {
+    // every case enters with valid ghosts (mpcf_batch_add) and every stage
+    // writes the ghosts of its output, so a step is seven launches
     const int cpu_form = (b->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0;
-    b->L->b_sos(b->dev, b->tot_sos, cpu_form, b->stream);
-    b->L->b_dt(b->dev, cfl, dt_max, b->stream);
+    b->L->b_sos(b->dev, b->tot_sos, cpu_form, cfl, dt_max, b->stream);
     for (int s = 0; s < 3; ++s) {
-        b->L->b_ghosts(b->dev, b->tot_ghost, b->stream);
         b->L->b_xy(b->dev, b->tot_xy, kA[s], kB[s], b->stream);
         b->L->b_z(b->dev, b->tot_z, kA[s], kB[s], b->stream);
-        b->L->b_flip(b->dev, b->stream);
     }
 }
 
@@ -675,6 +690,8 @@ extern "C" int mpcf_batch_add(mpcf_batch* b, int nx, int ny, int nz, double h, c
         if (rc != MPCF_OK) { mpcf_destroy(c); return rc; }
     }
     rc = mpcf_upload(c, MPCF_DATA, fields);
+    if (rc == MPCF_OK) rc = mpcf_load_ghosts(c);
+    if (rc == MPCF_OK) rc = mpcf_synchronize(c);
     if (rc != MPCF_OK) { mpcf_destroy(c); return rc; }
     b->cases.push_back(c);
     b->ids.push_back(b->next_id);

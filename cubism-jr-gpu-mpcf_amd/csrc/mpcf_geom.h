@@ -62,7 +62,8 @@ inline Geom make_geom(int nx, int ny, int nz)
 }
 
 // device-resident step state: the max speed of sound of the last reduction,
-// the time step derived from it and the dt/h every stage reads.  Builds whose
+// the time step derived from it and the dt/h every stage reads.  The
+// reduction targets and counters must be zero when a block is created.  Builds whose
 // speed-of-sound reduction runs in binary64 (the binary64 scheme and the
 // selective-binary64 build) use the *_d members; the others the binary32 ones.
 struct StepState
@@ -76,6 +77,10 @@ struct StepState
     double dtinvh_d;      // dt/h in binary64
     double sos_d;         // max speed of sound as binary64
     unsigned long long sos_bits_d;   // atomicMax target (non-negative doubles order as u64)
+    // blocks of the running reduction / batched z sweep that have finished;
+    // the last one finalizes and puts the counter back to zero
+    unsigned int sos_count;
+    unsigned int z_count;
 };
 
 } // namespace mpcf

@@ -46,40 +46,27 @@ struct Launchers
 {
     decltype(&mpcf::parity::launch_ghosts) ghosts;
     decltype(&mpcf::parity::launch_sos) sos;
-    decltype(&mpcf::parity::launch_dt) dt;
     decltype(&mpcf::parity::launch_sweep) sweep;
     decltype(&mpcf::parity::launch_update) update;
     decltype(&mpcf::parity::launch_pack) pack;
     decltype(&mpcf::parity::launch_test_weno) test_weno;
     decltype(&mpcf::parity::launch_test_face) test_face;
     decltype(&mpcf::parity::batch_sos) b_sos;
-    decltype(&mpcf::parity::batch_dt) b_dt;
     decltype(&mpcf::parity::batch_ghosts) b_ghosts;
     decltype(&mpcf::parity::batch_xy) b_xy;
     decltype(&mpcf::parity::batch_z) b_z;
-    decltype(&mpcf::parity::batch_flip) b_flip;
 };
 
-inline const Launchers kParity = {
-    mpcf::parity::launch_ghosts, mpcf::parity::launch_sos, mpcf::parity::launch_dt, mpcf::parity::launch_sweep,
-    mpcf::parity::launch_update, mpcf::parity::launch_pack, mpcf::parity::launch_test_weno,
-    mpcf::parity::launch_test_face, mpcf::parity::batch_sos, mpcf::parity::batch_dt, mpcf::parity::batch_ghosts,
-    mpcf::parity::batch_xy, mpcf::parity::batch_z, mpcf::parity::batch_flip};
-inline const Launchers kFast = {
-    mpcf::fast::launch_ghosts, mpcf::fast::launch_sos, mpcf::fast::launch_dt, mpcf::fast::launch_sweep,
-    mpcf::fast::launch_update, mpcf::fast::launch_pack, mpcf::fast::launch_test_weno,
-    mpcf::fast::launch_test_face, mpcf::fast::batch_sos, mpcf::fast::batch_dt, mpcf::fast::batch_ghosts,
-    mpcf::fast::batch_xy, mpcf::fast::batch_z, mpcf::fast::batch_flip};
-inline const Launchers kFastP = {
-    mpcf::fastp::launch_ghosts, mpcf::fastp::launch_sos, mpcf::fastp::launch_dt, mpcf::fastp::launch_sweep,
-    mpcf::fastp::launch_update, mpcf::fastp::launch_pack, mpcf::fastp::launch_test_weno,
-    mpcf::fastp::launch_test_face, mpcf::fastp::batch_sos, mpcf::fastp::batch_dt, mpcf::fastp::batch_ghosts,
-    mpcf::fastp::batch_xy, mpcf::fastp::batch_z, mpcf::fastp::batch_flip};
-inline const Launchers kF64 = {
-    mpcf::f64::launch_ghosts, mpcf::f64::launch_sos, mpcf::f64::launch_dt, mpcf::f64::launch_sweep,
-    mpcf::f64::launch_update, mpcf::f64::launch_pack, mpcf::f64::launch_test_weno,
-    mpcf::f64::launch_test_face, mpcf::f64::batch_sos, mpcf::f64::batch_dt, mpcf::f64::batch_ghosts,
-    mpcf::f64::batch_xy, mpcf::f64::batch_z, mpcf::f64::batch_flip};
+#define MPCF_LAUNCHERS(NS)                                                                           \
+    {mpcf::NS::launch_ghosts, mpcf::NS::launch_sos, mpcf::NS::launch_sweep, mpcf::NS::launch_update, \
+     mpcf::NS::launch_pack, mpcf::NS::launch_test_weno, mpcf::NS::launch_test_face,                   \
+     mpcf::NS::batch_sos, mpcf::NS::batch_ghosts, mpcf::NS::batch_xy, mpcf::NS::batch_z}
+
+inline const Launchers kParity = MPCF_LAUNCHERS(parity);
+inline const Launchers kFast = MPCF_LAUNCHERS(fast);
+inline const Launchers kFastP = MPCF_LAUNCHERS(fastp);
+inline const Launchers kF64 = MPCF_LAUNCHERS(f64);
+#undef MPCF_LAUNCHERS
 
 inline const unsigned kAllFlags = MPCF_FLAG_FAST | MPCF_FLAG_SOS_CPU | MPCF_FLAG_SPLIT_XY | MPCF_FLAG_FUSED_XY |
                            MPCF_FLAG_P64 | MPCF_FLAG_FP64;
@@ -132,6 +119,7 @@ struct mpcf_ctx
     StepState* st_host; // pinned host mirror
     hipStream_t stream;
     bool own_stream;
+    bool ghosts_valid;  // the face ghosts of u[cur] match its interior and bc
     size_t bytes;
     const Launchers* L;
     // optional per-kernel event timing (mpcf_timing_enable)
@@ -179,6 +167,37 @@ inline void case_desc(const mpcf_ctx* c, const int cur, const int len_xy, const 
     d.nb_sos = rows4 < 2048 ? rows4 : 2048;
     blocks[3] = d.nb_sos;
     (void)cur;
+}
+
+}  // namespace detail
+}  // namespace mpcf
+
+namespace mpcf {
+namespace detail {
+
+// bit f set: face f is reflecting
+inline int reflect_mask(const int bc[6])
+{
+    int m = 0;
+    for (int f = 0; f < 6; ++f) m |= (bc[f] == MPCF_BC_REFLECTING) << f;
+    return m;
+}
+
+// march length of the batch kernels when neither forced nor tuned: split the
+// marched axis until the active cases together have about 2^16 marching
+// threads (one wave per SIMD), but march at least 8 cells.  Few small cases
+// then run short chains (a step's latency is the chain length times the
+// per-cell latency); many or large cases keep long marches, which recompute
+// fewer warm-up cells.  `columns` is the marching columns of all active cases
+// in this sweep.
+inline int adaptive_march(const int nd, const long long columns)
+{
+    const long long target = 1LL << 16;
+    long long nchunks = columns > 0 ? (target + columns - 1) / columns : 1;
+    if (nchunks < 1) nchunks = 1;
+    int len = (int)((nd + nchunks - 1) / nchunks);
+    if (len < 8) len = nd < 8 ? nd : 8;
+    return len;
 }
 
 }  // namespace detail

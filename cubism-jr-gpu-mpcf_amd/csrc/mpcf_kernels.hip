@@ -71,6 +71,8 @@ struct StageArgs
     int64_t fs;                    // field stride (elements)
     Real a, b, dtinvh;
     const StepState* st;           // if non-null, dtinvh is read from st->dtinvh
+    int ghosts;                    // MODE_STAGE: also write the face ghosts of uo
+    int reflect;                   // bit f set: face f is reflecting (else absorbing)
 };
 
 // operands of one cell's accumulator update, loaded one step ahead
@@ -95,11 +97,48 @@ __device__ __forceinline__ void emit_load(const StageArgs& A, const int e, EmitI
     }
 }
 
+// The face ghosts of the stage output, written by the cell they copy
+// (BoundaryConditions.h:60-109): absorbing -- the boundary cell fills all three
+// layers; reflecting -- cell j fills the layer at distance j+1 outside the
+// face, with the normal momentum negated.  Only face ghosts are ever read
+// (every stencil is one-directional), so edges and corners are left alone.
+struct CellPos { int x, y, z; };
+
+__device__ __forceinline__ void put_ghost(const StageArgs& A, const int ed, const Real* v, const int neg)
+{
+#pragma unroll
+    for (int q = 0; q < NVAR; ++q) A.uo[q*A.fs + ed] = q == neg ? -v[q] : v[q];
+}
+
+__device__ __forceinline__ void face_ghosts(const StageArgs& A, const int e, const Real* v, const int i, const int n,
+                                            const int stride, const int face_lo, const int neg)
+{
+    if (i < 3) {
+        if ((A.reflect >> face_lo) & 1) put_ghost(A, e - (2 * i + 1) * stride, v, neg);
+        else if (i == 0) { put_ghost(A, e - stride, v, -1); put_ghost(A, e - 2 * stride, v, -1); put_ghost(A, e - 3 * stride, v, -1); }
+    }
+    const int j = n - 1 - i;
+    if (j < 3) {
+        if ((A.reflect >> (face_lo + 1)) & 1) put_ghost(A, e + (2 * j + 1) * stride, v, neg);
+        else if (j == 0) { put_ghost(A, e + stride, v, -1); put_ghost(A, e + 2 * stride, v, -1); put_ghost(A, e + 3 * stride, v, -1); }
+    }
+}
+
+__device__ __forceinline__ void stage_ghosts(const StageArgs& A, const Geom& g, const int e, const CellPos& c,
+                                             const Real* v)
+{
+    face_ghosts(A, e, v, c.x, g.nx, 1, 0, FU);
+    face_ghosts(A, e, v, c.y, g.ny, g.px, 2, FV);
+    face_ghosts(A, e, v, c.z, g.nz, g.px * g.py, 4, FW);
+}
+
 // write the contribution of one cell.  F = flux at the cell's right face,
 // Fl = flux at its left face; cG/cP = plus-state + minus-state of G/P.
+// pos: the cell's coordinates (MODE_STAGE ghost writes only).
 template <int MODE>
 __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, const int e, const EmitIn<MODE>& in,
-                                     const FaceFlux& F, const FaceFlux& Fl, const Real cG, const Real cP)
+                                     const FaceFlux& F, const FaceFlux& Fl, const Real cG, const Real cP,
+                                     const Geom* g = nullptr, const CellPos* pos = nullptr)
 {
     Real* __restrict__ acc = A.acc;
     const int64_t fs = A.fs;
@@ -123,6 +162,7 @@ __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, cons
         const Real sP = in.acc[A_SUMP] + cP;
         const Real dU = in.acc[A_DIVU] + (F.vel - Fl.vel);
         const Real factor6 = Real(1.0f) / Real(6.0f);
+        Real v[NVAR];
 #pragma unroll
         for (int q = 0; q < NVAR; ++q) {
             const Real t = (in.acc[q] + F.f[q]) - Fl.f[q];
@@ -132,8 +172,12 @@ __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, cons
             else              rhs = dtinvh*t;
             const Real Rn = A.a*in.R[q] - rhs;
             A.R[q*fs + e] = Rn;
-            if (MODE == MODE_STAGE) A.uo[q*fs + e] = A.b*Rn + in.u[q];
+            if (MODE == MODE_STAGE) {
+                v[q] = A.b*Rn + in.u[q];
+                A.uo[q*fs + e] = v[q];
+            }
         }
+        if (MODE == MODE_STAGE && A.ghosts && pos) stage_ghosts(A, *g, e, *pos, v);
     }
 }
 
@@ -150,12 +194,13 @@ struct March
 
     // reconstruct cell k, solve face k-1/2, emit cell k-1 (operands in `in`)
     __device__ __forceinline__ void step(const StageArgs& A, const Real dtinvh, const int k, const int c0, const int e_km1,
-                                         const EmitIn<MODE>& in, const bool store = true)
+                                         const EmitIn<MODE>& in, const bool store = true,
+                                         const Geom* g = nullptr, const CellPos* pos = nullptr)
     {
         const CellRec r = reconstruct<Real, kFast>(w0, w1, w2, w3, w4);
         if (k >= c0) {
             const FaceFlux F = riemann<D, Real, kFast>(pm, r.p);
-            if (k >= c0 + 1 && store) emit<MODE>(A, dtinvh, e_km1, in, F, Fl, pGp + pm.G, pPp + pm.P);
+            if (k >= c0 + 1 && store) emit<MODE>(A, dtinvh, e_km1, in, F, Fl, pGp + pm.G, pPp + pm.P, g, pos);
             Fl = F;
         }
         pm = r.m; pGp = r.p.G; pPp = r.p.P;
@@ -186,7 +231,7 @@ __device__ __forceinline__ void sweep_yz_body(const Geom& g, const StageArgs& A,
     const int nt = D == 1 ? g.nz : g.ny;
     const int nd = D == 1 ? g.ny : g.nz;
     const int col = bcol * 256 + threadIdx.x;
-    if (col >= g.nx * nt) return;   // no barriers below
+    if (col >= g.nx * nt) return;   // no barriers in the body
     const int x = col % g.nx;
     const int t = col / g.nx;
     const int c0 = bz * chunk;
@@ -231,7 +276,12 @@ __device__ __forceinline__ void sweep_yz_body(const Geom& g, const StageArgs& A,
             for (int f = 0; f < NVAR; ++f) ein.u[f] = r0[f];                 // cell k-1
         }
         m.push(q);
-        m.step(A, dtinvh, k, c0, e0 + (k - 1) * s, ein);
+        if (D == 2) {
+            const CellPos pos{x, t, k - 1};
+            m.step(A, dtinvh, k, c0, e0 + (k - 1) * s, ein, true, &g, &pos);
+        } else {
+            m.step(A, dtinvh, k, c0, e0 + (k - 1) * s, ein);
+        }
         ein = ein_next;
     }
 }
@@ -514,19 +564,54 @@ void k_ghosts(const Geom g, Real* __restrict__ u, const int64_t fs, const BCSet 
 }
 
 // ---------------------------------------------------------------------------
-// max speed of sound: reset, reduce, finalize dt (GPUkernels.cu:1182-1222,
-// MaxSpeedOfSound.cpp:18-55, LSRK3_IntegratorMPI.cpp:37-38)
+// max speed of sound and dt in one launch (GPUkernels.cu:1182-1222,
+// MaxSpeedOfSound.cpp:18-55, LSRK3_IntegratorMPI.cpp:37-38): every block
+// folds its maximum into the step state with an atomic max, then counts
+// itself done; the last block reads the maximum, derives dt and dt/h, and puts
+// the reduction target and the counter back to zero for the next step.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void sos_reset(StepState* st) { st->sos_bits = 0u; st->sos_bits_d = 0ull; }
-
-__global__ void k_sos_reset(StepState* st) { sos_reset(st); }
 
 // scalar type of the speed-of-sound reduction
 typedef std::conditional<kSosD, double, float>::type SosT;
 
+struct DtArgs
+{
+    int with_dt;                  // 0: only the maximum (mpcf_max_sos)
+    double cfl, h, dt_max;
+};
+
+__device__ __forceinline__ void sos_finalize(StepState* st, const DtArgs& d)
+{
+    double sos;
+    if (kSosD) {
+        const unsigned long long bits = atomicOr(&st->sos_bits_d, 0ull);   // coherent read
+        sos = __longlong_as_double((long long)bits);
+        st->sos_d = sos;
+        st->sos = (float)sos;
+        atomicExch(&st->sos_bits_d, 0ull);
+    } else {
+        const float f = __uint_as_float(atomicOr(&st->sos_bits, 0u));
+        sos = f;
+        st->sos = f;
+        st->sos_d = f;
+        atomicExch(&st->sos_bits, 0u);
+    }
+    st->sos_is_d = kSosD ? 1 : 0;
+    if (d.with_dt) {
+        // binary64 dt from the binary32 (or binary64) maximum
+        double dt = d.cfl*d.h/sos;
+        dt = d.dt_max < dt ? d.dt_max : dt;
+        st->dt = dt;
+        st->h = d.h; st->cfl = d.cfl; st->dt_max = d.dt_max;
+        st->dtinvh = (float)(dt/d.h);
+        st->dtinvh_d = dt/d.h;
+    }
+    atomicExch(&st->sos_count, 0u);
+}
+
 template <int CPU_FORM>
 __device__ __forceinline__ void sos_body(const Geom& g, const Real* __restrict__ u, const int64_t fs, StepState* st,
-                                         const int b, const int nb, SosT* wmax)
+                                         const int b, const int nb, SosT* wmax, const DtArgs& d)
 {
     const int rows = g.ny * g.nz;
     SosT sos = 0.0f;
@@ -551,42 +636,20 @@ __device__ __forceinline__ void sos_body(const Geom& g, const Real* __restrict__
         const SosT m = mmax(mmax(wmax[0], wmax[1]), mmax(wmax[2], wmax[3]));
         if (kSosD) atomicMax(&st->sos_bits_d, (unsigned long long)__double_as_longlong((double)m));
         else       atomicMax(&st->sos_bits, __float_as_uint((float)m));
+        __threadfence();
+        if (atomicAdd(&st->sos_count, 1u) == (unsigned)nb - 1u) {
+            __threadfence();
+            sos_finalize(st, d);
+        }
     }
 }
 
 template <int CPU_FORM>
 __global__ __launch_bounds__(256)
-void k_sos(const Geom g, const Real* __restrict__ u, const int64_t fs, StepState* st)
+void k_sos(const Geom g, const Real* __restrict__ u, const int64_t fs, StepState* st, const DtArgs d)
 {
     __shared__ SosT wmax[4];
-    sos_body<CPU_FORM>(g, u, fs, st, blockIdx.x, gridDim.x, wmax);
-}
-
-__device__ __forceinline__ void dt_body(StepState* st, const double cfl, const double h, const double dt_max)
-{
-    double dt;
-    if (kSosD) {
-        const double sos = __longlong_as_double((long long)st->sos_bits_d);
-        st->sos_d = sos;
-        st->sos = (float)sos;
-        dt = cfl*h/sos;
-    } else {
-        const float sos = __uint_as_float(st->sos_bits);
-        st->sos = sos;
-        st->sos_d = sos;
-        dt = cfl*h/sos;
-    }
-    st->sos_is_d = kSosD ? 1 : 0;
-    dt = dt_max < dt ? dt_max : dt;
-    st->dt = dt;
-    st->h = h; st->cfl = cfl; st->dt_max = dt_max;
-    st->dtinvh = (float)(dt/h);
-    st->dtinvh_d = dt/h;
-}
-
-__global__ void k_dt_finalize(StepState* st, const double cfl, const double h, const double dt_max)
-{
-    dt_body(st, cfl, h, dt_max);
+    sos_body<CPU_FORM>(g, u, fs, st, blockIdx.x, gridDim.x, wmax, d);
 }
 
 // ---------------------------------------------------------------------------
@@ -677,32 +740,24 @@ __device__ __forceinline__ int find_case(const int* __restrict__ off, const int 
 __device__ __forceinline__ StageArgs case_args(const CaseDesc& C, const int cur, const Real a, const Real b)
 {
     StageArgs A;
+    A.ghosts = 1;
+    A.reflect = 0;
+    for (int f = 0; f < 6; ++f) A.reflect |= (C.bc[f] == 1) << f;
     A.u = (const Real*)C.u[cur]; A.R = (Real*)C.R; A.acc = (Real*)C.acc; A.uo = (Real*)C.u[1 - cur];
     A.fs = C.g.fstride;
     A.a = a; A.b = b; A.dtinvh = Real(0.0f); A.st = C.st;
     return A;
 }
 
-__global__ void kb_sos_reset(const BatchDev B)
-{
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < B.n) sos_reset(B.cases[c].st);
-}
-
 template <int CPU_FORM>
 __global__ __launch_bounds__(256)
-void kb_sos(const BatchDev B)
+void kb_sos(const BatchDev B, const double cfl, const double dt_max)
 {
     __shared__ SosT wmax[4];
     const int c = find_case(B.off_sos, B.n, blockIdx.x);
     const CaseDesc& C = B.cases[c];
-    sos_body<CPU_FORM>(C.g, (const Real*)C.u[B.cur[c]], C.g.fstride, C.st, blockIdx.x - B.off_sos[c], C.nb_sos, wmax);
-}
-
-__global__ void kb_dt(const BatchDev B, const double cfl, const double dt_max)
-{
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < B.n) dt_body(B.cases[c].st, cfl, B.cases[c].h, dt_max);
+    const DtArgs d{1, cfl, C.h, dt_max};
+    sos_body<CPU_FORM>(C.g, (const Real*)C.u[B.cur[c]], C.g.fstride, C.st, blockIdx.x - B.off_sos[c], C.nb_sos, wmax, d);
 }
 
 __global__ __launch_bounds__(256)
@@ -724,6 +779,8 @@ void kb_sweep_xy(const BatchDev B, const Real a, const Real b)
     sweep_xy_body(C.g, A, C.chunk_xy, local % C.nbx_xy, local / C.nbx_xy);
 }
 
+// the z sweep finishes the stage; the last block of each case to finish
+// flips the case's current-buffer index (every block has read it by then)
 __global__ __launch_bounds__(256)
 void kb_sweep_z(const BatchDev B, const Real a, const Real b)
 {
@@ -732,12 +789,15 @@ void kb_sweep_z(const BatchDev B, const Real a, const Real b)
     const int local = blockIdx.x - B.off_z[c];
     const StageArgs A = case_args(C, B.cur[c], a, b);
     sweep_yz_body<2, MODE_STAGE>(C.g, A, C.chunk_z, local % C.nbx_z, local / C.nbx_z);
-}
-
-__global__ void kb_flip(const BatchDev B)
-{
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c < B.n) B.cur[c] = 1 - B.cur[c];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const unsigned nblk = (unsigned)(B.off_z[c + 1] - B.off_z[c]);
+        if (atomicAdd(&C.st->z_count, 1u) == nblk - 1u) {
+            atomicExch(&B.cur[c], 1 - B.cur[c]);
+            atomicExch(&C.st->z_count, 0u);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -772,18 +832,14 @@ void launch_ghosts(const Geom& g, void* uv, const int bc[6], hipStream_t s)
     hipLaunchKernelGGL(k_ghosts, dim3((tmax + 255) / 256, 6), dim3(256), 0, s, g, (Real*)uv, g.fstride, b);
 }
 
-void launch_sos(const Geom& g, const void* uv, StepState* st, const int cpu_form, hipStream_t s)
+void launch_sos(const Geom& g, const void* uv, StepState* st, const int cpu_form, const int with_dt, const double cfl,
+                const double h, const double dt_max, hipStream_t s)
 {
     const Real* u = (const Real*)uv;
-    hipLaunchKernelGGL(k_sos_reset, dim3(1), dim3(1), 0, s, st);
     const int nb = rows_grid(g) < 2048 ? rows_grid(g) : 2048;
-    if (cpu_form) hipLaunchKernelGGL(k_sos<1>, dim3(nb), dim3(64, 4), 0, s, g, u, g.fstride, st);
-    else          hipLaunchKernelGGL(k_sos<0>, dim3(nb), dim3(64, 4), 0, s, g, u, g.fstride, st);
-}
-
-void launch_dt(StepState* st, const double cfl, const double h, const double dt_max, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_dt_finalize, dim3(1), dim3(1), 0, s, st, cfl, h, dt_max);
+    const DtArgs d{with_dt, cfl, h, dt_max};
+    if (cpu_form) hipLaunchKernelGGL(k_sos<1>, dim3(nb), dim3(64, 4), 0, s, g, u, g.fstride, st, d);
+    else          hipLaunchKernelGGL(k_sos<0>, dim3(nb), dim3(64, 4), 0, s, g, u, g.fstride, st, d);
 }
 
 // one direction sweep.  dir 0 (x) assigns the accumulators, dir 1 (y)
@@ -795,6 +851,8 @@ void launch_sweep(const Geom& g, const int dir, const SweepParams& p, const int 
     StageArgs A;
     A.u = (const Real*)p.u; A.R = (Real*)p.R; A.acc = (Real*)p.acc; A.uo = (Real*)p.uo; A.fs = g.fstride;
     A.a = (Real)p.a; A.b = (Real)p.b; A.dtinvh = (Real)p.dtinvh; A.st = p.st;
+    A.ghosts = p.ghosts;
+    A.reflect = p.reflect;
     if (dir == 3) {
         const int chunk = pick_chunk(g.ny, (int64_t)g.nx * g.nz, 1);
         int dims[3];
@@ -872,16 +930,11 @@ void z_grid(const Geom& g, const int chunk, int dims[3])
     dims[0] = (g.nx * g.ny + 255) / 256; dims[1] = 1; dims[2] = (g.nz + chunk - 1) / chunk;
 }
 
-void batch_sos(const BatchDev& B, const int blocks, const int cpu_form, hipStream_t s)
+void batch_sos(const BatchDev& B, const int blocks, const int cpu_form, const double cfl, const double dt_max,
+               hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_sos_reset, dim3((B.n + 63) / 64), dim3(64), 0, s, B);
-    if (cpu_form) hipLaunchKernelGGL(kb_sos<1>, dim3(blocks), dim3(64, 4), 0, s, B);
-    else          hipLaunchKernelGGL(kb_sos<0>, dim3(blocks), dim3(64, 4), 0, s, B);
-}
-
-void batch_dt(const BatchDev& B, const double cfl, const double dt_max, hipStream_t s)
-{
-    hipLaunchKernelGGL(kb_dt, dim3((B.n + 63) / 64), dim3(64), 0, s, B, cfl, dt_max);
+    if (cpu_form) hipLaunchKernelGGL(kb_sos<1>, dim3(blocks), dim3(64, 4), 0, s, B, cfl, dt_max);
+    else          hipLaunchKernelGGL(kb_sos<0>, dim3(blocks), dim3(64, 4), 0, s, B, cfl, dt_max);
 }
 
 void batch_ghosts(const BatchDev& B, const int blocks, hipStream_t s)
@@ -897,11 +950,6 @@ void batch_xy(const BatchDev& B, const int blocks, const double a, const double 
 void batch_z(const BatchDev& B, const int blocks, const double a, const double b, hipStream_t s)
 {
     hipLaunchKernelGGL(kb_sweep_z, dim3(blocks), dim3(256), 0, s, B, (Real)a, (Real)b);
-}
-
-void batch_flip(const BatchDev& B, hipStream_t s)
-{
-    hipLaunchKernelGGL(kb_flip, dim3((B.n + 63) / 64), dim3(64), 0, s, B);
 }
 
 } // namespace MPCF_NS

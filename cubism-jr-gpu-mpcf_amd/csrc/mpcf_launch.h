@@ -19,6 +19,8 @@ struct SweepParams
     double a, b, dtinvh;   // stage coefficients, rounded to the storage type by
                            // the launcher; dtinvh ignored when st != NULL
     const StepState* st;   // device step state carrying dt/h
+    int ghosts;            // stage == 1: also write the face ghosts of uo
+    int reflect;           // bit f set: face f reflecting (else absorbing)
 };
 
 // one case of a batched ensemble, as the batch kernels see it
@@ -56,19 +58,17 @@ struct BatchDev
 #define MPCF_DECLARE_LAUNCHERS(NS)                                                                         \
 namespace mpcf { namespace NS {                                                                            \
 void launch_ghosts(const Geom& g, void* u, const int bc[6], hipStream_t s);                                \
-void launch_sos(const Geom& g, const void* u, StepState* st, int cpu_form, hipStream_t s);                  \
-void launch_dt(StepState* st, double cfl, double h, double dt_max, hipStream_t s);                         \
+void launch_sos(const Geom& g, const void* u, StepState* st, int cpu_form, int with_dt, double cfl,          \
+                double h, double dt_max, hipStream_t s);                                                   \
 void launch_sweep(const Geom& g, int dir, const SweepParams& p, int stage, hipStream_t s);                \
 void launch_update(const Geom& g, void* u, const void* R, double b, hipStream_t s);                        \
 void launch_pack(const Geom& g, void* pad, void* dense, int to_padded, int nq, int dense_f64, hipStream_t s); \
 void launch_test_weno(int n, const void* in5, void* m, void* p, hipStream_t s);                            \
 void launch_test_face(int n, int dir, const void* st, void* out, hipStream_t s);                           \
-void batch_sos(const BatchDev& B, int blocks, int cpu_form, hipStream_t s);                                \
-void batch_dt(const BatchDev& B, double cfl, double dt_max, hipStream_t s);                                \
+void batch_sos(const BatchDev& B, int blocks, int cpu_form, double cfl, double dt_max, hipStream_t s);    \
 void batch_ghosts(const BatchDev& B, int blocks, hipStream_t s);                                           \
 void batch_xy(const BatchDev& B, int blocks, double a, double b, hipStream_t s);                           \
 void batch_z(const BatchDev& B, int blocks, double a, double b, hipStream_t s);                            \
-void batch_flip(const BatchDev& B, hipStream_t s);                                                         \
 void xy_grid(const Geom& g, int chunk, int dims[3]);                                                       \
 void z_grid(const Geom& g, int chunk, int dims[3]);                                                        \
 int default_chunk(int nd, long long columns);                                                              \

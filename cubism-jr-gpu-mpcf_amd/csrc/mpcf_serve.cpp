@@ -19,7 +19,8 @@
 //   launch per kernel kind per stage (the batch kernels of mpcf_kernels.hip),
 //   captured as a hipGraph while the subset is unchanged.  Which cases are
 //   active is the policy: all live cases; or, filled up to a cell budget, the
-//   oldest first or those with the least remaining work first.
+//   oldest first, those with the least remaining work first, or the oldest
+//   one and then the least remaining work.
 // * Each case's current-buffer index is mirrored on the host (a step flips it
 //   once), so membership changes need no read-back.  At most `depth` steps are
 //   in flight; the engine waits on the oldest beyond that.
@@ -311,7 +312,10 @@ bool progress(mpcf_server* S, bool* changed)
             if (hipMemcpyAsync(c->acc, S->staging[r->stage_buf].p, 7 * n * sizeof(float), hipMemcpyHostToDevice,
                                S->stream) != hipSuccess) return false;
             S->L->pack(c->g, c->u[0], c->acc, 1, 7, 0, S->stream);
+            S->L->ghosts(c->g, c->u[0], c->bc, S->stream);
+            // R starts at zero; the step state's reduction targets and counters too
             if (hipMemsetAsync(c->R, 0, 7 * (size_t)c->g.fstride * sizeof(float), S->stream) != hipSuccess) return false;
+            if (hipMemsetAsync(c->st, 0, sizeof(StepState), S->stream) != hipSuccess) return false;
             r->ev = take_event(S);
             if (!r->ev || hipEventRecord(r->ev, S->stream) != hipSuccess) return false;
             r->state = S_LIVE;
@@ -382,12 +386,18 @@ void choose_active(mpcf_server* S, std::vector<Req*>& act)
     if (S->policy == MPCF_SERVE_ALL || live.empty()) {
         act = live;
     } else {
-        if (S->policy == MPCF_SERVE_SRPT_FILL)
-            std::stable_sort(live.begin(), live.end(), [](const Req* a, const Req* b) {
-                return a->cells() * a->left < b->cells() * b->left;
-            });
-        else
-            std::stable_sort(live.begin(), live.end(), [](const Req* a, const Req* b) { return a->seq < b->seq; });
+        auto by_age = [](const Req* a, const Req* b) { return a->seq < b->seq; };
+        auto by_work = [](const Req* a, const Req* b) { return a->cells() * a->left < b->cells() * b->left; };
+        if (S->policy == MPCF_SERVE_FIFO_FILL) {
+            std::stable_sort(live.begin(), live.end(), by_age);
+        } else {
+            std::stable_sort(live.begin(), live.end(), by_work);
+            if (S->policy == MPCF_SERVE_HYBRID) {
+                // the oldest request always steps, so no request starves
+                auto oldest = std::min_element(live.begin(), live.end(), by_age);
+                std::rotate(live.begin(), oldest, oldest + 1);
+            }
+        }
         long long cells = 0;
         for (Req* r : live) {
             if (!act.empty() && cells + r->cells() > S->fill_cells) continue;
@@ -440,12 +450,16 @@ bool rebuild(mpcf_server* S)
     int* og = oxy + 2 * (cap + 1);
     int* os = oxy + 3 * (cap + 1);
     oxy[0] = oz[0] = og[0] = os[0] = 0;
-    long long cells = 0;
+    long long cells = 0, cols_xy = 0, cols_z = 0;
+    for (Req* r : S->active) {
+        cols_xy += (long long)(r->c->g.nx + 6) * r->c->g.nz;
+        cols_z += (long long)r->c->g.nx * r->c->g.ny;
+    }
     for (int i = 0; i < n; ++i) {
         mpcf_ctx* c = S->active[i]->c;
         int blocks[4];
-        mpcf::detail::case_desc(c, c->cur, mpcf::detail::batch_march(c->g.ny, 0, 0),
-                                mpcf::detail::batch_march(c->g.nz, 0, 0), hc[i], blocks);
+        mpcf::detail::case_desc(c, c->cur, mpcf::detail::adaptive_march(c->g.ny, cols_xy),
+                                mpcf::detail::adaptive_march(c->g.nz, cols_z), hc[i], blocks);
         oxy[i + 1] = oxy[i] + blocks[0];
         oz[i + 1] = oz[i] + blocks[1];
         og[i + 1] = og[i] + blocks[2];
@@ -472,14 +486,13 @@ bool rebuild(mpcf_server* S)
 
 void enqueue_step_kernels(mpcf_server* S)
 {
+    // cases enter with their ghosts filled and every stage writes the ghosts
+    // of its output: seven launches per step
     const int cpu_form = (S->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0;
-    S->L->b_sos(S->dev, S->tot[3], cpu_form, S->stream);
-    S->L->b_dt(S->dev, S->cfl, 1e30, S->stream);
+    S->L->b_sos(S->dev, S->tot[3], cpu_form, S->cfl, 1e30, S->stream);
     for (int s = 0; s < 3; ++s) {
-        S->L->b_ghosts(S->dev, S->tot[2], S->stream);
         S->L->b_xy(S->dev, S->tot[0], kA[s], kB[s], S->stream);
         S->L->b_z(S->dev, S->tot[1], kA[s], kB[s], S->stream);
-        S->L->b_flip(S->dev, S->stream);
     }
 }
 
@@ -607,7 +620,7 @@ extern "C" int mpcf_server_create(mpcf_server** out, unsigned flags, long long h
     *out = nullptr;
     if (flags & ~(unsigned)(MPCF_FLAG_FAST | MPCF_FLAG_SOS_CPU | MPCF_FLAG_P64 | MPCF_FLAG_SERVE_EAGER))
         return fail(MPCF_EINVAL, "mpcf_server_create: unsupported flag (binary32 storage only)");
-    if (policy < MPCF_SERVE_ALL || policy > MPCF_SERVE_SRPT_FILL) return fail(MPCF_EINVAL, "mpcf_server_create: bad policy");
+    if (policy < MPCF_SERVE_ALL || policy > MPCF_SERVE_HYBRID) return fail(MPCF_EINVAL, "mpcf_server_create: bad policy");
     if (hbm_bytes <= 0 || max_live < 1 || !(cfl > 0)) return fail(MPCF_EINVAL, "mpcf_server_create: bad budget / slots / cfl");
     if (flags & MPCF_FLAG_P64) flags |= MPCF_FLAG_FAST;
     mpcf_server* S = new mpcf_server();

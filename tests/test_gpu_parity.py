@@ -155,6 +155,31 @@ def test_lsrk3_steps_bitexact(mpcf, oracle, cpu_sos, split):
     assert_equal(R, ob.tmp, "R after 3 steps")
 
 
+@pytest.mark.parametrize("shape,bc", [((5, 7, 6), [1] * 6), ((6, 5, 9), [0, 1, 1, 0, 0, 1]), ((9, 6, 5), [1, 0, 0, 1, 1, 1])],
+                         ids=["x5-reflect", "y5-mixed", "z5-mixed"])
+def test_lsrk3_small_extents_bitexact(mpcf, oracle, shape, bc):
+    """Extents of 5-6 cells: the stage outputs write the ghosts of both faces
+    of an axis from overlapping boundary cells."""
+    nx, ny, nz = shape
+    h = 1.0 / max(shape)
+    s = states.random_smooth(nx, ny, nz, seed=13)
+    say("lsrk3 small extents", shape, bc)
+    ob = oracle.Block(nx, ny, nz)
+    ob.data = s
+    for split in (False, True):
+        ob.data = s
+        ob.tmp = np.zeros_like(s)
+        with mpcf.Block(nx, ny, nz, h, bc=bc, split_xy=split) as b:
+            b.data = s
+            for it in range(4):
+                dt = b.step(0.3)
+                dref, _ = ob.step(h, 0.3, 1e30, bc)
+                assert dt == dref, (split, it, dt, dref)
+            u, R = b.data, b.tmp
+        assert_equal(u, ob.data, f"u after 4 steps (split {split})")
+        assert_equal(R, ob.tmp, f"R after 4 steps (split {split})")
+
+
 def test_dt_max_clamp(mpcf, oracle):
     nx = ny = nz = 16
     s = states.uniform(nx, ny, nz, rho=1.0, u=0.1, p=1.0)

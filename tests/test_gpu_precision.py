@@ -118,3 +118,19 @@ def test_mass_conserved_in_closed_box(mpcf, prec):
         m1 = b.integrals()[0]
     say(prec, "mass drift", (m1 - m0) / m0)
     assert abs(m1 - m0) <= tol * abs(m0), (m0, m1)
+
+
+def test_precision_driver_small_cloud(mpcf):
+    """The precision report runs end to end on a small bubble cloud: every
+    variant follows the binary64 dt sequence exactly (the driver raises if
+    not), and the errors and each variant's own dt stay small."""
+    import precision
+    say("precision driver 48^3 x 10 steps")
+    res = precision.run(n=48, steps=10, nbubbles=6, seed=3)
+    for name, v in res["variants"].items():
+        for f in precision.FIELDS:
+            # the momenta start from rest, so their relative error is taken
+            # against a small field and is larger
+            bound = 1e-2 if f.startswith("rho_") else 1e-4
+            assert np.isfinite(v["rel_l1"][f]) and v["rel_l1"][f] < bound, (name, f, v["rel_l1"][f])
+        assert v["dt_own_rel_err_max"] < 1e-5, (name, v["dt_own_rel_err_max"])
