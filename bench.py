@@ -72,11 +72,11 @@ def parse():
     p.add_argument("--no-serving", action="store_true", help="skip the served-stream comparison")
     p.add_argument("--serving-requests", type=int, default=48)
     p.add_argument("--serving-policy", default="fifo_fill", choices=("all", "fifo_fill", "srpt_fill"))
-    p.add_argument("--serving-fill", type=int, default=4 << 20, help="active-subset cell budget of the *_fill policies")
+    p.add_argument("--serving-fill", type=int, default=10 << 20, help="active-subset cell budget of the *_fill policies")
     p.add_argument("--no-precision", action="store_true", help="skip the binary32-vs-binary64 leg")
     p.add_argument("--precision-n", type=int, default=128)
     p.add_argument("--precision-steps", type=int, default=50)
-    p.add_argument("--serving-load", type=float, nargs="+", default=[0.7, 1.1],
+    p.add_argument("--serving-load", type=float, nargs="+", default=[0.7, 1.2],
                    help="offered load as a fraction of the batched ensemble throughput")
     return p.parse_args()
 
@@ -106,19 +106,24 @@ def cpu_baseline(n: int):
 
 def ensemble_leg(a):
     """64 bubble-collapse cases (edges {32..128}), 100 steps each: batched
-    (one launch per kernel kind, graph-captured steps, autotuned march
-    length) against the plain one-case-at-a-time solver."""
+    (all cases together, one launch per kernel kind, graph-captured steps,
+    autotuned march length), the same batch kernels through the serving
+    engine with least-remaining-work-first ordering, and the plain
+    one-case-at-a-time solver."""
     import ensemble as E
     cases = E.ensemble_cases(a.ensemble_cases, a.ensemble_steps, seed=0)
     work = sum(c.work for c in cases)
     tb, mb, _ = E.run_batched(cases, cfl=a.cfl, fast=True, graph=True, autotune=True)
     log(f"ensemble batched: {mb:.2f} s")
+    ts, ms, _ = E.run_served(cases, cfl=a.cfl, fast=True, policy="srpt_fill", fill_cells=16 << 20)
+    log(f"ensemble served (least remaining work first): {ms:.2f} s")
     tp, mp, _ = E.run_plain(cases, cfl=a.cfl, fast=True)
     log(f"ensemble plain: {mp:.2f} s")
     return {
         "workload": f"{len(cases)} bubble-collapse cases, edges in {{32,48,64,96,128}}, {a.ensemble_steps} LSRK3 steps each, FP32",
         "cell_updates": work,
         "batched": dict(cell_updates_per_s=work / mb, makespan_s=mb, **E.latency_stats(list(tb.values()))),
+        "served_srpt": dict(cell_updates_per_s=work / ms, makespan_s=ms, **E.latency_stats(list(ts.values()))),
         "plain": dict(cell_updates_per_s=work / mp, makespan_s=mp, **E.latency_stats(list(tp.values()))),
     }
 
@@ -129,12 +134,13 @@ def serving_leg(a):
     continuous-batching server (mpcf_server) and by the plain sequential
     solver; time-to-solution = completion - arrival."""
     import ensemble as E
-    # the plain solver's capacity on this request mix: mean service time of a
-    # sample of requests run back to back
-    probe = E.stream_cases(12, seed=2, rate=1.0)
-    _, mk, _ = E.run_plain(probe, cfl=a.cfl, fast=True)
-    plain_rate = len(probe) / mk
-    log(f"plain solver capacity on the request mix: {plain_rate:.2f} requests/s")
+    # the plain solver's capacity on this request list: its requests run back
+    # to back (request sizes do not depend on the arrival rate; only the
+    # arrival times scale with it)
+    base = E.stream_cases(a.serving_requests, seed=1, rate=1.0)
+    _, mk, _ = E.run_plain(base, cfl=a.cfl, fast=True)
+    plain_rate = len(base) / mk
+    log(f"plain solver capacity on the request list: {plain_rate:.2f} requests/s")
     out = {"workload": f"{a.serving_requests} requests per load, edges 32..192 step 16, 50..500 steps, "
                        f"single-bubble collapse, FP32; Poisson arrivals",
            "plain_capacity_per_s": plain_rate, "policy": a.serving_policy, "fill_cells": a.serving_fill,

@@ -91,7 +91,6 @@ void run_rhs(mpcf_ctx* c, const void* u, void* uo, double a, double b, double dt
     mpcf::SweepParams p;
     p.u = u; p.R = c->R; p.acc = c->acc; p.uo = uo;
     p.a = a; p.b = b; p.dtinvh = dtinvh; p.st = st;
-    p.ghosts = stage;                       // a stage output carries its own ghosts
     p.reflect = mpcf::detail::reflect_mask(c->bc);
     // the fused x+y sweep wins when the arithmetic is cheap (FAST); the split
     // pair wins when it is VALU-bound (bit-exact build), unless forced
@@ -145,16 +144,20 @@ extern "C" int mpcf_create(mpcf_ctx** out, int nx, int ny, int nz, double h, uns
         delete c;
         return fail(MPCF_ENOMEM, std::string("hipMalloc of block state: ") + hipGetErrorString(e));
     }
-    e = hipMemset(c->base, 0, c->bytes);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&c->st_host, sizeof(StepState), hipHostMallocDefault);
+    e = hipHostMalloc((void**)&c->st_host, sizeof(StepState), hipHostMallocDefault);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+    // zero the state on the context's own stream and wait: a non-blocking
+    // stream is not ordered after work on the null stream, so a plain
+    // hipMemset could land after the first upload or stage (freed memory of
+    // an earlier block may hold NaN, and R must start at zero)
+    if (e == hipSuccess) e = hipMemsetAsync(c->base, 0, c->bytes, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
         hipFree(c->base);
         delete c;
         return fail(MPCF_EHIP, std::string("mpcf_create: ") + hipGetErrorString(e));
     }
     c->own_stream = true;
-    c->ghosts_valid = false;
     const size_t fb = (size_t)g.fstride * c->esize;   // bytes per field (a multiple of 256)
     c->u[0] = c->base;
     c->u[1] = c->base + 7 * fb;
@@ -204,7 +207,6 @@ extern "C" int mpcf_set_bc(mpcf_ctx* c, const int bc[6])
         if (bc[i] != MPCF_BC_ABSORBING && bc[i] != MPCF_BC_REFLECTING)
             return fail(MPCF_EINVAL, "mpcf_set_bc: unknown boundary kind");
     for (int i = 0; i < 6; ++i) c->bc[i] = bc[i];
-    c->ghosts_valid = false;
     return MPCF_OK;
 }
 
@@ -224,7 +226,6 @@ int transfer(mpcf_ctx* c, const int which, void* const* fields, const int host_f
     const size_t hb = host_f64 ? sizeof(double) : sizeof(float);
     const size_t fb = (size_t)c->g.fstride * c->esize;
     char* blk = (char*)(which == MPCF_DATA ? c->u[c->cur] : c->R);
-    if (to_device && which == MPCF_DATA) c->ghosts_valid = false;
     for (int q = 0; q < 7; ++q) {
         void* dq = blk + q * fb;
         if (to_device) {
@@ -268,7 +269,6 @@ extern "C" int mpcf_load_ghosts(mpcf_ctx* c)
     if (!c) return fail(MPCF_EINVAL, "mpcf_load_ghosts: NULL context");
     { Timed t(c, K_GHOSTS); c->L->ghosts(c->g, c->u[c->cur], c->bc, c->stream); }
     LAUNCH_CHECK("ghosts");
-    c->ghosts_valid = true;
     return MPCF_OK;
 }
 
@@ -299,7 +299,6 @@ extern "C" int mpcf_update(mpcf_ctx* c, float b)
     if (!c) return fail(MPCF_EINVAL, "mpcf_update: NULL context");
     { Timed t(c, K_UPDATE); c->L->update(c->g, c->u[c->cur], c->R, b, c->stream); }
     LAUNCH_CHECK("update");
-    c->ghosts_valid = false;
     return MPCF_OK;
 }
 
@@ -309,7 +308,6 @@ extern "C" int mpcf_process_all(mpcf_ctx* c, float a, float b, float dtinvh)
     run_rhs(c, c->u[c->cur], c->u[1 - c->cur], a, b, dtinvh, nullptr, 1);
     LAUNCH_CHECK("process_all");
     c->cur = 1 - c->cur;
-    c->ghosts_valid = true;
     return MPCF_OK;
 }
 
@@ -323,15 +321,12 @@ extern "C" int mpcf_step(mpcf_ctx* c, double cfl, double dt_max, double* dt)
         c->L->sos(c->g, c->u[c->cur], c->st, (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0, 1, cfl, c->h, dt_max, c->stream);
     }
     LAUNCH_CHECK("sos/dt");
-    // ghosts are filled here only when the block was changed from outside a
-    // stage; every stage writes the ghosts of its output
-    if (!c->ghosts_valid) { Timed t(c, K_GHOSTS); c->L->ghosts(c->g, c->u[c->cur], c->bc, c->stream); }
+    // the sweeps apply the boundary conditions as they load: no ghost fill
     for (int s = 0; s < 3; ++s) {
         run_rhs(c, c->u[c->cur], c->u[1 - c->cur], kA[s], kB[s], 0.0, c->st, 1);
         LAUNCH_CHECK("stage");
         c->cur = 1 - c->cur;
     }
-    c->ghosts_valid = true;
     if (dt) {
         double d;
         float sos;
@@ -508,11 +503,11 @@ struct mpcf_batch
     // device tables
     mpcf::CaseDesc* d_cases = nullptr;
     int* d_cur = nullptr;
-    int* d_off = nullptr;                    // 4 x (cap + 1)
+    int* d_off = nullptr;                    // 3 x (cap + 1)
     int cap = 0;
     bool dirty = true;
     bool cur_on_device = false;              // d_cur holds the truth (after steps)
-    int tot_xy = 0, tot_z = 0, tot_ghost = 0, tot_sos = 0;
+    int tot_xy = 0, tot_z = 0, tot_sos = 0;
     mpcf::BatchDev dev{};
     // graph of one step
     bool use_graph = false;
@@ -573,16 +568,15 @@ int batch_rebuild(mpcf_batch* b)
         const int cap = std::max(16, 2 * n);
         hipError_t e = hipMalloc((void**)&b->d_cases, cap * sizeof(mpcf::CaseDesc));
         if (e == hipSuccess) e = hipMalloc((void**)&b->d_cur, cap * sizeof(int));
-        if (e == hipSuccess) e = hipMalloc((void**)&b->d_off, 4 * (cap + 1) * sizeof(int));
+        if (e == hipSuccess) e = hipMalloc((void**)&b->d_off, 3 * (cap + 1) * sizeof(int));
         if (e != hipSuccess) return fail(MPCF_ENOMEM, std::string("batch tables: ") + hipGetErrorString(e));
         b->cap = cap;
     }
     std::vector<mpcf::CaseDesc> desc(n);
-    std::vector<int> cur(n), off(4 * (b->cap + 1), 0);
+    std::vector<int> cur(n), off(3 * (b->cap + 1), 0);
     int* oxy = &off[0];
     int* oz = &off[b->cap + 1];
-    int* og = &off[2 * (b->cap + 1)];
-    int* os = &off[3 * (b->cap + 1)];
+    int* os = &off[2 * (b->cap + 1)];
     long long cols_xy = 0, cols_z = 0;
     for (mpcf_ctx* c : b->cases) {
         cols_xy += (long long)(c->g.nx + 6) * c->g.nz;
@@ -590,13 +584,12 @@ int batch_rebuild(mpcf_batch* b)
     }
     for (int i = 0; i < n; ++i) {
         mpcf_ctx* c = b->cases[i];
-        int blocks[4];
+        int blocks[3];
         mpcf::detail::case_desc(c, c->cur, march_len(b, i, c->g.ny, 0, cols_xy), march_len(b, i, c->g.nz, 1, cols_z),
                                 desc[i], blocks);
         oxy[i + 1] = oxy[i] + blocks[0];
         oz[i + 1] = oz[i] + blocks[1];
-        og[i + 1] = og[i] + blocks[2];
-        os[i + 1] = os[i] + blocks[3];
+        os[i + 1] = os[i] + blocks[2];
         cur[i] = c->cur;
     }
     if (n > 0) {
@@ -605,10 +598,9 @@ int batch_rebuild(mpcf_batch* b)
         HIP_TRY(hipMemcpyAsync(b->d_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
         HIP_TRY(hipStreamSynchronize(b->stream));
     }
-    b->tot_xy = oxy[n]; b->tot_z = oz[n]; b->tot_ghost = og[n]; b->tot_sos = os[n];
+    b->tot_xy = oxy[n]; b->tot_z = oz[n]; b->tot_sos = os[n];
     b->dev.cases = b->d_cases; b->dev.cur = b->d_cur; b->dev.n = n;
-    b->dev.off_xy = b->d_off; b->dev.off_z = b->d_off + (b->cap + 1);
-    b->dev.off_ghost = b->d_off + 2 * (b->cap + 1); b->dev.off_sos = b->d_off + 3 * (b->cap + 1);
+    b->dev.off_xy = b->d_off; b->dev.off_z = b->d_off + (b->cap + 1); b->dev.off_sos = b->d_off + 2 * (b->cap + 1);
     b->dirty = false;
     batch_drop_graph(b);
     return MPCF_OK;
@@ -616,8 +608,8 @@ int batch_rebuild(mpcf_batch* b)
 
 void batch_enqueue_step(mpcf_batch* b, double cfl, double dt_max)
 {
-    // every case enters with valid ghosts (mpcf_batch_add) and every stage
-    // writes the ghosts of its output, so a step is seven launches
+    // the sweeps apply each case's boundary conditions as they load: a step is
+    // seven launches
     const int cpu_form = (b->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0;
     b->L->b_sos(b->dev, b->tot_sos, cpu_form, cfl, dt_max, b->stream);
     for (int s = 0; s < 3; ++s) {
@@ -690,8 +682,6 @@ extern "C" int mpcf_batch_add(mpcf_batch* b, int nx, int ny, int nz, double h, c
         if (rc != MPCF_OK) { mpcf_destroy(c); return rc; }
     }
     rc = mpcf_upload(c, MPCF_DATA, fields);
-    if (rc == MPCF_OK) rc = mpcf_load_ghosts(c);
-    if (rc == MPCF_OK) rc = mpcf_synchronize(c);
     if (rc != MPCF_OK) { mpcf_destroy(c); return rc; }
     b->cases.push_back(c);
     b->ids.push_back(b->next_id);

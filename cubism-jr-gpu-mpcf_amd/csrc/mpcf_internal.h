@@ -52,7 +52,6 @@ struct Launchers
     decltype(&mpcf::parity::launch_test_weno) test_weno;
     decltype(&mpcf::parity::launch_test_face) test_face;
     decltype(&mpcf::parity::batch_sos) b_sos;
-    decltype(&mpcf::parity::batch_ghosts) b_ghosts;
     decltype(&mpcf::parity::batch_xy) b_xy;
     decltype(&mpcf::parity::batch_z) b_z;
 };
@@ -60,7 +59,7 @@ struct Launchers
 #define MPCF_LAUNCHERS(NS)                                                                           \
     {mpcf::NS::launch_ghosts, mpcf::NS::launch_sos, mpcf::NS::launch_sweep, mpcf::NS::launch_update, \
      mpcf::NS::launch_pack, mpcf::NS::launch_test_weno, mpcf::NS::launch_test_face,                   \
-     mpcf::NS::batch_sos, mpcf::NS::batch_ghosts, mpcf::NS::batch_xy, mpcf::NS::batch_z}
+     mpcf::NS::batch_sos, mpcf::NS::batch_xy, mpcf::NS::batch_z}
 
 inline const Launchers kParity = MPCF_LAUNCHERS(parity);
 inline const Launchers kFast = MPCF_LAUNCHERS(fast);
@@ -119,7 +118,6 @@ struct mpcf_ctx
     StepState* st_host; // pinned host mirror
     hipStream_t stream;
     bool own_stream;
-    bool ghosts_valid;  // the face ghosts of u[cur] match its interior and bc
     size_t bytes;
     const Launchers* L;
     // optional per-kernel event timing (mpcf_timing_enable)
@@ -143,8 +141,8 @@ inline int batch_march(const int nd, const int forced, const int tuned)
 }
 
 // descriptor of one case for the batch kernels, and its block counts in the
-// flat grids: blocks[0] x+y sweep, [1] z sweep, [2] ghost fill, [3] speed of sound
-inline void case_desc(const mpcf_ctx* c, const int cur, const int len_xy, const int len_z, CaseDesc& d, int blocks[4])
+// flat grids: blocks[0] x+y sweep, [1] z sweep, [2] speed of sound
+inline void case_desc(const mpcf_ctx* c, const int cur, const int len_xy, const int len_z, CaseDesc& d, int blocks[3])
 {
     d.g = c->g;
     d.u[0] = c->u[0]; d.u[1] = c->u[1]; d.R = c->R; d.acc = c->acc; d.st = c->st; d.h = c->h;
@@ -158,14 +156,9 @@ inline void case_desc(const mpcf_ctx* c, const int cur, const int len_xy, const 
     parity::z_grid(c->g, d.chunk_z, dims);
     d.nbx_z = dims[0]; d.nby_z = dims[1];
     blocks[1] = dims[0] * dims[1] * dims[2];
-    int tmax = c->g.ny * c->g.nz;
-    if (c->g.nx * c->g.nz > tmax) tmax = c->g.nx * c->g.nz;
-    if (c->g.nx * c->g.ny > tmax) tmax = c->g.nx * c->g.ny;
-    d.nb_ghost = (tmax + 255) / 256;
-    blocks[2] = 6 * d.nb_ghost;
     const int rows4 = (c->g.ny * c->g.nz + 3) / 4;
     d.nb_sos = rows4 < 2048 ? rows4 : 2048;
-    blocks[3] = d.nb_sos;
+    blocks[2] = d.nb_sos;
     (void)cur;
 }
 
@@ -184,20 +177,22 @@ inline int reflect_mask(const int bc[6])
 }
 
 // march length of the batch kernels when neither forced nor tuned: split the
-// marched axis until the active cases together have about 2^16 marching
-// threads (one wave per SIMD), but march at least 8 cells.  Few small cases
-// then run short chains (a step's latency is the chain length times the
-// per-cell latency); many or large cases keep long marches, which recompute
-// fewer warm-up cells.  `columns` is the marching columns of all active cases
-// in this sweep.
+// marched axis until the active cases together have about 2^17 marching
+// threads (two waves per SIMD), keeping every march between 8 and 64 cells.
+// A step's latency is the march length times the per-cell latency, so few
+// small cases run short marches; long marches recompute fewer warm-up cells,
+// but beyond 64 cells a case leaves too few waves to hide latency (measured
+// on MI355X: a 192^3 case steps in 2.2 ms with 64-cell marches, 3.1 ms with
+// 96).  `columns` is the marching columns of all active cases in this sweep.
 inline int adaptive_march(const int nd, const long long columns)
 {
-    const long long target = 1LL << 16;
+    const long long target = 1LL << 17;
     long long nchunks = columns > 0 ? (target + columns - 1) / columns : 1;
     if (nchunks < 1) nchunks = 1;
     int len = (int)((nd + nchunks - 1) / nchunks);
-    if (len < 8) len = nd < 8 ? nd : 8;
-    return len;
+    if (len > 64) len = 64;
+    if (len < 8) len = 8;
+    return len < nd ? len : nd;
 }
 
 }  // namespace detail

@@ -71,9 +71,42 @@ struct StageArgs
     int64_t fs;                    // field stride (elements)
     Real a, b, dtinvh;
     const StepState* st;           // if non-null, dtinvh is read from st->dtinvh
-    int ghosts;                    // MODE_STAGE: also write the face ghosts of uo
     int reflect;                   // bit f set: face f is reflecting (else absorbing)
 };
+
+// Boundary conditions at load time (BoundaryConditions.h:60-109).  A stencil
+// read at index i outside [0, n) along an axis is served by the cell its
+// ghost would copy: absorbing -- the boundary cell; reflecting -- the mirror
+// cell (-1-i on the low face, 2n-1-i on the high face) with the axis' normal
+// momentum negated (sg = -1).  The values are the ghost values bit for bit,
+// so no kernel fills or reads a ghost band.
+__device__ __forceinline__ int bc_index(const int i, const int n, const int reflect, const int face_lo, Real& sg)
+{
+    sg = Real(1.0f);
+    if (i < 0) {
+        if ((reflect >> face_lo) & 1) { sg = Real(-1.0f); return -1 - i; }
+        return 0;
+    }
+    if (i >= n) {
+        if ((reflect >> (face_lo + 1)) & 1) { sg = Real(-1.0f); return 2 * n - 1 - i; }
+        return n - 1;
+    }
+    return i;
+}
+
+// raw conserved values of the cell at index `row` of a marched column (e0 =
+// element of row 0, s = element stride of the axis), boundary-mapped.  Returns
+// the sign the axis' normal momentum takes; the caller applies it where the
+// values are consumed, so that a prefetch is not waited on when it is issued.
+__device__ __forceinline__ Real load_row(const StageArgs& A, const int e0, const int row, const int n, const int s,
+                                         const int face_lo, Real* out)
+{
+    Real sg;
+    const int r = bc_index(row, n, A.reflect, face_lo, sg);
+#pragma unroll
+    for (int q = 0; q < NVAR; ++q) out[q] = A.u[q*A.fs + e0 + r * s];
+    return sg;
+}
 
 // operands of one cell's accumulator update, loaded one step ahead
 template <int MODE>
@@ -97,48 +130,11 @@ __device__ __forceinline__ void emit_load(const StageArgs& A, const int e, EmitI
     }
 }
 
-// The face ghosts of the stage output, written by the cell they copy
-// (BoundaryConditions.h:60-109): absorbing -- the boundary cell fills all three
-// layers; reflecting -- cell j fills the layer at distance j+1 outside the
-// face, with the normal momentum negated.  Only face ghosts are ever read
-// (every stencil is one-directional), so edges and corners are left alone.
-struct CellPos { int x, y, z; };
-
-__device__ __forceinline__ void put_ghost(const StageArgs& A, const int ed, const Real* v, const int neg)
-{
-#pragma unroll
-    for (int q = 0; q < NVAR; ++q) A.uo[q*A.fs + ed] = q == neg ? -v[q] : v[q];
-}
-
-__device__ __forceinline__ void face_ghosts(const StageArgs& A, const int e, const Real* v, const int i, const int n,
-                                            const int stride, const int face_lo, const int neg)
-{
-    if (i < 3) {
-        if ((A.reflect >> face_lo) & 1) put_ghost(A, e - (2 * i + 1) * stride, v, neg);
-        else if (i == 0) { put_ghost(A, e - stride, v, -1); put_ghost(A, e - 2 * stride, v, -1); put_ghost(A, e - 3 * stride, v, -1); }
-    }
-    const int j = n - 1 - i;
-    if (j < 3) {
-        if ((A.reflect >> (face_lo + 1)) & 1) put_ghost(A, e + (2 * j + 1) * stride, v, neg);
-        else if (j == 0) { put_ghost(A, e + stride, v, -1); put_ghost(A, e + 2 * stride, v, -1); put_ghost(A, e + 3 * stride, v, -1); }
-    }
-}
-
-__device__ __forceinline__ void stage_ghosts(const StageArgs& A, const Geom& g, const int e, const CellPos& c,
-                                             const Real* v)
-{
-    face_ghosts(A, e, v, c.x, g.nx, 1, 0, FU);
-    face_ghosts(A, e, v, c.y, g.ny, g.px, 2, FV);
-    face_ghosts(A, e, v, c.z, g.nz, g.px * g.py, 4, FW);
-}
-
 // write the contribution of one cell.  F = flux at the cell's right face,
 // Fl = flux at its left face; cG/cP = plus-state + minus-state of G/P.
-// pos: the cell's coordinates (MODE_STAGE ghost writes only).
 template <int MODE>
 __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, const int e, const EmitIn<MODE>& in,
-                                     const FaceFlux& F, const FaceFlux& Fl, const Real cG, const Real cP,
-                                     const Geom* g = nullptr, const CellPos* pos = nullptr)
+                                     const FaceFlux& F, const FaceFlux& Fl, const Real cG, const Real cP)
 {
     Real* __restrict__ acc = A.acc;
     const int64_t fs = A.fs;
@@ -162,7 +158,6 @@ __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, cons
         const Real sP = in.acc[A_SUMP] + cP;
         const Real dU = in.acc[A_DIVU] + (F.vel - Fl.vel);
         const Real factor6 = Real(1.0f) / Real(6.0f);
-        Real v[NVAR];
 #pragma unroll
         for (int q = 0; q < NVAR; ++q) {
             const Real t = (in.acc[q] + F.f[q]) - Fl.f[q];
@@ -172,12 +167,8 @@ __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, cons
             else              rhs = dtinvh*t;
             const Real Rn = A.a*in.R[q] - rhs;
             A.R[q*fs + e] = Rn;
-            if (MODE == MODE_STAGE) {
-                v[q] = A.b*Rn + in.u[q];
-                A.uo[q*fs + e] = v[q];
-            }
+            if (MODE == MODE_STAGE) A.uo[q*fs + e] = A.b*Rn + in.u[q];
         }
-        if (MODE == MODE_STAGE && A.ghosts && pos) stage_ghosts(A, *g, e, *pos, v);
     }
 }
 
@@ -194,13 +185,12 @@ struct March
 
     // reconstruct cell k, solve face k-1/2, emit cell k-1 (operands in `in`)
     __device__ __forceinline__ void step(const StageArgs& A, const Real dtinvh, const int k, const int c0, const int e_km1,
-                                         const EmitIn<MODE>& in, const bool store = true,
-                                         const Geom* g = nullptr, const CellPos* pos = nullptr)
+                                         const EmitIn<MODE>& in, const bool store = true)
     {
         const CellRec r = reconstruct<Real, kFast>(w0, w1, w2, w3, w4);
         if (k >= c0) {
             const FaceFlux F = riemann<D, Real, kFast>(pm, r.p);
-            if (k >= c0 + 1 && store) emit<MODE>(A, dtinvh, e_km1, in, F, Fl, pGp + pm.G, pPp + pm.P, g, pos);
+            if (k >= c0 + 1 && store) emit<MODE>(A, dtinvh, e_km1, in, F, Fl, pGp + pm.G, pPp + pm.P);
             Fl = F;
         }
         pm = r.m; pGp = r.p.G; pPp = r.p.P;
@@ -242,46 +232,39 @@ __device__ __forceinline__ void sweep_yz_body(const Geom& g, const StageArgs& A,
     const int64_t fs = A.fs;
     constexpr bool kRaw = MODE == MODE_STAGE;   // the update needs the raw u of the emitted cell
 
+    constexpr int NEGQ = D == 1 ? FV : FW;   // normal momentum of the marched axis
+    constexpr int FLO = 2 * D;               // its low face
     March<D, MODE> m;
-    // raw conserved values of cells k-1 .. k+2 (kept for the register update)
-    Real r0[NVAR], r1[NVAR], r2[NVAR], r3[NVAR];
-#pragma unroll
-    for (int q = 0; q < NVAR; ++q) {
-        r1[q] = A.u[q*fs + e0 + (c0 - 2) * s];
-        r2[q] = A.u[q*fs + e0 + (c0 - 1) * s];
-        r3[q] = A.u[q*fs + e0 + (c0    ) * s];
-    }
-    m.w1 = load_prim(A.u, fs, e0 + (c0 - 3) * s);
+    // raw conserved values of cells k-1 .. k+2 (kept for the register update;
+    // the emitted cell k-1 is always interior, so its raw values are exact)
+    Real r0[NVAR], r1[NVAR], r2[NVAR], r3[NVAR], w[NVAR];
+    r1[NEGQ] *= load_row(A, e0, c0 - 2, nd, s, FLO, r1);
+    r2[NEGQ] *= load_row(A, e0, c0 - 1, nd, s, FLO, r2);
+    r3[NEGQ] *= load_row(A, e0, c0, nd, s, FLO, r3);
+    w[NEGQ] *= load_row(A, e0, c0 - 3, nd, s, FLO, w);
+    m.w1 = to_prim<Real, kFast, kP64>(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
     m.w2 = to_prim<Real, kFast, kP64>(r1[0], r1[1], r1[2], r1[3], r1[4], r1[5], r1[6]);
     m.w3 = to_prim<Real, kFast, kP64>(r2[0], r2[1], r2[2], r2[3], r2[4], r2[5], r2[6]);
     m.w4 = to_prim<Real, kFast, kP64>(r3[0], r3[1], r3[2], r3[3], r3[4], r3[5], r3[6]);
     // software pipeline: raw values of the next incoming cell and the emit
     // operands of the next step are loaded one step ahead
     Real nxt[NVAR];
-#pragma unroll
-    for (int q = 0; q < NVAR; ++q) nxt[q] = A.u[q*fs + e0 + (c0 + 1) * s];
+    Real nsg = load_row(A, e0, c0 + 1, nd, s, FLO, nxt);
     EmitIn<MODE> ein, ein_next;
     for (int k = c0 - 1; k <= c1; ++k) {
+        nxt[NEGQ] *= nsg;   // boundary sign of the prefetched row
         // rotate the raw ring: r0 = cell k-1, r3 = cell k+2
 #pragma unroll
         for (int q = 0; q < NVAR; ++q) { r0[q] = r1[q]; r1[q] = r2[q]; r2[q] = r3[q]; r3[q] = nxt[q]; }
         const Prim q = to_prim<Real, kFast, kP64>(nxt[0], nxt[1], nxt[2], nxt[3], nxt[4], nxt[5], nxt[6]);  // cell k+2
-        if (k + 3 <= c1 + 2) {
-#pragma unroll
-            for (int f = 0; f < NVAR; ++f) nxt[f] = A.u[f*fs + e0 + (k + 3) * s];
-        }
+        if (k + 3 <= c1 + 2) nsg = load_row(A, e0, k + 3, nd, s, FLO, nxt);
         if (k >= c0 && k < c1) emit_load<MODE>(A, e0 + k * s, ein_next);   // emitted at step k+1
         if (kRaw) {
 #pragma unroll
             for (int f = 0; f < NVAR; ++f) ein.u[f] = r0[f];                 // cell k-1
         }
         m.push(q);
-        if (D == 2) {
-            const CellPos pos{x, t, k - 1};
-            m.step(A, dtinvh, k, c0, e0 + (k - 1) * s, ein, true, &g, &pos);
-        } else {
-            m.step(A, dtinvh, k, c0, e0 + (k - 1) * s, ein);
-        }
+        m.step(A, dtinvh, k, c0, e0 + (k - 1) * s, ein);
         ein = ein_next;
     }
 }
@@ -347,12 +330,14 @@ void k_sweep_x(const Geom g, const StageArgs A, const int chunk)
 
     // raw values of the next run: cell xs + xl of the lane's 8 staging pencils
     Real pre[8][NVAR];
+    Real pre_sg;   // boundary sign of the prefetched run's x-momentum, applied when it is staged
     auto load_run = [&](const int xs) {
-        const int xc = min(xs + xl, c1 + 2);   // reads past the run end are clamped into the band
+        const int xc = min(xs + xl, c1 + 2);   // reads past the run end are clamped to the stencil reach
+        const int xm = bc_index(xc, g.nx, A.reflect, 0, pre_sg);
 #pragma unroll
         for (int i = 0; i < 8; ++i)
 #pragma unroll
-            for (int q = 0; q < NVAR; ++q) pre[i][q] = A.u[q * fs + sbase[i] + xc];
+            for (int q = 0; q < NVAR; ++q) pre[i][q] = A.u[q * fs + sbase[i] + xm];
     };
 
     March<0, MODE_FIRST> m;
@@ -362,7 +347,8 @@ void k_sweep_x(const Geom g, const StageArgs A, const int chunk)
         // stage the run (previous run's consumers and storers are past the barrier below)
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
-            const Prim q = to_prim<Real, kFast, kP64>(pre[i][0], pre[i][1], pre[i][2], pre[i][3], pre[i][4], pre[i][5], pre[i][6]);
+            const Prim q = to_prim<Real, kFast, kP64>(pre[i][0], pre[i][1] * pre_sg, pre[i][2], pre[i][3], pre[i][4],
+                                                      pre[i][5], pre[i][6]);
             const int pl = i * 8 + lane / SX;
             S.in[FR][pl][xl] = q.r; S.in[FU][pl][xl] = q.u; S.in[FV][pl][xl] = q.v; S.in[FW][pl][xl] = q.w;
             S.in[FE][pl][xl] = q.p; S.in[FG][pl][xl] = q.G; S.in[FP][pl][xl] = q.P;
@@ -464,28 +450,35 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
     const bool store = lane >= 3 && lane <= 60 && x >= 0 && x < g.nx && z < g.nz;
     if ((bw * 4 + threadIdx.y) * XY_TILE >= rowlen * g.nz) return;   // whole wave past the end
     z = min(z, g.nz - 1);                         // lanes past the last plane read its last row
-    const int xc = x;
+    // lanes outside [0, nx) read the cell their x-ghost copies
+    Real sgx;
+    const int xm = bc_index(x, g.nx, A.reflect, 0, sgx);
+    const bool xmirror = (A.reflect & 3) != 0;
     const int c0 = bz * chunk;
     const int c1 = min(c0 + chunk, g.ny);
     const int s = g.px;
-    const int e0 = g.idx(xc, 0, z);
+    const int e0 = g.idx(xm, 0, z);
     const int64_t fs = A.fs;
     Real* __restrict__ acc = A.acc;
+    // boundary signs of a loaded row: y-momentum by the row's mapping,
+    // x-momentum by the lane's (applied where the values are consumed)
+    auto fix = [&](Real* o, const Real sgy) {
+        o[FV] *= sgy;
+        if (xmirror) o[FU] *= sgx;
+    };
 
     March<1, MODE_ACC> m;
-    m.w1 = load_prim(A.u, fs, e0 + (c0 - 3) * s);
-    m.w2 = load_prim(A.u, fs, e0 + (c0 - 2) * s);
-    m.w3 = load_prim(A.u, fs, e0 + (c0 - 1) * s);
-    m.w4 = load_prim(A.u, fs, e0 + (c0    ) * s);
+    Real w[NVAR];
+    fix(w, load_row(A, e0, c0 - 3, g.ny, s, 2, w)); m.w1 = to_prim<Real, kFast, kP64>(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
+    fix(w, load_row(A, e0, c0 - 2, g.ny, s, 2, w)); m.w2 = to_prim<Real, kFast, kP64>(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
+    fix(w, load_row(A, e0, c0 - 1, g.ny, s, 2, w)); m.w3 = to_prim<Real, kFast, kP64>(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
+    fix(w, load_row(A, e0, c0, g.ny, s, 2, w));     m.w4 = to_prim<Real, kFast, kP64>(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
     Real nxt[NVAR];
-#pragma unroll
-    for (int q = 0; q < NVAR; ++q) nxt[q] = A.u[q*fs + e0 + (c0 + 1) * s];
+    Real nsg = load_row(A, e0, c0 + 1, g.ny, s, 2, nxt);
     for (int k = c0 - 1; k <= c1; ++k) {
+        fix(nxt, nsg);
         const Prim q = to_prim<Real, kFast, kP64>(nxt[0], nxt[1], nxt[2], nxt[3], nxt[4], nxt[5], nxt[6]);  // row k+2
-        if (k + 3 <= c1 + 2) {
-#pragma unroll
-            for (int f = 0; f < NVAR; ++f) nxt[f] = A.u[f*fs + e0 + (k + 3) * s];
-        }
+        if (k + 3 <= c1 + 2) nsg = load_row(A, e0, k + 3, g.ny, s, 2, nxt);
         m.push(q);
         const CellRec ry = reconstruct<Real, kFast>(m.w0, m.w1, m.w2, m.w3, m.w4);   // row k, y-direction
         if (k >= c0) {
@@ -740,7 +733,6 @@ __device__ __forceinline__ int find_case(const int* __restrict__ off, const int 
 __device__ __forceinline__ StageArgs case_args(const CaseDesc& C, const int cur, const Real a, const Real b)
 {
     StageArgs A;
-    A.ghosts = 1;
     A.reflect = 0;
     for (int f = 0; f < 6; ++f) A.reflect |= (C.bc[f] == 1) << f;
     A.u = (const Real*)C.u[cur]; A.R = (Real*)C.R; A.acc = (Real*)C.acc; A.uo = (Real*)C.u[1 - cur];
@@ -758,15 +750,6 @@ void kb_sos(const BatchDev B, const double cfl, const double dt_max)
     const CaseDesc& C = B.cases[c];
     const DtArgs d{1, cfl, C.h, dt_max};
     sos_body<CPU_FORM>(C.g, (const Real*)C.u[B.cur[c]], C.g.fstride, C.st, blockIdx.x - B.off_sos[c], C.nb_sos, wmax, d);
-}
-
-__global__ __launch_bounds__(256)
-void kb_ghosts(const BatchDev B)
-{
-    const int c = find_case(B.off_ghost, B.n, blockIdx.x);
-    const CaseDesc& C = B.cases[c];
-    const int local = blockIdx.x - B.off_ghost[c];
-    ghosts_body(C.g, (Real*)C.u[B.cur[c]], C.g.fstride, C.bc, local % C.nb_ghost, local / C.nb_ghost);
 }
 
 __global__ __launch_bounds__(256)
@@ -851,7 +834,6 @@ void launch_sweep(const Geom& g, const int dir, const SweepParams& p, const int 
     StageArgs A;
     A.u = (const Real*)p.u; A.R = (Real*)p.R; A.acc = (Real*)p.acc; A.uo = (Real*)p.uo; A.fs = g.fstride;
     A.a = (Real)p.a; A.b = (Real)p.b; A.dtinvh = (Real)p.dtinvh; A.st = p.st;
-    A.ghosts = p.ghosts;
     A.reflect = p.reflect;
     if (dir == 3) {
         const int chunk = pick_chunk(g.ny, (int64_t)g.nx * g.nz, 1);
@@ -935,11 +917,6 @@ void batch_sos(const BatchDev& B, const int blocks, const int cpu_form, const do
 {
     if (cpu_form) hipLaunchKernelGGL(kb_sos<1>, dim3(blocks), dim3(64, 4), 0, s, B, cfl, dt_max);
     else          hipLaunchKernelGGL(kb_sos<0>, dim3(blocks), dim3(64, 4), 0, s, B, cfl, dt_max);
-}
-
-void batch_ghosts(const BatchDev& B, const int blocks, hipStream_t s)
-{
-    hipLaunchKernelGGL(kb_ghosts, dim3(blocks), dim3(256), 0, s, B);
 }
 
 void batch_xy(const BatchDev& B, const int blocks, const double a, const double b, hipStream_t s)

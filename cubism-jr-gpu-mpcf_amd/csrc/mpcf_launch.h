@@ -19,8 +19,8 @@ struct SweepParams
     double a, b, dtinvh;   // stage coefficients, rounded to the storage type by
                            // the launcher; dtinvh ignored when st != NULL
     const StepState* st;   // device step state carrying dt/h
-    int ghosts;            // stage == 1: also write the face ghosts of uo
-    int reflect;           // bit f set: face f reflecting (else absorbing)
+    int reflect;           // bit f set: face f reflecting (else absorbing); the
+                           // sweeps apply the boundary conditions as they load
 };
 
 // one case of a batched ensemble, as the batch kernels see it
@@ -36,7 +36,6 @@ struct CaseDesc
     int chunk_xy, chunk_z; // march lengths of the x+y and z sweeps
     int nbx_xy, nby_xy;    // x+y sweep grid: x tiles, z groups (y chunks = third dim)
     int nbx_z, nby_z;      // z sweep grid: x blocks, y groups (z chunks = third dim)
-    int nb_ghost;          // ghost-fill blocks per face
     int nb_sos;            // speed-of-sound blocks
 };
 
@@ -50,7 +49,6 @@ struct BatchDev
     int n;
     const int* off_xy;
     const int* off_z;
-    const int* off_ghost;
     const int* off_sos;
 };
 }
@@ -66,7 +64,6 @@ void launch_pack(const Geom& g, void* pad, void* dense, int to_padded, int nq, i
 void launch_test_weno(int n, const void* in5, void* m, void* p, hipStream_t s);                            \
 void launch_test_face(int n, int dir, const void* st, void* out, hipStream_t s);                           \
 void batch_sos(const BatchDev& B, int blocks, int cpu_form, double cfl, double dt_max, hipStream_t s);    \
-void batch_ghosts(const BatchDev& B, int blocks, hipStream_t s);                                           \
 void batch_xy(const BatchDev& B, int blocks, double a, double b, hipStream_t s);                           \
 void batch_z(const BatchDev& B, int blocks, double a, double b, hipStream_t s);                            \
 void xy_grid(const Geom& g, int chunk, int dims[3]);                                                       \

@@ -175,7 +175,7 @@ struct mpcf_server
     std::vector<Req*> active;
     std::vector<long long> active_seq;
     mpcf::BatchDev dev{};
-    int tot[4] = {0, 0, 0, 0};
+    int tot[3] = {0, 0, 0};   // blocks of the x+y sweep, z sweep, speed of sound
     hipGraphExec_t exec = nullptr;
     std::vector<std::pair<hipGraphExec_t, hipEvent_t>> retired;
     // statistics (engine-owned, read under mu via snapshot)
@@ -312,7 +312,6 @@ bool progress(mpcf_server* S, bool* changed)
             if (hipMemcpyAsync(c->acc, S->staging[r->stage_buf].p, 7 * n * sizeof(float), hipMemcpyHostToDevice,
                                S->stream) != hipSuccess) return false;
             S->L->pack(c->g, c->u[0], c->acc, 1, 7, 0, S->stream);
-            S->L->ghosts(c->g, c->u[0], c->bc, S->stream);
             // R starts at zero; the step state's reduction targets and counters too
             if (hipMemsetAsync(c->R, 0, 7 * (size_t)c->g.fstride * sizeof(float), S->stream) != hipSuccess) return false;
             if (hipMemsetAsync(c->st, 0, sizeof(StepState), S->stream) != hipSuccess) return false;
@@ -447,9 +446,8 @@ bool rebuild(mpcf_server* S)
     const int cap = S->max_live;
     int* oxy = S->h_off[sl];
     int* oz = oxy + (cap + 1);
-    int* og = oxy + 2 * (cap + 1);
-    int* os = oxy + 3 * (cap + 1);
-    oxy[0] = oz[0] = og[0] = os[0] = 0;
+    int* os = oxy + 2 * (cap + 1);
+    oxy[0] = oz[0] = os[0] = 0;
     long long cells = 0, cols_xy = 0, cols_z = 0;
     for (Req* r : S->active) {
         cols_xy += (long long)(r->c->g.nx + 6) * r->c->g.nz;
@@ -457,27 +455,25 @@ bool rebuild(mpcf_server* S)
     }
     for (int i = 0; i < n; ++i) {
         mpcf_ctx* c = S->active[i]->c;
-        int blocks[4];
+        int blocks[3];
         mpcf::detail::case_desc(c, c->cur, mpcf::detail::adaptive_march(c->g.ny, cols_xy),
                                 mpcf::detail::adaptive_march(c->g.nz, cols_z), hc[i], blocks);
         oxy[i + 1] = oxy[i] + blocks[0];
         oz[i + 1] = oz[i] + blocks[1];
-        og[i + 1] = og[i] + blocks[2];
-        os[i + 1] = os[i] + blocks[3];
+        os[i + 1] = os[i] + blocks[2];
         hcur[i] = c->cur;
         cells += c->g.nx * (long long)c->g.ny * c->g.nz;
     }
     if (n > 0) {
         if (hipMemcpyAsync(S->d_cases, hc, n * sizeof(mpcf::CaseDesc), hipMemcpyHostToDevice, S->stream) != hipSuccess ||
             hipMemcpyAsync(S->d_cur, hcur, n * sizeof(int), hipMemcpyHostToDevice, S->stream) != hipSuccess ||
-            hipMemcpyAsync(S->d_off, oxy, 4 * (cap + 1) * sizeof(int), hipMemcpyHostToDevice, S->stream) != hipSuccess)
+            hipMemcpyAsync(S->d_off, oxy, 3 * (cap + 1) * sizeof(int), hipMemcpyHostToDevice, S->stream) != hipSuccess)
             return false;
         if (hipEventRecord(S->slot_ev[sl], S->stream) != hipSuccess) return false;
     }
-    S->tot[0] = oxy[n]; S->tot[1] = oz[n]; S->tot[2] = og[n]; S->tot[3] = os[n];
+    S->tot[0] = oxy[n]; S->tot[1] = oz[n]; S->tot[2] = os[n];
     S->dev.cases = S->d_cases; S->dev.cur = S->d_cur; S->dev.n = n;
-    S->dev.off_xy = S->d_off; S->dev.off_z = S->d_off + (cap + 1);
-    S->dev.off_ghost = S->d_off + 2 * (cap + 1); S->dev.off_sos = S->d_off + 3 * (cap + 1);
+    S->dev.off_xy = S->d_off; S->dev.off_z = S->d_off + (cap + 1); S->dev.off_sos = S->d_off + 2 * (cap + 1);
     S->st_active_cells = cells;
     retire_graph(S);
     ++S->st_rebuilds;
@@ -486,10 +482,10 @@ bool rebuild(mpcf_server* S)
 
 void enqueue_step_kernels(mpcf_server* S)
 {
-    // cases enter with their ghosts filled and every stage writes the ghosts
-    // of its output: seven launches per step
+    // the sweeps apply each case's boundary conditions as they load: seven
+    // launches per step
     const int cpu_form = (S->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0;
-    S->L->b_sos(S->dev, S->tot[3], cpu_form, S->cfl, 1e30, S->stream);
+    S->L->b_sos(S->dev, S->tot[2], cpu_form, S->cfl, 1e30, S->stream);
     for (int s = 0; s < 3; ++s) {
         S->L->b_xy(S->dev, S->tot[0], kA[s], kB[s], S->stream);
         S->L->b_z(S->dev, S->tot[1], kA[s], kB[s], S->stream);
@@ -656,12 +652,12 @@ extern "C" int mpcf_server_create(mpcf_server** out, unsigned flags, long long h
     const int cap = max_live;
     if (hipMalloc((void**)&S->d_cases, cap * sizeof(mpcf::CaseDesc)) != hipSuccess ||
         hipMalloc((void**)&S->d_cur, cap * sizeof(int)) != hipSuccess ||
-        hipMalloc((void**)&S->d_off, 4 * (cap + 1) * sizeof(int)) != hipSuccess)
+        hipMalloc((void**)&S->d_off, 3 * (cap + 1) * sizeof(int)) != hipSuccess)
         return bail(MPCF_ENOMEM, "mpcf_server_create: tables");
     for (int k = 0; k < mpcf_server::NSLOT; ++k)
         if (hipHostMalloc((void**)&S->h_cases[k], cap * sizeof(mpcf::CaseDesc), hipHostMallocDefault) != hipSuccess ||
             hipHostMalloc((void**)&S->h_cur[k], cap * sizeof(int), hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void**)&S->h_off[k], 4 * (cap + 1) * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void**)&S->h_off[k], 3 * (cap + 1) * sizeof(int), hipHostMallocDefault) != hipSuccess ||
             hipEventCreateWithFlags(&S->slot_ev[k], hipEventDisableTiming) != hipSuccess)
             return bail(MPCF_ENOMEM, "mpcf_server_create: tables");
     S->copier = std::thread(copier_main, S);

@@ -206,3 +206,26 @@ def test_config1_shock_tube_10_steps_bitexact(mpcf, oracle):
         dref, _ = ob.step(1.0 / n, 0.3, 1e30, [0] * 6)
         assert dts[it] == dref, (it, dts[it], dref)
     assert_equal(u, ob.data, "config-1 state after 10 steps")
+
+
+def test_fresh_block_ignores_freed_memory(mpcf, oracle):
+    """A block created over memory freed by another block (here one holding
+    NaN everywhere) starts from zero: its steps match the oracle bit for bit.
+    The zeroing must be ordered on the block's own stream before its first
+    upload and stage."""
+    n = 96
+    bad = np.full((7, n, n, n), np.nan, dtype=np.float32)
+    say("freed-memory reuse", n)
+    with mpcf.Block(n, n, n, 1.0 / n) as b:
+        b.data = bad
+        b.tmp = bad
+    s = states.random_smooth(n, n, n, seed=17)
+    ob = oracle.Block(n, n, n)
+    ob.data = s
+    with mpcf.Block(n, n, n, 1.0 / n) as b:
+        b.data = s
+        for _ in range(2):
+            assert b.step(0.3) == ob.step(1.0 / n, 0.3, 1e30, [0] * 6)[0]
+        u, R = b.data, b.tmp
+    assert_equal(u, ob.data, "u after reuse")
+    assert_equal(R, ob.tmp, "R after reuse")
