@@ -20,7 +20,8 @@
 //   captured as a hipGraph while the subset is unchanged.  Which cases are
 //   active is the policy: all live cases; or, filled up to a cell budget, the
 //   oldest first, those with the least remaining work first, or the oldest
-//   one and then the least remaining work.
+//   one and then the least remaining work; or the oldest first with the
+//   budget raised to 1.25x the oldest request's cells.
 // * Each case's current-buffer index is mirrored on the host (a step flips it
 //   once), so membership changes need no read-back.  At most `depth` steps are
 //   in flight; the engine waits on the oldest beyond that.
@@ -387,7 +388,7 @@ void choose_active(mpcf_server* S, std::vector<Req*>& act)
     } else {
         auto by_age = [](const Req* a, const Req* b) { return a->seq < b->seq; };
         auto by_work = [](const Req* a, const Req* b) { return a->cells() * a->left < b->cells() * b->left; };
-        if (S->policy == MPCF_SERVE_FIFO_FILL) {
+        if (S->policy == MPCF_SERVE_FIFO_FILL || S->policy == MPCF_SERVE_FIFO_SLACK) {
             std::stable_sort(live.begin(), live.end(), by_age);
         } else {
             std::stable_sort(live.begin(), live.end(), by_work);
@@ -397,9 +398,17 @@ void choose_active(mpcf_server* S, std::vector<Req*>& act)
                 std::rotate(live.begin(), oldest, oldest + 1);
             }
         }
+        long long budget = S->fill_cells;
+        if (S->policy == MPCF_SERVE_FIFO_SLACK) {
+            // the oldest request steps at close to its own speed: companions
+            // may add at most a quarter of its cells (or fill the floor budget
+            // when it is small)
+            std::stable_sort(live.begin(), live.end(), by_age);
+            budget = std::max(budget, live.front()->cells() + live.front()->cells() / 4);
+        }
         long long cells = 0;
         for (Req* r : live) {
-            if (!act.empty() && cells + r->cells() > S->fill_cells) continue;
+            if (!act.empty() && cells + r->cells() > budget) continue;
             act.push_back(r);
             cells += r->cells();
         }
@@ -616,7 +625,7 @@ extern "C" int mpcf_server_create(mpcf_server** out, unsigned flags, long long h
     *out = nullptr;
     if (flags & ~(unsigned)(MPCF_FLAG_FAST | MPCF_FLAG_SOS_CPU | MPCF_FLAG_P64 | MPCF_FLAG_SERVE_EAGER))
         return fail(MPCF_EINVAL, "mpcf_server_create: unsupported flag (binary32 storage only)");
-    if (policy < MPCF_SERVE_ALL || policy > MPCF_SERVE_HYBRID) return fail(MPCF_EINVAL, "mpcf_server_create: bad policy");
+    if (policy < MPCF_SERVE_ALL || policy > MPCF_SERVE_FIFO_SLACK) return fail(MPCF_EINVAL, "mpcf_server_create: bad policy");
     if (hbm_bytes <= 0 || max_live < 1 || !(cfl > 0)) return fail(MPCF_EINVAL, "mpcf_server_create: bad budget / slots / cfl");
     if (flags & MPCF_FLAG_P64) flags |= MPCF_FLAG_FAST;
     mpcf_server* S = new mpcf_server();

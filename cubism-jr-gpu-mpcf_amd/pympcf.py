@@ -42,7 +42,7 @@ EXPORTS = [
 ]
 
 # serving policies and the eager-launch flag (mpcf_server_create)
-SERVE_ALL, SERVE_FIFO_FILL, SERVE_SRPT_FILL, SERVE_HYBRID = 0, 1, 2, 3
+SERVE_ALL, SERVE_FIFO_FILL, SERVE_SRPT_FILL, SERVE_HYBRID, SERVE_FIFO_SLACK = 0, 1, 2, 3, 4
 FLAG_SERVE_EAGER = 64
 
 # kernel classes of mpcf_timing_read
@@ -466,7 +466,8 @@ class Server:
     """Continuous-batching request server (mpcf_server): submit cases from this
     thread, the library's engine thread advances them on the device."""
 
-    POLICIES = {"all": SERVE_ALL, "fifo_fill": SERVE_FIFO_FILL, "srpt_fill": SERVE_SRPT_FILL, "hybrid": SERVE_HYBRID}
+    POLICIES = {"all": SERVE_ALL, "fifo_fill": SERVE_FIFO_FILL, "srpt_fill": SERVE_SRPT_FILL, "hybrid": SERVE_HYBRID,
+                "fifo_slack": SERVE_FIFO_SLACK}
 
     def __init__(self, hbm_bytes, policy="fifo_fill", fill_cells=0, max_live=256, cfl=0.3, fast=True,
                  p64=False, graph=True):

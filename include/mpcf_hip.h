@@ -278,7 +278,9 @@ enum {
     MPCF_SERVE_ALL = 0,        /* every admitted request steps every time                    */
     MPCF_SERVE_FIFO_FILL = 1,  /* oldest admitted first, added while the cells fit fill_cells */
     MPCF_SERVE_SRPT_FILL = 2,  /* least remaining work (cells x steps) first, same fill       */
-    MPCF_SERVE_HYBRID = 3      /* the oldest request, then least remaining work, same fill    */
+    MPCF_SERVE_HYBRID = 3,     /* the oldest request, then least remaining work, same fill    */
+    MPCF_SERVE_FIFO_SLACK = 4  /* oldest first; budget max(fill_cells, 1.25 x the oldest's
+                                  cells): the oldest steps at close to its solo speed        */
 };
 
 /* server flag: launch each step eagerly instead of replaying a hipGraph */

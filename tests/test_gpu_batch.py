@@ -122,8 +122,8 @@ def test_autotune_keeps_state_and_results(mpcf):
 
 @pytest.mark.parametrize("policy,fill,graph", [("all", 0, True), ("fifo_fill", 40000, True),
                                                ("srpt_fill", 40000, True), ("hybrid", 40000, True),
-                                               ("fifo_fill", 40000, False)],
-                         ids=["all", "fifo_fill", "srpt_fill", "hybrid", "fifo_fill-eager"])
+                                               ("fifo_slack", 8000, True), ("fifo_fill", 40000, False)],
+                         ids=["all", "fifo_fill", "srpt_fill", "hybrid", "fifo_slack", "fifo_fill-eager"])
 def test_server_results_equal_alone(mpcf, policy, fill, graph):
     """Requests submitted while others run, stepped in changing active subsets,
     each return exactly the state it reaches alone."""
