@@ -9,7 +9,9 @@ Objects:
   f64.o     mpcf_kernels.hip, MPCF_F64=1, -ffp-contract=off (the scheme in binary64)
   reduce.o  mpcf_reduce.hip  (binary64 block comparisons and integrals)
   capi.o    the C-ABI (include/mpcf_hip.h)
-  ic.o      initial conditions;  flops.o  host instantiations of the scheme
+  ic.o      initial conditions;  sic.o  the shock-induced cloud input and IC
+  io.o      dumps (raw + XDMF) and checkpoints (save.info + save.data)
+  flops.o   host instantiations of the scheme
 and the C++ driver mpcf_run (csrc/mpcf_run.cpp over csrc/mpcf.hpp).
 All HIP code is compiled with packed-FP32 instructions disabled
 (-target-feature -packed-fp32-ops).
@@ -67,6 +69,8 @@ def build(verbose: bool = False) -> str:
         ("capi.o", "mpcf_capi.cpp", ["-x", "hip"]),
         ("serve.o", "mpcf_serve.cpp", ["-x", "hip"]),
         ("ic.o", "ic.cpp", ["-x", "hip"]),
+        ("sic.o", "sic.cpp", ["-x", "hip"]),
+        ("io.o", "io.cpp", ["-x", "hip"]),
         ("flops.o", "flops.cpp", ["-x", "hip", "-ffp-contract=off"]),
     ]
     cmds = []

@@ -145,6 +145,21 @@ public:
 
     void synchronize() { check(mpcf_synchronize(ctx_)); }
 
+    // Sim_SteadyStateMPI::_dump / _save / _restart (Sim_SteadyStateMPI.cpp:138-179):
+    // raw [Z][Y][X][C] + XDMF dumps of a streamer, and save.info + save.data
+    void dump(int kind, const std::string& dir, const std::string& name, int step, double h)
+    {
+        check(mpcf_dump(ctx_, kind, dir.c_str(), name.c_str(), step, h));
+    }
+    void save(const std::string& dir, double t, int step, int fcount, double h)
+    {
+        check(mpcf_save(ctx_, dir.c_str(), t, step, fcount, h));
+    }
+    void restart(const std::string& dir, double& t, int& step, int& fcount)
+    {
+        check(mpcf_restart(ctx_, dir.c_str(), &t, &step, &fcount));
+    }
+
 private:
     static double seconds_since(std::chrono::steady_clock::time_point t0)
     {

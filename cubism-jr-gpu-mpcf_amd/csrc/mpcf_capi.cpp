@@ -264,6 +264,38 @@ extern "C" int mpcf_download_f64(mpcf_ctx* c, int which, double* const fields[7]
     return transfer(c, which, (void* const*)fields, 1, false, "mpcf_download_f64");
 }
 
+extern "C" int mpcf_stream_channels(int kind)
+{
+    return kind == MPCF_STREAM_TENSOR ? 9 : kind == MPCF_STREAM_SAVE ? 7 : kind == MPCF_STREAM_SCALAR ? 1 : 0;
+}
+
+extern "C" int mpcf_stream_out(mpcf_ctx* c, int kind, float* out)
+{
+    if (!c || !out) return fail(MPCF_EINVAL, "mpcf_stream_out: NULL argument");
+    const int nc = mpcf_stream_channels(kind);
+    if (!nc) return fail(MPCF_EINVAL, "mpcf_stream_out: unknown streamer kind");
+    const size_t n = (size_t)c->g.nx * c->g.ny * c->g.nz;
+    // binary32 storage streams through the bit-exact build's arithmetic
+    const Launchers* L = (c->flags & MPCF_FLAG_FP64) ? c->L : &kParity;
+    L->stream(c->g, c->u[c->cur], (float*)c->acc, kind, 1, c->stream);
+    LAUNCH_CHECK("stream out");
+    HIP_TRY(hipMemcpyAsync(out, c->acc, n * nc * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_stream_in(mpcf_ctx* c, const float* in)
+{
+    if (!c || !in) return fail(MPCF_EINVAL, "mpcf_stream_in: NULL argument");
+    const size_t n = (size_t)c->g.nx * c->g.ny * c->g.nz;
+    const Launchers* L = (c->flags & MPCF_FLAG_FP64) ? c->L : &kParity;
+    HIP_TRY(hipMemcpyAsync(c->acc, in, n * 7 * sizeof(float), hipMemcpyHostToDevice, c->stream));
+    L->stream(c->g, c->u[c->cur], (float*)c->acc, MPCF_STREAM_SAVE, 0, c->stream);
+    LAUNCH_CHECK("stream in");
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    return MPCF_OK;
+}
+
 extern "C" int mpcf_load_ghosts(mpcf_ctx* c)
 {
     if (!c) return fail(MPCF_EINVAL, "mpcf_load_ghosts: NULL context");

@@ -49,6 +49,7 @@ struct Launchers
     decltype(&mpcf::parity::launch_sweep) sweep;
     decltype(&mpcf::parity::launch_update) update;
     decltype(&mpcf::parity::launch_pack) pack;
+    decltype(&mpcf::parity::launch_stream) stream;
     decltype(&mpcf::parity::launch_test_weno) test_weno;
     decltype(&mpcf::parity::launch_test_face) test_face;
     decltype(&mpcf::parity::batch_sos) b_sos;
@@ -58,7 +59,8 @@ struct Launchers
 
 #define MPCF_LAUNCHERS(NS)                                                                           \
     {mpcf::NS::launch_ghosts, mpcf::NS::launch_sos, mpcf::NS::launch_sweep, mpcf::NS::launch_update, \
-     mpcf::NS::launch_pack, mpcf::NS::launch_test_weno, mpcf::NS::launch_test_face,                   \
+     mpcf::NS::launch_pack, mpcf::NS::launch_stream, mpcf::NS::launch_test_weno,                      \
+     mpcf::NS::launch_test_face,                                                                      \
      mpcf::NS::batch_sos, mpcf::NS::batch_xy, mpcf::NS::batch_z}
 
 inline const Launchers kParity = MPCF_LAUNCHERS(parity);

@@ -682,6 +682,54 @@ void k_pack(const Geom g, Real* __restrict__ pad, Dense* __restrict__ dense, con
     }
 }
 
+// Streamers of the reference's dumps (Types.h:188-280) between the padded
+// state and a dense channel-interleaved binary32 array [Z][Y][X][C]
+// (HDF5Dumper_MPI.h:118-141):
+//   KIND 0 myTensorialStreamer, 9 channels: rho, u, v, w, p, G, P, 0, 0 with
+//          u = rho u / rho in the storage type and p = (E - 0.5 |rho u|^2 / rho
+//          - P) / G evaluated in binary64 from the stored values (the
+//          reference's 0.5 is a double literal)
+//   KIND 1 mySaveStreamer, 7 channels: the conserved fields (also read back)
+//   KIND 2 myScalarStreamer, 1 channel: rho
+template <int KIND, int OUT>
+__global__ __launch_bounds__(256)
+void k_stream(const Geom g, Real* __restrict__ u, float* __restrict__ dense, const int64_t fs)
+{
+    constexpr int NC = KIND == 0 ? 9 : KIND == 1 ? 7 : 1;
+    const int rows = g.ny * g.nz;
+    for (int row = blockIdx.x * 4 + threadIdx.y; row < rows; row += gridDim.x * 4) {
+        const int e0 = g.idx(0, row % g.ny, row / g.ny);
+        const int64_t d0 = (int64_t)g.nx * row;
+        for (int x = threadIdx.x; x < g.nx; x += 64) {
+            const int e = e0 + x;
+            float* o = dense + (d0 + x) * NC;
+            if (!OUT) {
+#pragma unroll
+                for (int q = 0; q < NVAR; ++q) u[q*fs + e] = (Real)o[q];
+                continue;
+            }
+            const Real r = u[e], ru = u[fs + e], rv = u[2*fs + e], rw = u[3*fs + e];
+            const Real E = u[4*fs + e], G = u[5*fs + e], P = u[6*fs + e];
+            if (KIND == 2) { o[0] = (float)r; continue; }
+            if (KIND == 1) {
+                o[0] = (float)r; o[1] = (float)ru; o[2] = (float)rv; o[3] = (float)rw;
+                o[4] = (float)E; o[5] = (float)G; o[6] = (float)P;
+                continue;
+            }
+            const Real ss = ru*ru + rv*rv + rw*rw;
+            o[0] = (float)r;
+            o[1] = (float)(ru / r);
+            o[2] = (float)(rv / r);
+            o[3] = (float)(rw / r);
+            o[4] = (float)(((double)E - 0.5*(double)ss/(double)r - (double)P) / (double)G);
+            o[5] = (float)G;
+            o[6] = (float)P;
+            o[7] = 0.0f;
+            o[8] = 0.0f;
+        }
+    }
+}
+
 // device-function test hooks (the counterpart of GPU::TestKernel,
 // GPUkernels.cu:1354-1382): run weno_cell / riemann on arrays
 __global__ void k_test_weno(const int n, const Real* __restrict__ in5, Real* __restrict__ m, Real* __restrict__ p)
@@ -876,6 +924,16 @@ void launch_pack(const Geom& g, void* pad, void* dense, const int to_padded, con
         if (to_padded) hipLaunchKernelGGL((k_pack<1, float>), grid, block, 0, s, g, P, (float*)dense, g.fstride, nq);
         else           hipLaunchKernelGGL((k_pack<0, float>), grid, block, 0, s, g, P, (float*)dense, g.fstride, nq);
     }
+}
+
+void launch_stream(const Geom& g, void* pad, float* dense, const int kind, const int out, hipStream_t s)
+{
+    Real* P = (Real*)pad;
+    const dim3 grid(rows_grid(g)), block(64, 4);
+    if (!out)           hipLaunchKernelGGL((k_stream<1, 0>), grid, block, 0, s, g, P, dense, g.fstride);
+    else if (kind == 0) hipLaunchKernelGGL((k_stream<0, 1>), grid, block, 0, s, g, P, dense, g.fstride);
+    else if (kind == 1) hipLaunchKernelGGL((k_stream<1, 1>), grid, block, 0, s, g, P, dense, g.fstride);
+    else                hipLaunchKernelGGL((k_stream<2, 1>), grid, block, 0, s, g, P, dense, g.fstride);
 }
 
 void launch_test_weno(const int n, const void* in5v, void* mv, void* pv, hipStream_t s)

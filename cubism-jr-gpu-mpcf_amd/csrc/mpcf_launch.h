@@ -61,6 +61,7 @@ void launch_sos(const Geom& g, const void* u, StepState* st, int cpu_form, int w
 void launch_sweep(const Geom& g, int dir, const SweepParams& p, int stage, hipStream_t s);                \
 void launch_update(const Geom& g, void* u, const void* R, double b, hipStream_t s);                        \
 void launch_pack(const Geom& g, void* pad, void* dense, int to_padded, int nq, int dense_f64, hipStream_t s); \
+void launch_stream(const Geom& g, void* pad, float* dense, int kind, int out, hipStream_t s);             \
 void launch_test_weno(int n, const void* in5, void* m, void* p, hipStream_t s);                            \
 void launch_test_face(int n, int dir, const void* st, void* out, hipStream_t s);                           \
 void batch_sos(const BatchDev& B, int blocks, int cpu_form, double cfl, double dt_max, hipStream_t s);    \

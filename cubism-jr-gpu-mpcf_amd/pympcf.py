@@ -39,7 +39,23 @@ EXPORTS = [
     "mpcf_batch_autotune", "mpcf_batch_query", "mpcf_upload_f64", "mpcf_download_f64",
     "mpcf_last_sos64", "mpcf_compare", "mpcf_integrals", "mpcf_server_create", "mpcf_server_submit",
     "mpcf_server_poll", "mpcf_server_wait", "mpcf_server_stats", "mpcf_server_clock", "mpcf_server_destroy",
+    "mpcf_sic_defaults", "mpcf_sic_post_shock", "mpcf_read_cloud_config", "mpcf_read_cloud", "mpcf_ic_sic_cloud",
+    "mpcf_stream_channels", "mpcf_stream_out", "mpcf_stream_in", "mpcf_write_xdmf", "mpcf_write_save_info",
+    "mpcf_read_save_info", "mpcf_dump", "mpcf_save", "mpcf_restart",
 ]
+
+# streamer kinds of the dumps (mpcf_stream_out)
+STREAM_TENSOR, STREAM_SAVE, STREAM_SCALAR = 0, 1, 2
+
+
+class SicParams(C.Structure):
+    """mpcf_sic_params: the shock-induced cloud collapse parameters
+    (Sim_SICCloudMPI.cpp:53-77 for the defaults and option names)."""
+    _fields_ = [("rho0", C.c_double), ("u0", C.c_double), ("p0", C.c_double),
+                ("rhoB", C.c_double), ("uB", C.c_double), ("pB", C.c_double),
+                ("pressure_ratio", C.c_double), ("shock_n", C.c_double * 3), ("shock_s", C.c_double * 3),
+                ("g1", C.c_double), ("g2", C.c_double), ("pc1", C.c_double), ("pc2", C.c_double),
+                ("eps", C.c_double)]
 
 # serving policies and the eager-launch flag (mpcf_server_create)
 SERVE_ALL, SERVE_FIFO_FILL, SERVE_SRPT_FILL, SERVE_HYBRID, SERVE_FIFO_SLACK = 0, 1, 2, 3, 4
@@ -124,11 +140,27 @@ def lib():
         L.mpcf_server_stats.argtypes = [P, P]
         L.mpcf_server_clock.argtypes = []
         L.mpcf_server_destroy.argtypes = [P]
+        L.mpcf_sic_defaults.argtypes = [P]
+        L.mpcf_sic_post_shock.argtypes = [P, P]
+        L.mpcf_read_cloud_config.argtypes = [C.c_char_p, P]
+        L.mpcf_read_cloud.argtypes = [C.c_char_p, P, I, P]
+        L.mpcf_ic_sic_cloud.argtypes = [I, I, I, D, P, P, I, P]
+        S_ = C.c_char_p
+        L.mpcf_stream_channels.argtypes = [I]
+        L.mpcf_stream_out.argtypes = [P, I, P]
+        L.mpcf_stream_in.argtypes = [P, P]
+        L.mpcf_write_xdmf.argtypes = [S_, S_, I, I, I, I, D, I, S_]
+        L.mpcf_write_save_info.argtypes = [S_, D, I, I]
+        L.mpcf_read_save_info.argtypes = [S_, P, P, P]
+        L.mpcf_dump.argtypes = [P, I, S_, S_, I, D]
+        L.mpcf_save.argtypes = [P, S_, D, I, I, D]
+        L.mpcf_restart.argtypes = [P, S_, P, P, P]
         for name in EXPORTS:
             if name not in ("mpcf_last_error", "mpcf_version", "mpcf_get_stream", "mpcf_batch_get_stream",
-                            "mpcf_server_clock"):
+                            "mpcf_server_clock", "mpcf_sic_defaults"):
                 getattr(L, name).restype = I
         L.mpcf_server_clock.restype = D
+        L.mpcf_sic_defaults.restype = None
         _LIB = L
     return _LIB
 
@@ -227,6 +259,31 @@ class Block:
         out = (C.c_double * NVAR)()
         _check(lib().mpcf_integrals(self._h, which, C.cast(out, C.c_void_p)))
         return list(out)
+
+    def stream_out(self, kind=STREAM_TENSOR):
+        """The current solution through a dump streamer: (NZ, NY, NX, C) binary32."""
+        nc = lib().mpcf_stream_channels(kind)
+        out = np.empty(self.shape + (nc,), np.float32)
+        _check(lib().mpcf_stream_out(self._h, kind, _ptr(out)))
+        return out
+
+    def stream_in(self, arr):
+        a = np.ascontiguousarray(arr, dtype=np.float32)
+        assert a.shape == self.shape + (7,), a.shape
+        _check(lib().mpcf_stream_in(self._h, _ptr(a)))
+
+    def dump(self, kind, dirname, name, step=0):
+        _check(lib().mpcf_dump(self._h, kind, str(dirname).encode(), name.encode(), int(step), self.h))
+
+    def save(self, dirname, t, step, fcount=0):
+        _check(lib().mpcf_save(self._h, str(dirname).encode(), float(t), int(step), int(fcount), self.h))
+
+    def restart(self, dirname):
+        """Read a checkpoint; returns (t, step, fcount)."""
+        t, st, fc = C.c_double(), C.c_int(), C.c_int()
+        _check(lib().mpcf_restart(self._h, str(dirname).encode(), C.cast(C.byref(t), C.c_void_p),
+                                  C.cast(C.byref(st), C.c_void_p), C.cast(C.byref(fc), C.c_void_p)))
+        return t.value, st.value, fc.value
 
     def last_sos64(self):
         s = C.c_double()
@@ -372,6 +429,72 @@ def ic_bubble_collapse(nx, ny, nz, radius, pratio, centre=(0.5, 0.5, 0.5)):
     f = _new_fields(nx, ny, nz)
     _check(lib().mpcf_ic_bubble_collapse(nx, ny, nz, float(radius), float(pratio), *map(float, centre), _fields(f)))
     return f
+
+
+def sic_params(**kw):
+    """SicParams with the reference's defaults, overridden by keyword."""
+    p = SicParams()
+    lib().mpcf_sic_defaults(C.byref(p))
+    for k, v in kw.items():
+        if k in ("shock_n", "shock_s"):
+            getattr(p, k)[:] = [float(x) for x in v]
+        else:
+            setattr(p, k, float(v))
+    return p
+
+
+def sic_post_shock(p=None):
+    """(rho1, u1, p1, mach, c0) of the shock (Sim_SICCloudMPI.cpp:87-104)."""
+    p = p or sic_params()
+    out = (C.c_double * 5)()
+    _check(lib().mpcf_sic_post_shock(C.byref(p), C.cast(out, C.c_void_p)))
+    return tuple(out)
+
+
+def read_cloud_config(path):
+    out = (C.c_double * 11)()
+    _check(lib().mpcf_read_cloud_config(str(path).encode(), C.cast(out, C.c_void_p)))
+    keys = ("n_shapes", "n_small", "min_rad", "max_rad")
+    d = {k: out[i] for i, k in enumerate(keys)}
+    d["n_shapes"], d["n_small"] = int(d["n_shapes"]), int(d["n_small"])
+    d["seed_s"], d["seed_e"], d["n_sensors"] = tuple(out[4:7]), tuple(out[7:10]), int(out[10])
+    return d
+
+
+def read_cloud(path, cap=1 << 20):
+    """cloud.dat spheres as an (n, 4) array of x, y, z, r."""
+    buf = np.empty((cap, 4), np.float64)
+    n = C.c_int()
+    _check(lib().mpcf_read_cloud(str(path).encode(), _ptr(buf), cap, C.cast(C.byref(n), C.c_void_p)))
+    return buf[:min(n.value, cap)].copy()
+
+
+def ic_sic_cloud(nx, ny, nz, h=None, params=None, shapes=None):
+    """The shock-induced cloud collapse initial condition (27-point Simpson
+    averages of set_IC, Sim_SICCloudMPI.cpp:247-357)."""
+    f = _new_fields(nx, ny, nz)
+    p = params or sic_params()
+    sh = np.ascontiguousarray(np.zeros((0, 4)) if shapes is None else shapes, dtype=np.float64).reshape(-1, 4)
+    hh = h if h is not None else 1.0 / max(nx, ny, nz)
+    _check(lib().mpcf_ic_sic_cloud(nx, ny, nz, float(hh), C.byref(p), _ptr(sh) if len(sh) else None, len(sh),
+                                   _fields(f)))
+    return f
+
+
+def write_save_info(path, t, step, fcount):
+    _check(lib().mpcf_write_save_info(str(path).encode(), float(t), int(step), int(fcount)))
+
+
+def read_save_info(path):
+    t, st, fc = C.c_double(), C.c_int(), C.c_int()
+    _check(lib().mpcf_read_save_info(str(path).encode(), C.cast(C.byref(t), C.c_void_p),
+                                     C.cast(C.byref(st), C.c_void_p), C.cast(C.byref(fc), C.c_void_p)))
+    return t.value, st.value, fc.value
+
+
+def write_xdmf(path, raw_name, nx, ny, nz, channels, h, step, attribute):
+    _check(lib().mpcf_write_xdmf(str(path).encode(), raw_name.encode(), nx, ny, nz, channels, float(h), int(step),
+                                 attribute.encode()))
 
 
 def case_bytes(nx, ny, nz):

@@ -230,6 +230,84 @@ int mpcf_batch_autotune(mpcf_batch* batch, int steps, double cfl, int* best_len)
  * [4] steps since the last autotune, [5] tuned extents. */
 int mpcf_batch_query(mpcf_batch* batch, long long out[6]);
 
+/* ------------------------------------------------------------------------
+ * Dumps and checkpoints (source/IO/HDF5Dumper_MPI.h:30-186, Types.h:188-280,
+ * Sim_SteadyStateMPI.cpp:138-179).  The streamers turn the resident state
+ * into the reference's dense channel-interleaved [Z][Y][X][C] binary32 array;
+ * dumps are written as raw data with the reference's XDMF wrapper (a Binary
+ * DataItem in place of the HDF5 dataset).
+ * ------------------------------------------------------------------------ */
+enum {
+    MPCF_STREAM_TENSOR = 0,  /* myTensorialStreamer: rho, u, v, w, p, G, P, 0, 0 (9 channels) */
+    MPCF_STREAM_SAVE = 1,    /* mySaveStreamer: the 7 conserved fields                       */
+    MPCF_STREAM_SCALAR = 2   /* myScalarStreamer: rho                                        */
+};
+
+/* Channels of a streamer kind (0 for an unknown kind). */
+int mpcf_stream_channels(int kind);
+
+/* The current solution through a streamer into out[NX*NY*NZ*C] (synchronizes). */
+int mpcf_stream_out(mpcf_ctx* ctx, int kind, float* out);
+
+/* A save-streamer array (7 channels) into the current solution (synchronizes). */
+int mpcf_stream_in(mpcf_ctx* ctx, const float* in);
+
+/* The reference's XDMF wrapper for a [Z][Y][X][C] binary32 raw file. */
+int mpcf_write_xdmf(const char* xmf_path, const char* raw_name, int nx, int ny, int nz, int channels, double h,
+                    int step, const char* attribute);
+
+/* save.info: t (15 significant digits), step and dump counter, one per line. */
+int mpcf_write_save_info(const char* path, double t, int step, int fcount);
+int mpcf_read_save_info(const char* path, double* t, int* step, int* fcount);
+
+/* <dir>/<name>.raw + <dir>/<name>.xmf of the current solution (Sim_SteadyStateMPI::_dump). */
+int mpcf_dump(mpcf_ctx* ctx, int kind, const char* dir, const char* name, int step, double h);
+
+/* <dir>/save.info + <dir>/save.data.{raw,xmf} (Sim_SteadyStateMPI::_save). */
+int mpcf_save(mpcf_ctx* ctx, const char* dir, double t, int step, int fcount, double h);
+
+/* Read <dir>/save.info and save.data.raw into the context (Sim_SteadyStateMPI::_restart). */
+int mpcf_restart(mpcf_ctx* ctx, const char* dir, double* t, int* step, int* fcount);
+
+/* ------------------------------------------------------------------------
+ * Shock-induced cloud collapse input (Sim_SICCloudMPI, source/Sim/
+ * Sim_SICCloudMPI.cpp:43-357): parameters with the reference's defaults and
+ * command-line meaning, the post-shock state, the cloud files, and the
+ * 27-point Simpson initial condition.  Host-side, binary64 arithmetic.
+ * ------------------------------------------------------------------------ */
+typedef struct {
+    double rho0, u0, p0;        /* pre-shock liquid (-rho0 -u0 -p0)                        */
+    double rhoB, uB, pB;        /* gas in the bubbles (-rhoB -uB -pB)                      */
+    double pressure_ratio;      /* p1/p0 across the shock (-pressureratio)                 */
+    double shock_n[3];          /* shock normal, normalized by the callee (-shockNx/Ny/Nz) */
+    double shock_s[3];          /* a point on the shock plane (-shockSx/Sy/Sz)             */
+    double g1, g2, pc1, pc2;    /* liquid / gas stiffened-gas gamma and pc (-g1 -g2 -pc1 -pc2) */
+    double eps;                 /* bubble mollifier half-width (SimTools::EPSILON; the
+                                   reference's cloud case leaves it 0: sharp interfaces)   */
+} mpcf_sic_params;
+
+/* Fill p with the reference's defaults (Sim_SICCloudMPI.cpp:53-77). */
+void mpcf_sic_defaults(mpcf_sic_params* p);
+
+/* Post-shock liquid state (Sim_SICCloudMPI.cpp:87-104): out = rho1, u1, p1,
+ * shock Mach number, pre-shock speed of sound c0. */
+int mpcf_sic_post_shock(const mpcf_sic_params* p, double out[5]);
+
+/* cloud_config.dat (Sim_SICCloudMPI.cpp:134-160): out = n_shapes, n_small,
+ * min_rad, max_rad, seed_s[3], seed_e[3], n_sensors (0 when absent). */
+int mpcf_read_cloud_config(const char* path, double out[11]);
+
+/* cloud.dat (Sim_SICCloudMPI.h:148-181), lines "idx x y z r": writes up to
+ * `cap` spheres as (x, y, z, r) into shapes[4*cap], the count read into *n. */
+int mpcf_read_cloud(const char* path, double* shapes, int cap, int* n);
+
+/* The initial condition of _ic_quad (Sim_SICCloudMPI.cpp:306-357) on an
+ * NX x NY x NZ block with spacing h (cell centres at h(i+1/2)): every cell is
+ * the 27-point Simpson average of set_IC over the cell.  shapes: nshapes
+ * spheres (x, y, z, r).  fields: seven NX*NY*NZ binary32 arrays. */
+int mpcf_ic_sic_cloud(int nx, int ny, int nz, double h, const mpcf_sic_params* p, const double* shapes, int nshapes,
+                      float* const fields[7]);
+
 /* Operation count of the scheme as implemented here, from a host run of the
  * same per-cell code with a counting scalar (binary32/64 add, sub, mul, div,
  * sqrt = 1 FLOP each; min/max and conversions listed apart):

@@ -49,3 +49,49 @@ def test_driver_tube_bitexact(tmp_path, mpcf, oracle):
         tref += ob.step(1.0 / n, 0.3, 1e30, [0] * 6)[0]
     assert t == tref
     assert np.array_equal(u, ob.data)
+
+
+def write_cloud(d):
+    (d / "cloud_config.dat").write_text("2 0\n0.05 0.12\n0.2 0.2 0.2\n0.8 0.8 0.8\n")
+    (d / "cloud.dat").write_text("0 0.45 0.5 0.5 0.12\n1 0.7 0.45 0.55 0.07\n")
+
+
+def test_driver_sic_needs_cloud_files(tmp_path):
+    r = run(["-sim", "sic", "-n", "8", "-steps", "1"], cwd=tmp_path)
+    assert r.returncode == 1 and "cloud_config.dat" in r.stderr
+
+
+@pytest.mark.gpu
+def test_driver_sic_dump_save_restart(tmp_path, mpcf):
+    """-sim sic from cloud files (initial state = mpcf_ic_sic_cloud), tensorial
+    dumps, and a checkpoint/restart that continues exactly like the
+    uninterrupted run."""
+    write_cloud(tmp_path)
+    (tmp_path / "a").mkdir()
+    (tmp_path / "b").mkdir()
+    n = 24
+    common = ["-sim", "sic", "-n", str(n), "-pressureratio", "20", "-shockSx", "0.15"]
+    print("[driver] sic 6 steps with dumps", flush=True)
+    r = run(common + ["-steps", "6", "-dumpperiod", "3", "-fpath", "a", "-dump", "full.bin"], cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert (tmp_path / "a" / "data_0000.raw").stat().st_size == n ** 3 * 9 * 4
+    assert (tmp_path / "a" / "data_0001.xmf").exists()
+    print("[driver] sic 4 steps + save, then restart for 2", flush=True)
+    r = run(common + ["-steps", "4", "-saveperiod", "4", "-fpath", "b"], cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    r = run(common + ["-steps", "2", "-restart", "1", "-fpath", "b", "-dump", "resumed.bin"], cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    s_full, t_full, u_full = read_dump(tmp_path / "full.bin")
+    s_res, t_res, u_res = read_dump(tmp_path / "resumed.bin")
+    assert np.array_equal(u_res, u_full)
+    assert t_res == pytest.approx(t_full, rel=1e-14)
+    # the first dump is the state after step 3 of the same initial condition
+    prm = mpcf.sic_params(pressure_ratio=20, shock_s=(0.15, 0, 0))
+    f0 = mpcf.ic_sic_cloud(n, n, n, params=prm, shapes=mpcf.read_cloud(tmp_path / "cloud.dat"))
+    with mpcf.Block(n, n, n, 1.0 / n) as b:
+        b.data = f0
+        for _ in range(3):
+            b.step(0.3)
+        ten = b.stream_out(mpcf.STREAM_TENSOR)
+    raw = np.fromfile(tmp_path / "a" / "data_0000.raw", dtype="<f4").reshape(n, n, n, 9)
+    assert np.array_equal(raw, ten)
