@@ -10,6 +10,7 @@ Objects:
   reduce.o  mpcf_reduce.hip  (binary64 block comparisons and integrals)
   capi.o    the C-ABI (include/mpcf_hip.h)
   ic.o      initial conditions;  sic.o  the shock-induced cloud input and IC
+  sim_ic.o  the Sod, 2D shock-bubble and static-index drivers' ICs (-ffp-contract=off)
   io.o      dumps (raw + XDMF) and checkpoints (save.info + save.data)
   flops.o   host instantiations of the scheme
 and the C++ driver mpcf_run (csrc/mpcf_run.cpp over csrc/mpcf.hpp).
@@ -70,6 +71,7 @@ def build(verbose: bool = False) -> str:
         ("serve.o", "mpcf_serve.cpp", ["-x", "hip"]),
         ("ic.o", "ic.cpp", ["-x", "hip"]),
         ("sic.o", "sic.cpp", ["-x", "hip"]),
+        ("sim_ic.o", "sim_ic.cpp", ["-x", "hip", "-ffp-contract=off"]),
         ("io.o", "io.cpp", ["-x", "hip"]),
         ("flops.o", "flops.cpp", ["-x", "hip", "-ffp-contract=off"]),
     ]

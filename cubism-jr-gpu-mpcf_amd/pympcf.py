@@ -41,7 +41,9 @@ EXPORTS = [
     "mpcf_server_poll", "mpcf_server_wait", "mpcf_server_stats", "mpcf_server_clock", "mpcf_server_destroy",
     "mpcf_sic_defaults", "mpcf_sic_post_shock", "mpcf_read_cloud_config", "mpcf_read_cloud", "mpcf_ic_sic_cloud",
     "mpcf_stream_channels", "mpcf_stream_out", "mpcf_stream_in", "mpcf_write_xdmf", "mpcf_write_save_info",
-    "mpcf_read_save_info", "mpcf_dump", "mpcf_save", "mpcf_restart",
+    "mpcf_read_save_info", "mpcf_dump", "mpcf_save", "mpcf_restart", "mpcf_post_shock_ratio",
+    "mpcf_sod_defaults", "mpcf_ic_sod", "mpcf_sbi_defaults", "mpcf_sbi_config", "mpcf_read_bubbles",
+    "mpcf_ic_sbi", "mpcf_ic_static", "mpcf_ic_steady",
 ]
 
 # streamer kinds of the dumps (mpcf_stream_out)
@@ -56,6 +58,24 @@ class SicParams(C.Structure):
                 ("pressure_ratio", C.c_double), ("shock_n", C.c_double * 3), ("shock_s", C.c_double * 3),
                 ("g1", C.c_double), ("g2", C.c_double), ("pc1", C.c_double), ("pc2", C.c_double),
                 ("eps", C.c_double)]
+
+
+
+class SodParams(C.Structure):
+    """mpcf_sod_params: the Sod tube of Sim_SodMPI (Sim_SodMPI.cpp:27-56)."""
+    _fields_ = [("x0", C.c_double), ("rho1", C.c_double), ("u1", C.c_double), ("p1", C.c_double),
+                ("rho2", C.c_double), ("u2", C.c_double), ("p2", C.c_double),
+                ("g1", C.c_double), ("g2", C.c_double), ("pc1", C.c_double), ("pc2", C.c_double),
+                ("reverse", C.c_int), ("dir", C.c_int)]
+
+
+class SbiParams(C.Structure):
+    """mpcf_sbi_params: the 2D shock-bubble case of Sim_2DSBIMPI
+    (Sim_2DSBIMPI.cpp:20-63, 95-108)."""
+    _fields_ = [("x0", C.c_double), ("mach", C.c_double), ("rho", C.c_double), ("u", C.c_double),
+                ("p", C.c_double), ("rhobubble", C.c_double), ("pbubble", C.c_double),
+                ("g1", C.c_double), ("g2", C.c_double), ("pc1", C.c_double), ("pc2", C.c_double),
+                ("mollfactor", C.c_int), ("reverse", C.c_int), ("dims", C.c_int * 3)]
 
 # serving policies and the eager-launch flag (mpcf_server_create)
 SERVE_ALL, SERVE_FIFO_FILL, SERVE_SRPT_FILL, SERVE_HYBRID, SERVE_FIFO_SLACK = 0, 1, 2, 3, 4
@@ -145,6 +165,15 @@ def lib():
         L.mpcf_read_cloud_config.argtypes = [C.c_char_p, P]
         L.mpcf_read_cloud.argtypes = [C.c_char_p, P, I, P]
         L.mpcf_ic_sic_cloud.argtypes = [I, I, I, D, P, P, I, P]
+        L.mpcf_post_shock_ratio.argtypes = [P, D, D, D, I, P]
+        L.mpcf_sod_defaults.argtypes = [P]
+        L.mpcf_ic_sod.argtypes = [I, I, I, D, P, P]
+        L.mpcf_sbi_defaults.argtypes = [P]
+        L.mpcf_sbi_config.argtypes = [C.c_char_p, P]
+        L.mpcf_read_bubbles.argtypes = [C.c_char_p, P, I, P]
+        L.mpcf_ic_sbi.argtypes = [I, I, I, D, P, P, I, P]
+        L.mpcf_ic_static.argtypes = [I, I, I, P, P]
+        L.mpcf_ic_steady.argtypes = [I, I, I, P, P]
         S_ = C.c_char_p
         L.mpcf_stream_channels.argtypes = [I]
         L.mpcf_stream_out.argtypes = [P, I, P]
@@ -157,10 +186,12 @@ def lib():
         L.mpcf_restart.argtypes = [P, S_, P, P, P]
         for name in EXPORTS:
             if name not in ("mpcf_last_error", "mpcf_version", "mpcf_get_stream", "mpcf_batch_get_stream",
-                            "mpcf_server_clock", "mpcf_sic_defaults"):
+                            "mpcf_server_clock", "mpcf_sic_defaults", "mpcf_post_shock_ratio",
+                            "mpcf_sod_defaults", "mpcf_sbi_defaults"):
                 getattr(L, name).restype = I
         L.mpcf_server_clock.restype = D
-        L.mpcf_sic_defaults.restype = None
+        for name in ("mpcf_sic_defaults", "mpcf_post_shock_ratio", "mpcf_sod_defaults", "mpcf_sbi_defaults"):
+            getattr(L, name).restype = None
         _LIB = L
     return _LIB
 
@@ -478,6 +509,115 @@ def ic_sic_cloud(nx, ny, nz, h=None, params=None, shapes=None):
     hh = h if h is not None else 1.0 / max(nx, ny, nz)
     _check(lib().mpcf_ic_sic_cloud(nx, ny, nz, float(hh), C.byref(p), _ptr(sh) if len(sh) else None, len(sh),
                                    _fields(f)))
+    return f
+
+
+def post_shock_ratio(pre, mach, gamma, pc=0.0, real32=False):
+    """(rho, u, p) behind a shock of Mach number `mach` running into pre =
+    (rho, u, p) (SimTools::getPostShockRatio, Types.h:101-109); real32 rounds
+    like the reference's binary32 build."""
+    a = (C.c_double * 3)(*map(float, pre))
+    out = (C.c_double * 3)()
+    lib().mpcf_post_shock_ratio(C.cast(a, C.c_void_p), float(mach), float(gamma), float(pc), int(bool(real32)),
+                                C.cast(out, C.c_void_p))
+    return tuple(out)
+
+
+def _set_fields(p, kw, ints):
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise TypeError(f"unknown parameter {k!r}")
+        if k == "dims":
+            p.dims[:] = [int(x) for x in v]
+        else:
+            setattr(p, k, int(v) if k in ints else float(v))
+    return p
+
+
+def sod_params(**kw):
+    """SodParams with the reference's defaults (Sim_SodMPI.cpp:27-40),
+    overridden by keyword; config="x"|"y"|"z" sets the tube axis."""
+    p = SodParams()
+    lib().mpcf_sod_defaults(C.byref(p))
+    conf = kw.pop("config", None)
+    if conf is not None:
+        if conf not in ("x", "y", "z"):
+            raise ValueError(f"-config {conf} is not possible")
+        p.dir = "xyz".index(conf)
+    return _set_fields(p, kw, ("reverse", "dir"))
+
+
+def ic_sod(nx, ny, nz, h=None, params=None):
+    """The Sim_SodMPI initial condition (Sim_SodMPI.cpp:58-95)."""
+    f = _new_fields(nx, ny, nz)
+    p = params or sod_params()
+    hh = h if h is not None else 1.0 / max(nx, ny, nz)
+    _check(lib().mpcf_ic_sod(nx, ny, nz, float(hh), C.byref(p), _fields(f)))
+    return f
+
+
+def sbi_config(conf):
+    """dims (principal, minor, dummy) for -config xy|xz|yx|yz|zx|zy."""
+    d = (C.c_int * 3)()
+    _check(lib().mpcf_sbi_config(str(conf).encode(), C.cast(d, C.c_void_p)))
+    return tuple(d)
+
+
+def sbi_params(**kw):
+    """SbiParams with the reference's defaults (Sim_2DSBIMPI.cpp:95-108),
+    overridden by keyword; config="xy"|... sets dims."""
+    p = SbiParams()
+    lib().mpcf_sbi_defaults(C.byref(p))
+    conf = kw.pop("config", None)
+    if conf is not None:
+        p.dims[:] = sbi_config(conf)
+    return _set_fields(p, kw, ("mollfactor", "reverse", "dims"))
+
+
+def sbi_bc(dims):
+    """Boundary kinds of the 2D shock-bubble case: reflecting on the two faces
+    normal to the minor axis dims[1], absorbing elsewhere
+    (GPUlab2DSBI_{x,y,z}reflect, Sim_2DSBIMPI.h:27-81, Sim_2DSBIMPI.cpp:66-84)."""
+    return [BC_REFLECTING if f // 2 == dims[1] else BC_ABSORBING for f in range(6)]
+
+
+def sbi_bubble_file(dims):
+    """The bubble file the reference reads for dims (Sim_2DSBIMPI.cpp:124-131)."""
+    return "xyz"[dims[0]] + "xyz"[dims[1]] + "bubbles.dat"
+
+
+def read_bubbles(path, cap=1 << 20):
+    """A <config>bubbles.dat file as an (n, 4) array of x, y, z, r."""
+    buf = np.empty((cap, 4), np.float64)
+    n = C.c_int()
+    _check(lib().mpcf_read_bubbles(str(path).encode(), _ptr(buf), cap, C.cast(C.byref(n), C.c_void_p)))
+    return buf[:min(n.value, cap)].copy()
+
+
+def ic_sbi(nx, ny, nz, h=None, params=None, bubbles=None):
+    """The Sim_2DSBIMPI initial condition (Sim_2DSBIMPI.cpp:86-233);
+    bubbles: (n, 4) array of x, y, z, r."""
+    f = _new_fields(nx, ny, nz)
+    p = params or sbi_params()
+    b = np.ascontiguousarray(np.zeros((0, 4)) if bubbles is None else bubbles, dtype=np.float64).reshape(-1, 4)
+    hh = h if h is not None else 1.0 / max(nx, ny, nz)
+    _check(lib().mpcf_ic_sbi(nx, ny, nz, float(hh), C.byref(p), _ptr(b) if len(b) else None, len(b), _fields(f)))
+    return f
+
+
+def ic_static(nx, ny, nz, dims=(0, 1, 2)):
+    """The Sim_StaticIC layout probe (Sim_StaticIC.cpp:76-97)."""
+    f = _new_fields(nx, ny, nz)
+    d = (C.c_int * 3)(*map(int, dims))
+    _check(lib().mpcf_ic_static(nx, ny, nz, C.cast(d, C.c_void_p), _fields(f)))
+    return f
+
+
+def ic_steady(nx, ny, nz, rho=1.0, u=0.0, v=0.0, w=0.0, p=1.0, g=1.4, pc=0.0):
+    """The Sim_SteadyStateMPI uniform state (Sim_SteadyStateMPI.cpp:95-134)."""
+    f = _new_fields(nx, ny, nz)
+    st = (C.c_double * 7)(*map(float, (rho, u, v, w, p, g, pc)))
+    _check(lib().mpcf_ic_steady(nx, ny, nz, C.cast(st, C.c_void_p), _fields(f)))
     return f
 
 

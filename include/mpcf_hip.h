@@ -308,6 +308,76 @@ int mpcf_read_cloud(const char* path, double* shapes, int cap, int* n);
 int mpcf_ic_sic_cloud(int nx, int ny, int nz, double h, const mpcf_sic_params* p, const double* shapes, int nshapes,
                       float* const fields[7]);
 
+/* ------------------------------------------------------------------------
+ * The other drivers' initial conditions (host-side, binary64 arithmetic,
+ * binary32 stores; cell centres at h(i+1/2); fields: seven NX*NY*NZ
+ * binary32 arrays, x fastest).
+ * ------------------------------------------------------------------------ */
+
+/* SimTools::getPostShockRatio (source/Types.h:101-109): the state behind a
+ * shock of Mach number `mach` running into pre = (rho, u, p) of a stiffened
+ * gas (gamma, pc); post = (rho, u, p).  The density and pressure ratios are
+ * the ideal-gas ones (pc enters only the speeds of sound).  real32 != 0 rounds
+ * like the reference's default build (Real = binary32: inputs, Mach-number
+ * square, speeds of sound and results in binary32), as mpcf_ic_sbi does. */
+void mpcf_post_shock_ratio(const double pre[3], double mach, double gamma, double pc, int real32, double post[3]);
+
+/* Sim_SodMPI (source/Sim/Sim_SodMPI.cpp:19-95): two states split at x0 along
+ * one axis.  Like the reference, u1/u2 are velocities, the stored momentum is
+ * rho*u along that axis and E = G p + P + (rho u)^2 / (2 rho). */
+typedef struct {
+    double x0;                  /* interface position (-x0)                               */
+    double rho1, u1, p1;        /* state at pos < x0 (-rho1 -u1 -p1)                       */
+    double rho2, u2, p2;        /* state at pos > x0 (-rho2 -u2 -p2)                       */
+    double g1, g2, pc1, pc2;    /* stiffened-gas gamma and pc of the two states            */
+    int reverse;                /* -reverse: state 1 at pos > x0                           */
+    int dir;                    /* tube axis, 0/1/2 for -config x/y/z                      */
+} mpcf_sod_params;
+
+/* Fill p with the reference's defaults (Sim_SodMPI.cpp:27-40). */
+void mpcf_sod_defaults(mpcf_sod_params* p);
+
+int mpcf_ic_sod(int nx, int ny, int nz, double h, const mpcf_sod_params* p, float* const fields[7]);
+
+/* Sim_2DSBIMPI (source/Sim/Sim_2DSBIMPI.cpp:86-233): a planar shock at x0
+ * along dims[0] (post-shock state from mpcf_post_shock_ratio with g1, pc1)
+ * running into gas 1 holding cylindrical bubbles of gas 2 (axes along
+ * dims[2]), mollified over mollfactor*sqrt(3)*h. */
+typedef struct {
+    double x0, mach;            /* shock position and Mach number (-x0 -mach)             */
+    double rho, u, p;           /* pre-shock gas (-rho -u -p)                              */
+    double rhobubble, pbubble;  /* bubble gas (-rhobubble -pbubble)                        */
+    double g1, g2, pc1, pc2;    /* stiffened-gas gamma and pc of the gas and the bubbles   */
+    int mollfactor;             /* -mollfactor                                             */
+    int reverse;                /* -reverse: point-mirror the block, momenta negated       */
+    int dims[3];                /* principal, minor, dummy axis (mpcf_sbi_config)          */
+} mpcf_sbi_params;
+
+/* Fill p with the reference's defaults (Sim_2DSBIMPI.cpp:95-108), -config xy. */
+void mpcf_sbi_defaults(mpcf_sbi_params* p);
+
+/* dims for -config xy|xz|yx|yz|zx|zy (Sim_2DSBIMPI.cpp:20-63); MPCF_EINVAL
+ * for anything else. */
+int mpcf_sbi_config(const char* conf, int dims[3]);
+
+/* <config>bubbles.dat (Sim_2DSBIMPI.cpp:126-155): the count, then "x y z r"
+ * per bubble; writes up to `cap` bubbles into bubbles[4*cap], the count read
+ * into *n. */
+int mpcf_read_bubbles(const char* path, double* bubbles, int cap, int* n);
+
+int mpcf_ic_sbi(int nx, int ny, int nz, double h, const mpcf_sbi_params* p, const double* bubbles, int nb,
+                float* const fields[7]);
+
+/* Sim_SteadyStateMPI (source/Sim/Sim_SteadyStateMPI.cpp:95-134): one
+ * uniform state, state = rho, u, v, w, p, gamma, pc (reference defaults
+ * 1, 0, 0, 0, 1, 1.4, 0). */
+int mpcf_ic_steady(int nx, int ny, int nz, const double state[7], float* const fields[7]);
+
+/* Sim_StaticIC (source/Sim/Sim_StaticIC.cpp:76-97): every field of a cell
+ * holds the cell's running index in the dims[0]-fastest order (a layout
+ * probe for dumps), rounded to binary32. */
+int mpcf_ic_static(int nx, int ny, int nz, const int dims[3], float* const fields[7]);
+
 /* Operation count of the scheme as implemented here, from a host run of the
  * same per-cell code with a counting scalar (binary32/64 add, sub, mul, div,
  * sqrt = 1 FLOP each; min/max and conversions listed apart):

@@ -95,3 +95,95 @@ def test_driver_sic_dump_save_restart(tmp_path, mpcf):
         ten = b.stream_out(mpcf.STREAM_TENSOR)
     raw = np.fromfile(tmp_path / "a" / "data_0000.raw", dtype="<f4").reshape(n, n, n, 9)
     assert np.array_equal(raw, ten)
+
+
+@pytest.mark.parametrize("args,code,msg", [
+    (["-sim", "sod", "-config", "w"], 2, "-config x|y|z"),
+    (["-sim", "2DSBIMPI", "-config", "xy"], 1, "xybubbles.dat"),
+    (["-sim", "2dsbi"], 1, "-config"),
+    (["-sim", "StaticIC"], 1, "-config"),
+    (["-sim", "steady", "-nsteps", "0"], 2, "needs -tend"),
+])
+def test_driver_case_argument_errors(tmp_path, args, code, msg):
+    r = run(args + ["-n", "8"], cwd=tmp_path)
+    assert r.returncode == code and msg in r.stderr, (r.returncode, r.stderr)
+
+
+def oracle_run(oracle, f, n, bc, steps=None, tend=1e30, dumpinterval=0.0, nsteps=0):
+    """The driver's loop (Sim_SteadyStateMPI.cpp:182-221) on the oracle: dt cut
+    to land on the dump times, (float) t == (float) tnextdump."""
+    ob = oracle.Block(n, n, n)
+    ob.data = f
+    t, done, tnext, dumps = 0.0, 0, dumpinterval, 1 if dumpinterval > 0 else 0
+    while (steps is None or done < steps) and t < tend:
+        dt_max = tend - t
+        if dumpinterval > 0 and tnext - t < dt_max:
+            dt_max = tnext - t
+        t += ob.step(1.0 / n, 0.3, dt_max, bc)[0]
+        done += 1
+        if dumpinterval > 0 and np.float32(t) == np.float32(tnext):
+            tnext += dumpinterval
+            dumps += 1
+        if nsteps and done == nsteps:
+            break
+    return t, done, ob.data, dumps + (1 if dumpinterval > 0 else 0)
+
+
+@pytest.mark.gpu
+def test_driver_sod_bitexact(tmp_path, mpcf, oracle):
+    n = 32
+    print("[driver] SodMPI -config y, 5 steps", flush=True)
+    r = run(["-sim", "SodMPI", "-config", "y", "-rho1", "2", "-u1", "0.5", "-n", str(n), "-steps", "5",
+             "-dump", "sod.bin"], cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    steps, t, u = read_dump(tmp_path / "sod.bin")
+    f = mpcf.ic_sod(n, n, n, params=mpcf.sod_params(config="y", rho1=2.0, u1=0.5))
+    tr, done, ur, _ = oracle_run(oracle, f, n, [0] * 6, steps=5)
+    assert steps == done == 5 and t == tr
+    assert np.array_equal(u, ur)
+
+
+@pytest.mark.gpu
+def test_driver_2dsbi_bitexact_reflecting(tmp_path, mpcf, oracle):
+    n = 32
+    (tmp_path / "yxbubbles.dat").write_text("2\n0.4 0.3 0.5 0.15\n0.6 0.7 0.5 0.1\n")
+    print("[driver] 2DSBIMPI -config yx, 6 steps", flush=True)
+    r = run(["-sim", "2DSBIMPI", "-config", "yx", "-n", str(n), "-steps", "6", "-dump", "sbi.bin"], cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    steps, t, u = read_dump(tmp_path / "sbi.bin")
+    prm = mpcf.sbi_params(config="yx")
+    f = mpcf.ic_sbi(n, n, n, params=prm, bubbles=mpcf.read_bubbles(tmp_path / "yxbubbles.dat"))
+    tr, done, ur, _ = oracle_run(oracle, f, n, mpcf.sbi_bc(tuple(prm.dims)), steps=6)
+    assert steps == done == 6 and t == tr
+    assert np.array_equal(u, ur)
+
+
+@pytest.mark.gpu
+def test_driver_dumpinterval_loop(tmp_path, mpcf, oracle):
+    """-tend / -dumpinterval / -nsteps 0: the reference's loop, dumps at the
+    initial state, every interval and the end."""
+    n = 32
+    (tmp_path / "d").mkdir()
+    print("[driver] SodMPI tend 0.05, dumpinterval 0.0125", flush=True)
+    r = run(["-sim", "sod", "-n", str(n), "-tend", "0.05", "-dumpinterval", "0.0125", "-nsteps", "0",
+             "-fpath", "d", "-dump", "end.bin"], cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    steps, t, u = read_dump(tmp_path / "end.bin")
+    f = mpcf.ic_sod(n, n, n)
+    tr, done, ur, ndumps = oracle_run(oracle, f, n, [0] * 6, tend=0.05, dumpinterval=0.0125)
+    assert steps == done and t == tr and np.float32(t) == np.float32(0.05)
+    assert np.array_equal(u, ur)
+    assert ndumps == 6
+    for k in range(ndumps):
+        assert (tmp_path / "d" / f"data_{k:04d}.raw").stat().st_size == n ** 3 * 9 * 4
+    assert not (tmp_path / "d" / f"data_{ndumps:04d}.raw").exists()
+
+
+@pytest.mark.gpu
+def test_driver_static_probe(tmp_path, mpcf):
+    n = 8
+    print("[driver] StaticIC -config zy", flush=True)
+    r = run(["-sim", "StaticIC", "-config", "zy", "-n", str(n), "-fpath", ".", "-fname", "probe"], cwd=tmp_path)
+    assert r.returncode == 0, r.stderr
+    raw = np.fromfile(tmp_path / "probe_0000.raw", dtype="<f4").reshape(n, n, n, 9)
+    assert np.array_equal(raw[..., 0], mpcf.ic_static(n, n, n, dims=(2, 1, 0))[0])
