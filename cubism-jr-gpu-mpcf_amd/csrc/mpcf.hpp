@@ -145,6 +145,18 @@ public:
 
     void synchronize() { check(mpcf_synchronize(ctx_)); }
 
+    // the device side of GPUlab's halo exchange (GPUlab.cpp:76-112,636-641):
+    // a face's three boundary cells out to a device buffer, a neighbour's
+    // into the ghost band (faces set to MPCF_BC_HALO); asynchronous
+    long long halo_count(const int face)
+    {
+        long long n = 0;
+        check(mpcf_halo_count(ctx_, face, &n));
+        return n;
+    }
+    void halo_export(const int face, void* dev_dst) { check(mpcf_halo_export(ctx_, face, dev_dst)); }
+    void halo_import(const int face, const void* dev_src) { check(mpcf_halo_import(ctx_, face, dev_src)); }
+
     // Sim_SteadyStateMPI::_dump / _save / _restart (Sim_SteadyStateMPI.cpp:138-179):
     // raw [Z][Y][X][C] + XDMF dumps of a streamer, and save.info + save.data
     void dump(int kind, const std::string& dir, const std::string& name, int step, double h)

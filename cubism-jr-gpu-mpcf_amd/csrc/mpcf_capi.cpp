@@ -91,7 +91,7 @@ void run_rhs(mpcf_ctx* c, const void* u, void* uo, double a, double b, double dt
     mpcf::SweepParams p;
     p.u = u; p.R = c->R; p.acc = c->acc; p.uo = uo;
     p.a = a; p.b = b; p.dtinvh = dtinvh; p.st = st;
-    p.reflect = mpcf::detail::reflect_mask(c->bc);
+    p.bcmask = mpcf::detail::bc_mask(c->bc);
     // the fused x+y sweep wins when the arithmetic is cheap (FAST); the split
     // pair wins when it is VALU-bound (bit-exact build), unless forced
     const bool split = (c->flags & MPCF_FLAG_SPLIT_XY) ? true
@@ -204,9 +204,54 @@ extern "C" int mpcf_set_bc(mpcf_ctx* c, const int bc[6])
 {
     if (!c || !bc) return fail(MPCF_EINVAL, "mpcf_set_bc: NULL argument");
     for (int i = 0; i < 6; ++i)
-        if (bc[i] != MPCF_BC_ABSORBING && bc[i] != MPCF_BC_REFLECTING)
+        if (bc[i] != MPCF_BC_ABSORBING && bc[i] != MPCF_BC_REFLECTING && bc[i] != MPCF_BC_HALO)
             return fail(MPCF_EINVAL, "mpcf_set_bc: unknown boundary kind");
     for (int i = 0; i < 6; ++i) c->bc[i] = bc[i];
+    return MPCF_OK;
+}
+
+namespace {
+
+int halo_args(mpcf_ctx* c, const int face, const void* buf, const char* who)
+{
+    if (!c || !buf) return fail(MPCF_EINVAL, std::string(who) + ": NULL argument");
+    if (face < 0 || face > 5) return fail(MPCF_EINVAL, std::string(who) + ": face must be 0..5");
+    return MPCF_OK;
+}
+
+long long halo_count(const mpcf_ctx* c, const int face)
+{
+    const int dir = face >> 1;
+    const long long area = dir == 0 ? (long long)c->g.ny * c->g.nz : dir == 1 ? (long long)c->g.nx * c->g.nz
+                                                                              : (long long)c->g.nx * c->g.ny;
+    return 7 * 3 * area;
+}
+
+}  // namespace
+
+extern "C" int mpcf_halo_count(mpcf_ctx* c, int face, long long* count)
+{
+    const int rc = halo_args(c, face, count, "mpcf_halo_count");
+    if (rc != MPCF_OK) return rc;
+    *count = halo_count(c, face);
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_halo_export(mpcf_ctx* c, int face, void* dst)
+{
+    const int rc = halo_args(c, face, dst, "mpcf_halo_export");
+    if (rc != MPCF_OK) return rc;
+    c->L->halo(c->g, c->u[c->cur], dst, face, 0, c->stream);
+    LAUNCH_CHECK("halo export");
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_halo_import(mpcf_ctx* c, int face, const void* src)
+{
+    const int rc = halo_args(c, face, src, "mpcf_halo_import");
+    if (rc != MPCF_OK) return rc;
+    c->L->halo(c->g, c->u[c->cur], const_cast<void*>(src), face, 1, c->stream);
+    LAUNCH_CHECK("halo import");
     return MPCF_OK;
 }
 
@@ -348,6 +393,9 @@ extern "C" int mpcf_step(mpcf_ctx* c, double cfl, double dt_max, double* dt)
     if (!c) return fail(MPCF_EINVAL, "mpcf_step: NULL context");
     if (!(cfl > 0) || !std::isfinite(cfl)) return fail(MPCF_EINVAL, "mpcf_step: cfl must be positive and finite");
     if (!(dt_max > 0)) return fail(MPCF_EINVAL, "mpcf_step: dt_max must be positive");
+    if (mpcf::detail::has_halo(c->bc))
+        return fail(MPCF_EINVAL, "mpcf_step: halo faces need an exchange before every stage; "
+                                 "drive the stages with mpcf_max_sos and mpcf_process_all");
     {
         Timed t(c, K_SOS);
         c->L->sos(c->g, c->u[c->cur], c->st, (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0, 1, cfl, c->h, dt_max, c->stream);
@@ -703,6 +751,8 @@ extern "C" int mpcf_batch_add(mpcf_batch* b, int nx, int ny, int nz, double h, c
                               const float* const fields[7], int* case_id)
 {
     if (!b || !fields || !case_id) return fail(MPCF_EINVAL, "mpcf_batch_add: NULL argument");
+    if (bc && mpcf::detail::has_halo(bc))
+        return fail(MPCF_EINVAL, "mpcf_batch_add: a batched case is a whole domain (no halo faces)");
     int rc = batch_pull_cur(b);
     if (rc != MPCF_OK) return rc;
     HIP_TRY(hipStreamSynchronize(b->stream));

@@ -45,6 +45,7 @@ namespace detail {
 struct Launchers
 {
     decltype(&mpcf::parity::launch_ghosts) ghosts;
+    decltype(&mpcf::parity::launch_halo) halo;
     decltype(&mpcf::parity::launch_sos) sos;
     decltype(&mpcf::parity::launch_sweep) sweep;
     decltype(&mpcf::parity::launch_update) update;
@@ -58,7 +59,7 @@ struct Launchers
 };
 
 #define MPCF_LAUNCHERS(NS)                                                                           \
-    {mpcf::NS::launch_ghosts, mpcf::NS::launch_sos, mpcf::NS::launch_sweep, mpcf::NS::launch_update, \
+    {mpcf::NS::launch_ghosts, mpcf::NS::launch_halo, mpcf::NS::launch_sos, mpcf::NS::launch_sweep, mpcf::NS::launch_update, \
      mpcf::NS::launch_pack, mpcf::NS::launch_stream, mpcf::NS::launch_test_weno,                      \
      mpcf::NS::launch_test_face,                                                                      \
      mpcf::NS::batch_sos, mpcf::NS::batch_xy, mpcf::NS::batch_z}
@@ -170,12 +171,20 @@ inline void case_desc(const mpcf_ctx* c, const int cur, const int len_xy, const 
 namespace mpcf {
 namespace detail {
 
-// bit f set: face f is reflecting
-inline int reflect_mask(const int bc[6])
+// the sweeps' boundary mask: bit f set -- face f reflecting; bit 8+f set --
+// face f a halo face (its ghost band is read as imported)
+inline int bc_mask(const int bc[6])
 {
     int m = 0;
-    for (int f = 0; f < 6; ++f) m |= (bc[f] == MPCF_BC_REFLECTING) << f;
+    for (int f = 0; f < 6; ++f) m |= (bc[f] == MPCF_BC_REFLECTING) << f | (bc[f] == MPCF_BC_HALO) << (8 + f);
     return m;
+}
+
+inline bool has_halo(const int bc[6])
+{
+    for (int f = 0; f < 6; ++f)
+        if (bc[f] == MPCF_BC_HALO) return true;
+    return false;
 }
 
 // march length of the batch kernels when neither forced nor tuned: split the

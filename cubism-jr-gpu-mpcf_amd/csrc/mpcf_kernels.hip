@@ -71,7 +71,8 @@ struct StageArgs
     int64_t fs;                    // field stride (elements)
     Real a, b, dtinvh;
     const StepState* st;           // if non-null, dtinvh is read from st->dtinvh
-    int reflect;                   // bit f set: face f is reflecting (else absorbing)
+    int bcmask;                    // bit f: face f reflecting; bit 8+f: face f a halo
+                                   // face (ghost band imported); else absorbing
 };
 
 // Boundary conditions at load time (BoundaryConditions.h:60-109).  A stencil
@@ -79,16 +80,20 @@ struct StageArgs
 // ghost would copy: absorbing -- the boundary cell; reflecting -- the mirror
 // cell (-1-i on the low face, 2n-1-i on the high face) with the axis' normal
 // momentum negated (sg = -1).  The values are the ghost values bit for bit,
-// so no kernel fills or reads a ghost band.
-__device__ __forceinline__ int bc_index(const int i, const int n, const int reflect, const int face_lo, Real& sg)
+// so no kernel fills or reads a ghost band -- except on a halo face, whose
+// ghost band holds a neighbouring block's cells (imported by k_halo) and is
+// read as it is.
+__device__ __forceinline__ int bc_index(const int i, const int n, const int bcmask, const int face_lo, Real& sg)
 {
     sg = Real(1.0f);
     if (i < 0) {
-        if ((reflect >> face_lo) & 1) { sg = Real(-1.0f); return -1 - i; }
+        if ((bcmask >> (8 + face_lo)) & 1) return i;
+        if ((bcmask >> face_lo) & 1) { sg = Real(-1.0f); return -1 - i; }
         return 0;
     }
     if (i >= n) {
-        if ((reflect >> (face_lo + 1)) & 1) { sg = Real(-1.0f); return 2 * n - 1 - i; }
+        if ((bcmask >> (9 + face_lo)) & 1) return i;
+        if ((bcmask >> (face_lo + 1)) & 1) { sg = Real(-1.0f); return 2 * n - 1 - i; }
         return n - 1;
     }
     return i;
@@ -102,7 +107,7 @@ __device__ __forceinline__ Real load_row(const StageArgs& A, const int e0, const
                                          const int face_lo, Real* out)
 {
     Real sg;
-    const int r = bc_index(row, n, A.reflect, face_lo, sg);
+    const int r = bc_index(row, n, A.bcmask, face_lo, sg);
 #pragma unroll
     for (int q = 0; q < NVAR; ++q) out[q] = A.u[q*A.fs + e0 + r * s];
     return sg;
@@ -333,7 +338,7 @@ void k_sweep_x(const Geom g, const StageArgs A, const int chunk)
     Real pre_sg;   // boundary sign of the prefetched run's x-momentum, applied when it is staged
     auto load_run = [&](const int xs) {
         const int xc = min(xs + xl, c1 + 2);   // reads past the run end are clamped to the stencil reach
-        const int xm = bc_index(xc, g.nx, A.reflect, 0, pre_sg);
+        const int xm = bc_index(xc, g.nx, A.bcmask, 0, pre_sg);
 #pragma unroll
         for (int i = 0; i < 8; ++i)
 #pragma unroll
@@ -452,8 +457,8 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
     z = min(z, g.nz - 1);                         // lanes past the last plane read its last row
     // lanes outside [0, nx) read the cell their x-ghost copies
     Real sgx;
-    const int xm = bc_index(x, g.nx, A.reflect, 0, sgx);
-    const bool xmirror = (A.reflect & 3) != 0;
+    const int xm = bc_index(x, g.nx, A.bcmask, 0, sgx);
+    const bool xmirror = (A.bcmask & 3) != 0;
     const int c0 = bz * chunk;
     const int c1 = min(c0 + chunk, g.ny);
     const int s = g.px;
@@ -519,7 +524,8 @@ void k_sweep_xy(const Geom g, const StageArgs A, const int chunk)
 
 // ---------------------------------------------------------------------------
 // ghost fill (BoundaryConditions.h:60-109 applied to the padded band).
-// blockIdx.y = face = 2*dir + side.  kind: 0 absorbing, 1 reflecting.
+// blockIdx.y = face = 2*dir + side.  kind: 0 absorbing, 1 reflecting, 2 halo
+// (left as imported).
 // ---------------------------------------------------------------------------
 struct BCSet { int kind[6]; };
 
@@ -534,6 +540,7 @@ __device__ __forceinline__ void ghosts_body(const Geom& g, Real* __restrict__ u,
     const int ta = t % na, tb = t / na;
     const int n = g.n(dir);
     const int kind = bc[face];
+    if (kind == 2) return;
 #pragma unroll
     for (int gl = 0; gl < 3; ++gl) {
         const int dst = side == 0 ? gl - 3 : n + gl;
@@ -554,6 +561,41 @@ __global__ __launch_bounds__(256)
 void k_ghosts(const Geom g, Real* __restrict__ u, const int64_t fs, const BCSet bc)
 {
     ghosts_body(g, u, fs, bc.kind, blockIdx.x, blockIdx.y);
+}
+
+// ---------------------------------------------------------------------------
+// face halos of a decomposed domain (GPUlab::_copysend_halos,
+// GPUlab.cpp:76-112, and the receive side that becomes the ghost band,
+// GPUlab.cpp:636-641).  The buffer holds the 7 fields one after the other,
+// each in the reference's ghost map order (Types.h:31-69): x faces
+// [NZ][NY][3], y faces [NZ][3][NX], z faces [3][NY][NX], layer l = the l-th
+// of the three cells in increasing index.  EXPORT reads the three boundary
+// cells (0..2 on the low face, n-3..n-1 on the high face), IMPORT writes the
+// ghost band (-3..-1 low, n..n+2 high): a high face's export is the high
+// neighbour's low-face import and vice versa.
+// ---------------------------------------------------------------------------
+template <int IMPORT>
+__global__ __launch_bounds__(256)
+void k_halo(const Geom g, Real* __restrict__ u, const int64_t fs, const int face, Real* __restrict__ buf)
+{
+    const int dir = face >> 1, side = face & 1;
+    const int area = dir == 0 ? g.ny * g.nz : dir == 1 ? g.nx * g.nz : g.nx * g.ny;
+    const int count = 3 * area;
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= count) return;
+    int x, y, z, layer;
+    if (dir == 0)      { layer = t % 3;                 y = (t / 3) % g.ny;    z = t / (3 * g.ny); x = 0; }
+    else if (dir == 1) { x = t % g.nx;                  layer = (t / g.nx) % 3; z = t / (3 * g.nx); y = 0; }
+    else               { x = t % g.nx;                  y = (t / g.nx) % g.ny;  layer = t / area;   z = 0; }
+    const int n = g.n(dir);
+    const int cell = IMPORT ? (side == 0 ? layer - 3 : n + layer) : (side == 0 ? layer : n - 3 + layer);
+    if (dir == 0) x = cell; else if (dir == 1) y = cell; else z = cell;
+    const int e = g.idx(x, y, z);
+#pragma unroll
+    for (int q = 0; q < NVAR; ++q) {
+        if (IMPORT) u[q * fs + e] = buf[(int64_t)q * count + t];
+        else        buf[(int64_t)q * count + t] = u[q * fs + e];
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -781,8 +823,8 @@ __device__ __forceinline__ int find_case(const int* __restrict__ off, const int 
 __device__ __forceinline__ StageArgs case_args(const CaseDesc& C, const int cur, const Real a, const Real b)
 {
     StageArgs A;
-    A.reflect = 0;
-    for (int f = 0; f < 6; ++f) A.reflect |= (C.bc[f] == 1) << f;
+    A.bcmask = 0;
+    for (int f = 0; f < 6; ++f) A.bcmask |= (C.bc[f] == 1) << f;
     A.u = (const Real*)C.u[cur]; A.R = (Real*)C.R; A.acc = (Real*)C.acc; A.uo = (Real*)C.u[1 - cur];
     A.fs = C.g.fstride;
     A.a = a; A.b = b; A.dtinvh = Real(0.0f); A.st = C.st;
@@ -863,6 +905,17 @@ void launch_ghosts(const Geom& g, void* uv, const int bc[6], hipStream_t s)
     hipLaunchKernelGGL(k_ghosts, dim3((tmax + 255) / 256, 6), dim3(256), 0, s, g, (Real*)uv, g.fstride, b);
 }
 
+void launch_halo(const Geom& g, void* uv, void* buf, const int face, const int import, hipStream_t s)
+{
+    const int dir = face >> 1;
+    const int count = 3 * (dir == 0 ? g.ny * g.nz : dir == 1 ? g.nx * g.nz : g.nx * g.ny);
+    const dim3 grid((count + 255) / 256);
+    if (import)
+        hipLaunchKernelGGL(k_halo<1>, grid, dim3(256), 0, s, g, (Real*)uv, g.fstride, face, (Real*)buf);
+    else
+        hipLaunchKernelGGL(k_halo<0>, grid, dim3(256), 0, s, g, (Real*)uv, g.fstride, face, (Real*)buf);
+}
+
 void launch_sos(const Geom& g, const void* uv, StepState* st, const int cpu_form, const int with_dt, const double cfl,
                 const double h, const double dt_max, hipStream_t s)
 {
@@ -882,7 +935,7 @@ void launch_sweep(const Geom& g, const int dir, const SweepParams& p, const int 
     StageArgs A;
     A.u = (const Real*)p.u; A.R = (Real*)p.R; A.acc = (Real*)p.acc; A.uo = (Real*)p.uo; A.fs = g.fstride;
     A.a = (Real)p.a; A.b = (Real)p.b; A.dtinvh = (Real)p.dtinvh; A.st = p.st;
-    A.reflect = p.reflect;
+    A.bcmask = p.bcmask;
     if (dir == 3) {
         const int chunk = pick_chunk(g.ny, (int64_t)g.nx * g.nz, 1);
         int dims[3];

@@ -17,7 +17,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmpcf_hip.so")
 
 OK, EINVAL, EHIP, ENOMEM, ENUMERIC, EBUSY = 0, 1, 2, 3, 4, 5
-BC_ABSORBING, BC_REFLECTING = 0, 1
+BC_ABSORBING, BC_REFLECTING, BC_HALO = 0, 1, 2
 FLAG_FAST, FLAG_SOS_CPU, FLAG_SPLIT_XY, FLAG_FUSED_XY, FLAG_P64, FLAG_FP64 = 1, 2, 4, 8, 16, 32
 DATA, TMP = 0, 1
 NVAR = 7
@@ -43,7 +43,8 @@ EXPORTS = [
     "mpcf_stream_channels", "mpcf_stream_out", "mpcf_stream_in", "mpcf_write_xdmf", "mpcf_write_save_info",
     "mpcf_read_save_info", "mpcf_dump", "mpcf_save", "mpcf_restart", "mpcf_post_shock_ratio",
     "mpcf_sod_defaults", "mpcf_ic_sod", "mpcf_sbi_defaults", "mpcf_sbi_config", "mpcf_read_bubbles",
-    "mpcf_ic_sbi", "mpcf_ic_static", "mpcf_ic_steady",
+    "mpcf_ic_sbi", "mpcf_ic_static", "mpcf_ic_steady", "mpcf_halo_count", "mpcf_halo_export",
+    "mpcf_halo_import",
 ]
 
 # streamer kinds of the dumps (mpcf_stream_out)
@@ -58,7 +59,6 @@ class SicParams(C.Structure):
                 ("pressure_ratio", C.c_double), ("shock_n", C.c_double * 3), ("shock_s", C.c_double * 3),
                 ("g1", C.c_double), ("g2", C.c_double), ("pc1", C.c_double), ("pc2", C.c_double),
                 ("eps", C.c_double)]
-
 
 
 class SodParams(C.Structure):
@@ -76,6 +76,7 @@ class SbiParams(C.Structure):
                 ("p", C.c_double), ("rhobubble", C.c_double), ("pbubble", C.c_double),
                 ("g1", C.c_double), ("g2", C.c_double), ("pc1", C.c_double), ("pc2", C.c_double),
                 ("mollfactor", C.c_int), ("reverse", C.c_int), ("dims", C.c_int * 3)]
+
 
 # serving policies and the eager-launch flag (mpcf_server_create)
 SERVE_ALL, SERVE_FIFO_FILL, SERVE_SRPT_FILL, SERVE_HYBRID, SERVE_FIFO_SLACK = 0, 1, 2, 3, 4
@@ -174,6 +175,9 @@ def lib():
         L.mpcf_ic_sbi.argtypes = [I, I, I, D, P, P, I, P]
         L.mpcf_ic_static.argtypes = [I, I, I, P, P]
         L.mpcf_ic_steady.argtypes = [I, I, I, P, P]
+        L.mpcf_halo_count.argtypes = [P, I, P]
+        L.mpcf_halo_export.argtypes = [P, I, P]
+        L.mpcf_halo_import.argtypes = [P, I, P]
         S_ = C.c_char_p
         L.mpcf_stream_channels.argtypes = [I]
         L.mpcf_stream_out.argtypes = [P, I, P]
@@ -369,6 +373,22 @@ class Block:
 
     def synchronize(self):
         _check(lib().mpcf_synchronize(self._h))
+
+    def halo_count(self, face):
+        """Elements of one face's halo buffer (7 fields x 3 layers x the face)."""
+        n = C.c_longlong()
+        _check(lib().mpcf_halo_count(self._h, int(face), C.cast(C.byref(n), C.c_void_p)))
+        return n.value
+
+    def halo_export(self, face, dev_ptr):
+        """The three boundary cells of `face` into the device buffer at dev_ptr
+        (asynchronous on this block's stream)."""
+        _check(lib().mpcf_halo_export(self._h, int(face), C.c_void_p(int(dev_ptr))))
+
+    def halo_import(self, face, dev_ptr):
+        """The device buffer at dev_ptr into the ghost band of `face`
+        (asynchronous on this block's stream)."""
+        _check(lib().mpcf_halo_import(self._h, int(face), C.c_void_p(int(dev_ptr))))
 
     @property
     def stream(self):

@@ -38,7 +38,10 @@ enum {
 /* boundary kinds per face (face = 2*dir + side; dir 0,1,2 = x,y,z; side 0 = low) */
 enum {
     MPCF_BC_ABSORBING = 0,   /* BoundaryConditions::applyBC_absorbing  (BoundaryConditions.h:60-82)  */
-    MPCF_BC_REFLECTING = 1   /* BoundaryConditions::applyBC_reflecting (BoundaryConditions.h:85-109) */
+    MPCF_BC_REFLECTING = 1,  /* BoundaryConditions::applyBC_reflecting (BoundaryConditions.h:85-109) */
+    MPCF_BC_HALO = 2         /* a face shared with a neighbouring block: its three ghost layers hold
+                                the neighbour's cells, written by mpcf_halo_import before each
+                                stage (the MPI halo faces of GPUlab::load_ghosts, GPUlab.cpp:622-644) */
 };
 
 /* context flags */
@@ -99,8 +102,32 @@ int mpcf_upload_f64(mpcf_ctx* ctx, int which, const double* const fields[7]);
 int mpcf_download_f64(mpcf_ctx* ctx, int which, double* const fields[7]);
 
 /* Fill the three-cell ghost band of u from the boundary conditions
- * (GPUlab::load_ghosts -> _apply_bc, source/GPUlab.cpp:622-644). */
+ * (GPUlab::load_ghosts -> _apply_bc, source/GPUlab.cpp:622-644); halo faces
+ * keep what mpcf_halo_import wrote. */
 int mpcf_load_ghosts(mpcf_ctx* ctx);
+
+/* Face halos of a block that is one part of a decomposed domain: the device
+ * side of GPUlab's MPI halo exchange (GPUlab::_copysend_halos,
+ * GPUlab.cpp:76-112, and the receives of GPUlab::load_ghosts, :636-641).
+ * face = 2*dir + side.  A halo buffer holds mpcf_halo_count() elements of the
+ * context's storage type (binary32; binary64 for MPCF_FLAG_FP64): the seven
+ * fields one after the other, each in the reference's ghost map order
+ * (Types.h:31-69) -- x faces [NZ][NY][3], y faces [NZ][3][NX], z faces
+ * [3][NY][NX], layer l = the l-th cell in increasing index.
+ *   mpcf_halo_export: the three boundary cells of the current u (0..2 on the
+ *     low face, n-3..n-1 on the high face) into dst, the neighbour's import;
+ *   mpcf_halo_import: src into the ghost band of the current u (-3..-1 low,
+ *     n..n+2 high), read by the next stage on faces set to MPCF_BC_HALO.
+ * dst/src are device pointers on the context's device; both calls are
+ * asynchronous on the context's stream (order them against another
+ * context's stream before that context reads the buffer).  A stage needs
+ * fresh halos, so a block with halo faces steps with mpcf_max_sos (the
+ * maximum over all blocks), dt = min(cfl h / sos, dt_max) on the host, and per
+ * stage: export, exchange, import, mpcf_process_all(a, b, (float)(dt/h));
+ * mpcf_step refuses halo faces. */
+int mpcf_halo_count(mpcf_ctx* ctx, int face, long long* count);
+int mpcf_halo_export(mpcf_ctx* ctx, int face, void* dst);
+int mpcf_halo_import(mpcf_ctx* ctx, int face, const void* src);
 
 /* Max over the block of c + max(|rho u|,|rho v|,|rho w|)/rho
  * (GPUlab::max_sos / MaxSpeedOfSound_CUDA::compute, GPUlab.cpp:647-700,
