@@ -131,38 +131,43 @@ MPCF_HD RecT<T> weno_cell(const T a, const T b, const T c, const T d, const T e)
     const T eps = T(1e-6);
     const T iA = isA + eps, iB = isB + eps, iC = isC + eps;
 
-    T mA0, mA1, mA2, pA0, pA1, pA2;
+    const T k1_3 = T(1./3.), k7_6 = T(7./6.), k11_6 = T(11./6.);
+    const T k1_6 = T(1./6.), k5_6 = T(5./6.), mk1_6 = T(-(1./6.));
+    // the three candidate polynomials of each state
+    const T m0 = k1_3*a - k7_6*b + k11_6*c, m1 = mk1_6*b + k5_6*c + k1_3*d, m2 = k1_3*c + k5_6*d - k1_6*e;
+    const T p0 = k1_3*e - k7_6*d + k11_6*c, p1 = mk1_6*d + k5_6*c + k1_3*b, p2 = k1_3*c + k5_6*b - k1_6*a;
+
+    RecT<T> out;
     if (FAST) {
-        const T rA = mrcp(10.0f*iA*iA);
-        const T rB = mrcp(10.0f*iB*iB);
-        const T rC = mrcp(10.0f*iC*iC);
-        mA0 = rA;        mA1 = 6.0f*rB; mA2 = 3.0f*rC;
-        pA0 = rC;        pA1 = mA1;     pA2 = 3.0f*rA;
+        // The weights enter only normalised, omega_k = alpha_k / sum(alpha), so
+        // the common factor 1/10 of alpha_k = d_k / (10 is_k^2) is dropped and
+        // the state is (sum alpha_k P_k) / sum(alpha): the same value with a
+        // different rounding, and 9 fewer operations per variable.
+        const T rA = mrcp(iA*iA), rB = mrcp(iB*iB), rC = mrcp(iC*iC);
+        const T aB = 6.0f*rB, mC = 3.0f*rC, pA = 3.0f*rA;
+        out.m = (rA*m0 + aB*m1 + mC*m2) * mrcp(rA + aB + mC);
+        out.p = (rC*p0 + aB*p1 + pA*p2) * mrcp(rC + aB + pA);
     } else {
         // binary64: the reference writes the literal 10.0 (GPUonly.cuh:111-113,150-152)
         const W rA = W(1.0) / (10.0*W(iA)*W(iA));
         const W rB = W(1.0) / (10.0*W(iB)*W(iB));
         const W rC = W(1.0) / (10.0*W(iC)*W(iC));
-        mA0 = T(rA);  mA1 = T(6.0*rB); mA2 = T(3.0*rC);
-        pA0 = T(rC);  pA1 = mA1;       pA2 = T(3.0*rA);
-    }
-    const T k1_3 = T(1./3.), k7_6 = T(7./6.), k11_6 = T(11./6.);
-    const T k1_6 = T(1./6.), k5_6 = T(5./6.), mk1_6 = T(-(1./6.));
-
-    RecT<T> out;
-    {   // _weno_minus(a,b,c,d,e)
-        const T sum = mA0 + mA1 + mA2;
-        const T inv = qdiv<FAST>(T(1.0f), sum);
-        const T w0 = mA0 * inv, w1 = mA1 * inv;
-        const T w2 = T(1.0f) - w0 - w1;
-        out.m = w0*(k1_3*a - k7_6*b + k11_6*c) + w1*(mk1_6*b + k5_6*c + k1_3*d) + w2*(k1_3*c + k5_6*d - k1_6*e);
-    }
-    {   // _weno_pluss(a,b,c,d,e)  (its b..f are this cell's a..e)
-        const T sum = pA0 + pA1 + pA2;
-        const T inv = qdiv<FAST>(T(1.0f), sum);
-        const T w0 = pA0 * inv, w1 = pA1 * inv;
-        const T w2 = T(1.0f) - w0 - w1;
-        out.p = w0*(k1_3*e - k7_6*d + k11_6*c) + w1*(mk1_6*d + k5_6*c + k1_3*b) + w2*(k1_3*c + k5_6*b - k1_6*a);
+        const T mA0 = T(rA), mA1 = T(6.0*rB), mA2 = T(3.0*rC);
+        const T pA0 = T(rC), pA1 = mA1,       pA2 = T(3.0*rA);
+        {   // _weno_minus(a,b,c,d,e)
+            const T sum = mA0 + mA1 + mA2;
+            const T inv = qdiv<FAST>(T(1.0f), sum);
+            const T w0 = mA0 * inv, w1 = mA1 * inv;
+            const T w2 = T(1.0f) - w0 - w1;
+            out.m = w0*m0 + w1*m1 + w2*m2;
+        }
+        {   // _weno_pluss(a,b,c,d,e)  (its b..f are this cell's a..e)
+            const T sum = pA0 + pA1 + pA2;
+            const T inv = qdiv<FAST>(T(1.0f), sum);
+            const T w0 = pA0 * inv, w1 = pA1 * inv;
+            const T w2 = T(1.0f) - w0 - w1;
+            out.p = w0*p0 + w1*p1 + w2*p2;
+        }
     }
     // clipping to the three central cells (identical bounds for both states)
     const T mn = mmin(mmin(b, c), d);
