@@ -124,12 +124,29 @@ MPCF_HD RecT<T> weno_cell(const T a, const T b, const T c, const T d, const T e)
     const T k4_3 = T(4./3.), k19_3 = T(19./3.), k11_3 = T(11./3.);
     const T k25_3 = T(25./3.), k31_3 = T(31./3.), k10_3 = T(10./3.);
     const T k13_3 = T(13./3.), k5_3 = T(5./3.);
-    // the three smoothness indicators (minus is0/is1/is2 == plus is2/is1/is0)
-    const T isA = a*(a*k4_3 - b*k19_3 + c*k11_3) + b*(b*k25_3 - c*k31_3) + c*c*k10_3;
-    const T isB = b*(b*k4_3 - c*k13_3 + d*k5_3) + c*(c*k13_3 - d*k13_3) + d*d*k4_3;
-    const T isC = c*(c*k10_3 - d*k31_3 + e*k11_3) + d*(d*k25_3 - e*k19_3) + e*e*k4_3;
+    // the three smoothness indicators (minus is0/is1/is2 == plus is2/is1/is0),
+    // plus WENOEPS
     const T eps = T(1e-6);
-    const T iA = isA + eps, iB = isB + eps, iC = isC + eps;
+    T iA, iB, iC;
+    if (FAST) {
+        // The same quadratics in Jiang-Shu form, 13/12 D^2 + 1/4 t^2 with D the
+        // second difference of the window (expanding gives the reference's
+        // coefficients 4/3, -19/3, 11/3, 25/3, -31/3, 10/3 and 4/3, -13/3,
+        // 5/3, 13/3, -13/3, 4/3 term by term), eps folded into the sums.
+        const T k13_12 = T(13./12.);
+        const T Db = a - 2.0f*b + c, Dc = b - 2.0f*c + d, Dd = c - 2.0f*d + e;
+        const T tA = Db + 2.0f*(c - b);   // a - 4b + 3c
+        const T tB = b - d;
+        const T tC = Dd - 2.0f*(d - c);   // 3c - 4d + e
+        iA = Db*(k13_12*Db) + ((0.25f*tA)*tA + eps);
+        iB = Dc*(k13_12*Dc) + ((0.25f*tB)*tB + eps);
+        iC = Dd*(k13_12*Dd) + ((0.25f*tC)*tC + eps);
+    } else {
+        const T isA = a*(a*k4_3 - b*k19_3 + c*k11_3) + b*(b*k25_3 - c*k31_3) + c*c*k10_3;
+        const T isB = b*(b*k4_3 - c*k13_3 + d*k5_3) + c*(c*k13_3 - d*k13_3) + d*d*k4_3;
+        const T isC = c*(c*k10_3 - d*k31_3 + e*k11_3) + d*(d*k25_3 - e*k19_3) + e*e*k4_3;
+        iA = isA + eps; iB = isB + eps; iC = isC + eps;
+    }
 
     const T k1_3 = T(1./3.), k7_6 = T(7./6.), k11_6 = T(11./6.);
     const T k1_6 = T(1./6.), k5_6 = T(5./6.), mk1_6 = T(-(1./6.));
@@ -202,9 +219,11 @@ template <typename T> struct FaceFluxT { T f[NVAR]; T vel; };
 
 // D = 0,1,2: normal direction.  L = minus-state (left cell), Rt = plus-state.
 // GPUonly.cuh:802-1042 with argument orders of GPUkernels.cu:259-283 (x),
-// 638-662 (y), 934-957 (z).
+// 638-662 (y), 934-957 (z).  The states are taken by value: the FAST path
+// selects fields of one side, and selects between loads through references
+// would keep the callers' register-resident march state in memory.
 template <int D, typename T, bool FAST>
-MPCF_HD FaceFluxT<T> riemann(const PrimT<T>& L, const PrimT<T>& Rt)
+MPCF_HD FaceFluxT<T> riemann(const PrimT<T> L, const PrimT<T> Rt)
 {
     typedef typename Wide<T>::type W;
     // normal and the two transverse velocities in the order _hllc_e receives them
@@ -236,6 +255,37 @@ MPCF_HD FaceFluxT<T> riemann(const PrimT<T>& L, const PrimT<T>& Rt)
     const T facp = rp * (sp - vnp);
     const T ss = qdiv<FAST>(pp - pm + vnm*facm - vnp*facp, facm - facp);
 
+    FaceFluxT<T> o;
+    if constexpr (FAST) {
+        // One side's star state gives every flux: the left for ss > 0, the
+        // right for ss < 0.  At ss == 0 the reference averages the two, which
+        // are equal there (F*_R - F*_L = ss (U*_R - U*_L)), so the left is
+        // taken.  For rho, G, P and the transverse momenta the HLLC flux
+        // q vn + s (q chi - q) is q times the face velocity vel.
+        const bool lt = !(ss < 0.0f);
+        const T vn = lt ? vnm : vnp, r = lt ? rm : rp, p = lt ? pm : pp;
+        const T G = lt ? Gm : Gp, P = lt ? Pm : Pp;
+        const T v1 = lt ? v1m : v1p, v2 = lt ? v2m : v2p;
+        const T sk = lt ? sm : sp;
+        const T s = lt ? mmin(T(0.0f), sm) : mmax(T(0.0f), sp);
+        const T chi = qdiv<FAST>(sk - vn, sk - ss);
+        const T vel = vn + s*(chi - 1.0f);                      // _extraterm_hllc_vel
+        const T rv = r*vel;
+        const T rvn = r*vn;
+        const T fpv = rvn*vn + p + s*(r*ss*chi - rvn);          // _hllc_pvel
+        const T E = G*p + P + 0.5f*r*(vn*vn + v1*v1 + v2*v2);   // _hllc_e
+        const T fe = vn*(E + p) + s*(chi*(E + (ss - vn)*(r*ss + qdiv<FAST>(p, sk - vn))) - E);
+        o.f[FR] = rv;
+        if (D == 0) { o.f[FU] = fpv;    o.f[FV] = rv*v1; o.f[FW] = rv*v2; }
+        if (D == 1) { o.f[FU] = rv*v1;  o.f[FV] = fpv;   o.f[FW] = rv*v2; }
+        if (D == 2) { o.f[FU] = rv*v1;  o.f[FV] = rv*v2; o.f[FW] = fpv; }
+        o.f[FE] = fe;
+        o.f[FG] = G*vel;
+        o.f[FP] = P*vel;
+        o.vel = vel;
+        return o;
+    }
+
     // shared HLLC terms (GPUonly.cuh:883-889 and the identical lines in the
     // other helpers)
     const int sign_star = ss == 0.0f ? 0 : (ss < 0.0f ? -1 : 1);
@@ -247,7 +297,6 @@ MPCF_HD FaceFluxT<T> riemann(const PrimT<T>& L, const PrimT<T>& Rt)
     const float wr = 0.5f*(1 - sign_star);
     const double wr_d = 0.5*(1 - sign_star);
 
-    FaceFluxT<T> o;
     // _hllc_rho for rho, G, P: right weight is binary64 (GPUonly.cuh:900)
     auto hllc_rho = [&](const T qm, const T qp) -> T {
         const T q_deltam = qm*chi_m - qm;
