@@ -59,6 +59,7 @@ inline CF mmax(CF a, CF b) { if (g_t) g_t->minmax += 1; return CF(fmaxf(a.v, b.v
 inline CF msqrt(CF a) { if (g_t) g_t->sqrt += 1; return CF(sqrtf(a.v)); }
 inline CF msqrt_fast(CF a) { if (g_t) g_t->sqrt += 1; return CF(sqrtf(a.v)); }
 inline CF mrcp(CF a) { if (g_t) g_t->div += 1; return CF(1.0f / a.v); }
+inline CF mclamp(CF x, CF lo, CF hi) { if (g_t) g_t->minmax += 1; return CF(fminf(fmaxf(x.v, lo.v), hi.v)); }
 
 } // namespace
 

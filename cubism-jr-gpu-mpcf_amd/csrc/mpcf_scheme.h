@@ -64,6 +64,17 @@ MPCF_HD float msqrt_fast(float a)
 #endif
 }
 
+// x clamped to [lo, hi] for lo <= hi: one v_med3_f32 on the device (FAST)
+MPCF_HD float mclamp(float x, float lo, float hi)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_fmed3f(x, lo, hi);
+#else
+    return fminf(fmaxf(x, lo), hi);
+#endif
+}
+MPCF_HD double mclamp(double x, double lo, double hi) { return fmin(fmax(x, lo), hi); }
+
 // a / b: IEEE division (bit-exact build) or a * rcp(b) (FAST)
 template <bool FAST, typename T> MPCF_HD T qdiv(const T a, const T b) { return FAST ? a * mrcp(b) : a / b; }
 template <bool FAST, typename T> MPCF_HD T qsqrt(const T a) { return FAST ? msqrt_fast(a) : msqrt(a); }
@@ -124,47 +135,54 @@ MPCF_HD RecT<T> weno_cell(const T a, const T b, const T c, const T d, const T e)
     const T k4_3 = T(4./3.), k19_3 = T(19./3.), k11_3 = T(11./3.);
     const T k25_3 = T(25./3.), k31_3 = T(31./3.), k10_3 = T(10./3.);
     const T k13_3 = T(13./3.), k5_3 = T(5./3.);
-    // the three smoothness indicators (minus is0/is1/is2 == plus is2/is1/is0),
-    // plus WENOEPS
+    RecT<T> out;
     const T eps = T(1e-6);
-    T iA, iB, iC;
-    if (FAST) {
-        // The same quadratics in Jiang-Shu form, 13/12 D^2 + 1/4 t^2 with D the
-        // second difference of the window (expanding gives the reference's
-        // coefficients 4/3, -19/3, 11/3, 25/3, -31/3, 10/3 and 4/3, -13/3,
-        // 5/3, 13/3, -13/3, 4/3 term by term), eps folded into the sums.
+    if constexpr (FAST) {
+        // The same arithmetic rearranged (equal in exact arithmetic, rounded
+        // differently), on the first differences of the window:
+        //  * the indicators in Jiang-Shu form 13/12 D^2 + 1/4 t^2 (expanding
+        //    gives the reference's coefficients 4/3, -19/3, 11/3, 25/3, -31/3,
+        //    10/3 and 4/3, -13/3, 5/3, 13/3, -13/3, 4/3 term by term), eps
+        //    folded in;
+        //  * the weights only normalised, omega_k = alpha_k / sum(alpha): the
+        //    common 1/10 of alpha_k = d_k / (10 is_k^2) dropped, one reciprocal
+        //    of the sum per state;
+        //  * the candidates as c + (differences)/6, e.g. (2a - 7b + 11c)/6 =
+        //    c + (-2 (b-a) + 5 (c-b))/6, so a state is
+        //    c + sum(alpha_k n_k) / (6 sum(alpha));
+        //  * the clip to the three central cells as one median-of-three.
+        const T dab = b - a, dbc = c - b, dcd = d - c, dde = e - d;
+        const T Db = dbc - dab, Dc = dcd - dbc, Dd = dde - dcd;   // second differences
+        const T tA = Db + 2.0f*dbc;                               // a - 4b + 3c
+        const T tB = dbc + dcd;                                   // d - b
+        const T tC = Dd - 2.0f*dcd;                               // 3c - 4d + e
         const T k13_12 = T(13./12.);
-        const T Db = a - 2.0f*b + c, Dc = b - 2.0f*c + d, Dd = c - 2.0f*d + e;
-        const T tA = Db + 2.0f*(c - b);   // a - 4b + 3c
-        const T tB = b - d;
-        const T tC = Dd - 2.0f*(d - c);   // 3c - 4d + e
-        iA = Db*(k13_12*Db) + ((0.25f*tA)*tA + eps);
-        iB = Dc*(k13_12*Dc) + ((0.25f*tB)*tB + eps);
-        iC = Dd*(k13_12*Dd) + ((0.25f*tC)*tC + eps);
+        const T iA = Db*(k13_12*Db) + ((0.25f*tA)*tA + eps);
+        const T iB = Dc*(k13_12*Dc) + ((0.25f*tB)*tB + eps);
+        const T iC = Dd*(k13_12*Dd) + ((0.25f*tC)*tC + eps);
+        const T rA = mrcp(iA*iA), rB = mrcp(iB*iB), rC = mrcp(iC*iC);
+        const T aB = 6.0f*rB, mC = 3.0f*rC, pA = 3.0f*rA;
+        // minus-state candidates (window a..e), 6 (q_k - c)
+        const T nm0 = 5.0f*dbc - 2.0f*dab, nm1 = dbc + 2.0f*dcd, nm2 = 4.0f*dcd - dde;
+        // plus-state candidates (the mirror), 6 (q_k - c); the middle one as
+        // -(its value)
+        const T np0 = 2.0f*dde - 5.0f*dcd, np1n = dcd + 2.0f*dbc, np2 = dab - 4.0f*dbc;
+        const T k1_6 = T(1./6.);
+        const T om = (rA*nm0 + aB*nm1 + mC*nm2) * mrcp(rA + aB + mC);
+        const T op = (rC*np0 - aB*np1n + pA*np2) * mrcp(rC + aB + pA);
+        const T mn = mmin(mmin(b, c), d);
+        const T mx = mmax(mmax(b, c), d);
+        out.m = mclamp(om*k1_6 + c, mn, mx);
+        out.p = mclamp(op*k1_6 + c, mn, mx);
+        return out;
     } else {
+        // the three smoothness indicators (minus is0/is1/is2 == plus is2/is1/is0)
         const T isA = a*(a*k4_3 - b*k19_3 + c*k11_3) + b*(b*k25_3 - c*k31_3) + c*c*k10_3;
         const T isB = b*(b*k4_3 - c*k13_3 + d*k5_3) + c*(c*k13_3 - d*k13_3) + d*d*k4_3;
         const T isC = c*(c*k10_3 - d*k31_3 + e*k11_3) + d*(d*k25_3 - e*k19_3) + e*e*k4_3;
-        iA = isA + eps; iB = isB + eps; iC = isC + eps;
-    }
-
-    const T k1_3 = T(1./3.), k7_6 = T(7./6.), k11_6 = T(11./6.);
-    const T k1_6 = T(1./6.), k5_6 = T(5./6.), mk1_6 = T(-(1./6.));
-    // the three candidate polynomials of each state
-    const T m0 = k1_3*a - k7_6*b + k11_6*c, m1 = mk1_6*b + k5_6*c + k1_3*d, m2 = k1_3*c + k5_6*d - k1_6*e;
-    const T p0 = k1_3*e - k7_6*d + k11_6*c, p1 = mk1_6*d + k5_6*c + k1_3*b, p2 = k1_3*c + k5_6*b - k1_6*a;
-
-    RecT<T> out;
-    if (FAST) {
-        // The weights enter only normalised, omega_k = alpha_k / sum(alpha), so
-        // the common factor 1/10 of alpha_k = d_k / (10 is_k^2) is dropped and
-        // the state is (sum alpha_k P_k) / sum(alpha): the same value with a
-        // different rounding, and 9 fewer operations per variable.
-        const T rA = mrcp(iA*iA), rB = mrcp(iB*iB), rC = mrcp(iC*iC);
-        const T aB = 6.0f*rB, mC = 3.0f*rC, pA = 3.0f*rA;
-        out.m = (rA*m0 + aB*m1 + mC*m2) * mrcp(rA + aB + mC);
-        out.p = (rC*p0 + aB*p1 + pA*p2) * mrcp(rC + aB + pA);
-    } else {
+        const T iA = isA + eps, iB = isB + eps, iC = isC + eps;
+        const T k1_3 = T(1./3.), k7_6 = T(7./6.), k11_6 = T(11./6.);
+        const T k1_6 = T(1./6.), k5_6 = T(5./6.), mk1_6 = T(-(1./6.));
         // binary64: the reference writes the literal 10.0 (GPUonly.cuh:111-113,150-152)
         const W rA = W(1.0) / (10.0*W(iA)*W(iA));
         const W rB = W(1.0) / (10.0*W(iB)*W(iB));
@@ -176,22 +194,22 @@ MPCF_HD RecT<T> weno_cell(const T a, const T b, const T c, const T d, const T e)
             const T inv = qdiv<FAST>(T(1.0f), sum);
             const T w0 = mA0 * inv, w1 = mA1 * inv;
             const T w2 = T(1.0f) - w0 - w1;
-            out.m = w0*m0 + w1*m1 + w2*m2;
+            out.m = w0*(k1_3*a - k7_6*b + k11_6*c) + w1*(mk1_6*b + k5_6*c + k1_3*d) + w2*(k1_3*c + k5_6*d - k1_6*e);
         }
         {   // _weno_pluss(a,b,c,d,e)  (its b..f are this cell's a..e)
             const T sum = pA0 + pA1 + pA2;
             const T inv = qdiv<FAST>(T(1.0f), sum);
             const T w0 = pA0 * inv, w1 = pA1 * inv;
             const T w2 = T(1.0f) - w0 - w1;
-            out.p = w0*p0 + w1*p1 + w2*p2;
+            out.p = w0*(k1_3*e - k7_6*d + k11_6*c) + w1*(mk1_6*d + k5_6*c + k1_3*b) + w2*(k1_3*c + k5_6*b - k1_6*a);
         }
+        // clipping to the three central cells (identical bounds for both states)
+        const T mn = mmin(mmin(b, c), d);
+        const T mx = mmax(mmax(b, c), d);
+        out.m = mmin(mmax(out.m, mn), mx);
+        out.p = mmin(mmax(out.p, mn), mx);
+        return out;
     }
-    // clipping to the three central cells (identical bounds for both states)
-    const T mn = mmin(mmin(b, c), d);
-    const T mx = mmax(mmax(b, c), d);
-    out.m = mmin(mmax(out.m, mn), mx);
-    out.p = mmin(mmax(out.p, mn), mx);
-    return out;
 }
 
 // one cell's reconstructed face states in one direction
