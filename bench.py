@@ -44,10 +44,22 @@ PEAK_HBM_GBS = 8000.0         # MI355X HBM3E, published
 BYTES_PER_CELL_STAGE = 112.0  # 7 fields x 4 B x (read u, read R, write R, write u)
 # algorithmic HBM bytes per cell of each sweep kernel as built here
 KERNEL_BYTES_PER_CELL = {
-    "k_sweep_xy": 28 + 40,                 # read u, write 10 accumulators
-    "k_sweep_x": 28 + 40,                  # read u, write 10 accumulators
-    "k_sweep_yz<1>": 28 + 40 + 40,         # read u, read + write accumulators
-    "k_sweep_yz<2>": 28 + 40 + 28 + 28 + 28,  # read u, accumulators, R; write R, u'
+    # bit-exact build: the x+y sweep(s) leave 10 accumulators, the z sweep does
+    # the reference's single update of R and u
+    "parity": {
+        "k_sweep_xy": 28 + 40,                    # read u, write 10 accumulators
+        "k_sweep_x": 28 + 40,                     # read u, write 10 accumulators
+        "k_sweep_yz<1>": 28 + 40 + 40,            # read u, read + write accumulators
+        "k_sweep_yz<2>": 28 + 40 + 28 + 28 + 28,  # read u, accumulators, R; write R, u'
+    },
+    # FAST build: the x+y part is folded into R where it is computed; 3 G/P
+    # accumulators remain for the z sweep
+    "fast": {
+        "k_sweep_xy": 28 + 28 + 28 + 12,          # read u, R; write R, 3 accumulators
+        "k_sweep_x": 28 + 40,                     # read u, write 10 accumulators
+        "k_sweep_yz<1>": 28 + 40 + 28 + 28 + 12,  # read u, accumulators, R; write R, 3 accumulators
+        "k_sweep_yz<2>": 28 + 12 + 28 + 28 + 28,  # read u, 3 accumulators, R; write R, u'
+    },
 }
 # measured HBM bytes per cell of each kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE
 # at 512^3, calibrated on the pack copy kernel), written by the profiling run
@@ -258,7 +270,7 @@ def main():
         tr = json.load(open(TRAFFIC_FILE)).get(a.mode, {}).get(kname[dom])
         if tr is not None:
             traffic = tr * cells
-    alg_bytes = KERNEL_BYTES_PER_CELL[kname[dom]] * cells
+    alg_bytes = KERNEL_BYTES_PER_CELL[a.mode][kname[dom]] * cells
     out = {
         "metric": "RHS cell-updates/s at 512^3 FP32",
         "value": value,

@@ -59,6 +59,15 @@ typedef FaceFluxT<Real> FaceFlux;
 typedef CellRecT<Real> CellRec;
 typedef RecT<Real> Rec;
 constexpr bool kFast = MPCF_FAST != 0;
+// FAST builds fold the x and y parts of a stage into R as soon as they are
+// known, R <- a R - dtinvh (x+y flux differences), and keep only the three
+// G/P accumulators (sum G, sum P, div u) for the z sweep, which then finishes
+// R <- R - dtinvh (z flux differences - div u sum G / 6 ...).  The same
+// algebra as the reference's single update, rounded in two parts; it moves
+// the R read from the bandwidth-bound z sweep to the x+y sweep, which has
+// bandwidth to spare, and drops 7 accumulator fields from the z sweep's
+// reads.  The bit-exact builds keep the reference's single update.
+constexpr bool kFold = kFast;
 
 enum { MODE_FIRST = 0, MODE_ACC = 1, MODE_CONV = 2, MODE_STAGE = 3 };
 
@@ -127,9 +136,16 @@ __device__ __forceinline__ void emit_load(const StageArgs& A, const int e, EmitI
 {
     const int64_t fs = A.fs;
     if (MODE == MODE_FIRST) return;
+    if (!kFold || MODE == MODE_ACC) {
 #pragma unroll
-    for (int q = 0; q < NACC; ++q) in.acc[q] = A.acc[q*fs + e];
-    if (MODE == MODE_CONV || MODE == MODE_STAGE) {
+        for (int q = 0; q < NACC; ++q) in.acc[q] = A.acc[q*fs + e];
+    } else {
+        // folded: only the G/P accumulators remain
+        in.acc[A_SUMG] = A.acc[A_SUMG*fs + e];
+        in.acc[A_SUMP] = A.acc[A_SUMP*fs + e];
+        in.acc[A_DIVU] = A.acc[A_DIVU*fs + e];
+    }
+    if (MODE == MODE_CONV || MODE == MODE_STAGE || (kFold && MODE == MODE_ACC)) {
 #pragma unroll
         for (int q = 0; q < NVAR; ++q) in.R[q] = A.R[q*fs + e];
     }
@@ -153,7 +169,11 @@ __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, cons
     } else if (MODE == MODE_ACC) {
         // _yextraterm_hllc ("+=", GPUkernels.cu:439-441)
 #pragma unroll
-        for (int q = 0; q < NVAR; ++q) acc[q*fs + e] = (in.acc[q] + F.f[q]) - Fl.f[q];
+        for (int q = 0; q < NVAR; ++q) {
+            const Real t = (in.acc[q] + F.f[q]) - Fl.f[q];
+            if (kFold) A.R[q*fs + e] = A.a*in.R[q] - dtinvh*t;
+            else       acc[q*fs + e] = t;
+        }
         acc[A_SUMG*fs + e] = in.acc[A_SUMG] + cG;
         acc[A_SUMP*fs + e] = in.acc[A_SUMP] + cP;
         acc[A_DIVU*fs + e] = in.acc[A_DIVU] + (F.vel - Fl.vel);
@@ -165,12 +185,13 @@ __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, cons
         const Real factor6 = Real(1.0f) / Real(6.0f);
 #pragma unroll
         for (int q = 0; q < NVAR; ++q) {
-            const Real t = (in.acc[q] + F.f[q]) - Fl.f[q];
+            // folded: R already holds a R - dtinvh (x+y part)
+            const Real t = kFold ? F.f[q] - Fl.f[q] : (in.acc[q] + F.f[q]) - Fl.f[q];
             Real rhs;
             if (q == FG)      rhs = dtinvh*(t - dU * sG * factor6);
             else if (q == FP) rhs = dtinvh*(t - dU * sP * factor6);
             else              rhs = dtinvh*t;
-            const Real Rn = A.a*in.R[q] - rhs;
+            const Real Rn = (kFold ? in.R[q] : A.a*in.R[q]) - rhs;
             A.R[q*fs + e] = Rn;
             if (MODE == MODE_STAGE) A.uo[q*fs + e] = A.b*Rn + in.u[q];
         }
@@ -480,10 +501,18 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
     fix(w, load_row(A, e0, c0, g.ny, s, 2, w));     m.w4 = to_prim<Real, kFast, kP64>(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
     Real nxt[NVAR];
     Real nsg = load_row(A, e0, c0 + 1, g.ny, s, 2, nxt);
+    const Real dtinvh = stage_dtinvh(A);
     for (int k = c0 - 1; k <= c1; ++k) {
         fix(nxt, nsg);
         const Prim q = to_prim<Real, kFast, kP64>(nxt[0], nxt[1], nxt[2], nxt[3], nxt[4], nxt[5], nxt[6]);  // row k+2
         if (k + 3 <= c1 + 2) nsg = load_row(A, e0, k + 3, g.ny, s, 2, nxt);
+        // folded builds: R of row k-1, emitted below (every lane's address is
+        // an interior cell, stored or not)
+        Real Ro[NVAR];
+        if (kFold && k >= c0 + 1) {
+#pragma unroll
+            for (int f = 0; f < NVAR; ++f) Ro[f] = A.R[f*fs + e0 + (k - 1) * s];
+        }
         m.push(q);
         const CellRec ry = reconstruct<Real, kFast>(m.w0, m.w1, m.w2, m.w3, m.w4);   // row k, y-direction
         if (k >= c0) {
@@ -503,8 +532,11 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
                     const int e = e0 + (k - 1) * s;
                     const Real cGy = m.pGp + m.pm.G, cPy = m.pPp + m.pm.P;
 #pragma unroll
-                    for (int f = 0; f < NVAR; ++f)
-                        acc[f*fs + e] = ((Fxr.f[f] - Fxl.f[f]) + Fy.f[f]) - m.Fl.f[f];
+                    for (int f = 0; f < NVAR; ++f) {
+                        const Real t = ((Fxr.f[f] - Fxl.f[f]) + Fy.f[f]) - m.Fl.f[f];
+                        if (kFold) A.R[f*fs + e] = A.a*Ro[f] - dtinvh*t;
+                        else       acc[f*fs + e] = t;
+                    }
                     acc[A_SUMG*fs + e] = (rx.p.G + rx.m.G) + cGy;
                     acc[A_SUMP*fs + e] = (rx.p.P + rx.m.P) + cPy;
                     acc[A_DIVU*fs + e] = (Fxr.vel - Fxl.vel) + (Fy.vel - m.Fl.vel);
