@@ -86,11 +86,13 @@ struct Timed
 };
 
 // R <- a R - dtinvh L(u) (stage == 0), or that and u_out <- u + b R (stage == 1)
-void run_rhs(mpcf_ctx* c, const void* u, void* uo, double a, double b, double dtinvh, const StepState* st, int stage)
+void run_rhs(mpcf_ctx* c, const void* u, void* uo, double a, double b, double dtinvh, const StepState* st, int stage,
+             StepState* sos_next = nullptr)
 {
     mpcf::SweepParams p;
     p.u = u; p.R = c->R; p.acc = c->acc; p.uo = uo;
     p.a = a; p.b = b; p.dtinvh = dtinvh; p.st = st;
+    p.sos_next = sos_next; p.sos_cpu = (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0;
     p.bcmask = mpcf::detail::bc_mask(c->bc);
     // the fused x+y sweep wins when the arithmetic is cheap (FAST); the split
     // pair wins when it is VALU-bound (bit-exact build), unless forced
@@ -130,7 +132,7 @@ extern "C" int mpcf_create(mpcf_ctx** out, int nx, int ny, int nz, double h, uns
         return fail(MPCF_EINVAL, "mpcf_create: block too large for 32-bit element indices");
 
     mpcf_ctx* c = new mpcf_ctx();
-    c->g = g; c->h = h; c->flags = flags; c->cur = 0;
+    c->g = g; c->h = h; c->flags = flags; c->cur = 0; c->sos_folded = false;
     for (int i = 0; i < 6; ++i) c->bc[i] = MPCF_BC_ABSORBING;
     c->L = pick_launchers(flags);
     c->esize = elem_bytes(flags);
@@ -291,6 +293,7 @@ int transfer(mpcf_ctx* c, const int which, void* const* fields, const int host_f
 
 extern "C" int mpcf_upload(mpcf_ctx* c, int which, const float* const fields[7])
 {
+    if (c && which == MPCF_DATA) c->sos_folded = false;
     return transfer(c, which, (void* const*)fields, 0, true, "mpcf_upload");
 }
 
@@ -301,6 +304,7 @@ extern "C" int mpcf_download(mpcf_ctx* c, int which, float* const fields[7])
 
 extern "C" int mpcf_upload_f64(mpcf_ctx* c, int which, const double* const fields[7])
 {
+    if (c && which == MPCF_DATA) c->sos_folded = false;
     return transfer(c, which, (void* const*)fields, 1, true, "mpcf_upload_f64");
 }
 
@@ -332,6 +336,7 @@ extern "C" int mpcf_stream_out(mpcf_ctx* c, int kind, float* out)
 extern "C" int mpcf_stream_in(mpcf_ctx* c, const float* in)
 {
     if (!c || !in) return fail(MPCF_EINVAL, "mpcf_stream_in: NULL argument");
+    c->sos_folded = false;
     const size_t n = (size_t)c->g.nx * c->g.ny * c->g.nz;
     const Launchers* L = (c->flags & MPCF_FLAG_FP64) ? c->L : &kParity;
     HIP_TRY(hipMemcpyAsync(c->acc, in, n * 7 * sizeof(float), hipMemcpyHostToDevice, c->stream));
@@ -352,6 +357,7 @@ extern "C" int mpcf_load_ghosts(mpcf_ctx* c)
 extern "C" int mpcf_max_sos(mpcf_ctx* c, float* sos)
 {
     if (!c || !sos) return fail(MPCF_EINVAL, "mpcf_max_sos: NULL argument");
+    c->sos_folded = false;   // the full reduction clears the folded maximum
     {
         Timed t(c, K_SOS);
         c->L->sos(c->g, c->u[c->cur], c->st, (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0, 0, 0.0, c->h, 0.0, c->stream);
@@ -366,6 +372,7 @@ extern "C" int mpcf_max_sos(mpcf_ctx* c, float* sos)
 extern "C" int mpcf_convection(mpcf_ctx* c, float a, float dtinvh)
 {
     if (!c) return fail(MPCF_EINVAL, "mpcf_convection: NULL context");
+    c->sos_folded = false;
     run_rhs(c, c->u[c->cur], nullptr, a, 0.0, dtinvh, nullptr, 0);
     LAUNCH_CHECK("convection");
     return MPCF_OK;
@@ -374,6 +381,7 @@ extern "C" int mpcf_convection(mpcf_ctx* c, float a, float dtinvh)
 extern "C" int mpcf_update(mpcf_ctx* c, float b)
 {
     if (!c) return fail(MPCF_EINVAL, "mpcf_update: NULL context");
+    c->sos_folded = false;
     { Timed t(c, K_UPDATE); c->L->update(c->g, c->u[c->cur], c->R, b, c->stream); }
     LAUNCH_CHECK("update");
     return MPCF_OK;
@@ -382,6 +390,7 @@ extern "C" int mpcf_update(mpcf_ctx* c, float b)
 extern "C" int mpcf_process_all(mpcf_ctx* c, float a, float b, float dtinvh)
 {
     if (!c) return fail(MPCF_EINVAL, "mpcf_process_all: NULL context");
+    c->sos_folded = false;
     run_rhs(c, c->u[c->cur], c->u[1 - c->cur], a, b, dtinvh, nullptr, 1);
     LAUNCH_CHECK("process_all");
     c->cur = 1 - c->cur;
@@ -398,15 +407,21 @@ extern "C" int mpcf_step(mpcf_ctx* c, double cfl, double dt_max, double* dt)
                                  "drive the stages with mpcf_max_sos and mpcf_process_all");
     {
         Timed t(c, K_SOS);
-        c->L->sos(c->g, c->u[c->cur], c->st, (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0, 1, cfl, c->h, dt_max, c->stream);
+        // the maximum of this state was folded into the previous step's last
+        // z sweep unless something else wrote u since: then only dt remains
+        if (c->sos_folded) c->L->sos_folded(c->st, cfl, c->h, dt_max, c->stream);
+        else c->L->sos(c->g, c->u[c->cur], c->st, (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0, 1, cfl, c->h, dt_max,
+                       c->stream);
     }
     LAUNCH_CHECK("sos/dt");
+    c->sos_folded = false;
     // the sweeps apply the boundary conditions as they load: no ghost fill
     for (int s = 0; s < 3; ++s) {
-        run_rhs(c, c->u[c->cur], c->u[1 - c->cur], kA[s], kB[s], 0.0, c->st, 1);
+        run_rhs(c, c->u[c->cur], c->u[1 - c->cur], kA[s], kB[s], 0.0, c->st, 1, s == 2 ? c->st : nullptr);
         LAUNCH_CHECK("stage");
         c->cur = 1 - c->cur;
     }
+    c->sos_folded = true;
     if (dt) {
         double d;
         float sos;

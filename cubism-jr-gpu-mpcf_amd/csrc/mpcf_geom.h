@@ -81,6 +81,11 @@ struct StepState
     // the last one finalizes and puts the counter back to zero
     unsigned int sos_count;
     unsigned int z_count;
+    // maximum speed of sound of the state the last stage of mpcf_step wrote,
+    // folded into that stage's z sweep (atomicMax targets, as sos_bits[_d]);
+    // the next step turns it into dt instead of reading the state again
+    unsigned int next_sos_bits;
+    unsigned long long next_sos_bits_d;
 };
 
 } // namespace mpcf

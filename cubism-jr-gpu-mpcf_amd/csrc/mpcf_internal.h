@@ -47,6 +47,7 @@ struct Launchers
     decltype(&mpcf::parity::launch_ghosts) ghosts;
     decltype(&mpcf::parity::launch_halo) halo;
     decltype(&mpcf::parity::launch_sos) sos;
+    decltype(&mpcf::parity::launch_sos_folded) sos_folded;
     decltype(&mpcf::parity::launch_sweep) sweep;
     decltype(&mpcf::parity::launch_update) update;
     decltype(&mpcf::parity::launch_pack) pack;
@@ -59,7 +60,7 @@ struct Launchers
 };
 
 #define MPCF_LAUNCHERS(NS)                                                                           \
-    {mpcf::NS::launch_ghosts, mpcf::NS::launch_halo, mpcf::NS::launch_sos, mpcf::NS::launch_sweep, mpcf::NS::launch_update, \
+    {mpcf::NS::launch_ghosts, mpcf::NS::launch_halo, mpcf::NS::launch_sos, mpcf::NS::launch_sos_folded, mpcf::NS::launch_sweep, mpcf::NS::launch_update, \
      mpcf::NS::launch_pack, mpcf::NS::launch_stream, mpcf::NS::launch_test_weno,                      \
      mpcf::NS::launch_test_face,                                                                      \
      mpcf::NS::batch_sos, mpcf::NS::batch_xy, mpcf::NS::batch_z}
@@ -123,6 +124,10 @@ struct mpcf_ctx
     bool own_stream;
     size_t bytes;
     const Launchers* L;
+    // the device holds the max speed of sound of the current u, folded into
+    // the last stage of the previous mpcf_step (cleared by anything else that
+    // writes u or runs the full reduction)
+    bool sos_folded;
     // optional per-kernel event timing (mpcf_timing_enable)
     bool timing;
     std::vector<hipEvent_t> ev_pool;

@@ -54,6 +54,20 @@ typedef PrimT<Real> Prim;
 constexpr bool kP64 = MPCF_P64 != 0;
 // the speed-of-sound reduction and dt/h run in binary64
 constexpr bool kSosD = MPCF_P64 != 0 || MPCF_F64 != 0;
+// scalar type of the speed-of-sound reduction
+typedef std::conditional<kSosD, double, float>::type SosT;
+
+// c + max(|rho u|, |rho v|, |rho w|) / rho of one cell (GPUkernels.cu:1206-1212,
+// or MaxSpeedOfSound.cpp:43-47 with 0.5/rho in binary64 for CPU_FORM)
+__device__ __forceinline__ SosT sos_cell(const bool cpu_form, const SosT r, const SosT mu, const SosT mv,
+                                         const SosT mw, const SosT E, const SosT G, const SosT P)
+{
+    SosT p;
+    if (cpu_form) p = (SosT)((E - (mu*mu + mv*mv + mw*mw)*(0.5/r) - P)/G);
+    else          p = (E - (mu*mu + mv*mv + mw*mw)*(SosT(0.5f)/r) - P) / G;
+    const SosT c = msqrt(((p + P) / G + p) / r);
+    return c + mmax(mmax(fabs(mu), fabs(mv)), fabs(mw)) / r;
+}
 
 typedef FaceFluxT<Real> FaceFlux;
 typedef CellRecT<Real> CellRec;
@@ -80,6 +94,9 @@ struct StageArgs
     int64_t fs;                    // field stride (elements)
     Real a, b, dtinvh;
     const StepState* st;           // if non-null, dtinvh is read from st->dtinvh
+    StepState* sos_next;           // non-null: fold the max speed of sound of the written u into
+                                   // sos_next->next_sos_bits[_d] (last stage of a step)
+    int sos_cpu;                   // its form: MaxSpeedOfSound_CPP (1) or the GPU kernel's (0)
     int bcmask;                    // bit f: face f reflecting; bit 8+f: face f a halo
                                    // face (ghost band imported); else absorbing
 };
@@ -155,7 +172,8 @@ __device__ __forceinline__ void emit_load(const StageArgs& A, const int e, EmitI
 // Fl = flux at its left face; cG/cP = plus-state + minus-state of G/P.
 template <int MODE>
 __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, const int e, const EmitIn<MODE>& in,
-                                     const FaceFlux& F, const FaceFlux& Fl, const Real cG, const Real cP)
+                                     const FaceFlux& F, const FaceFlux& Fl, const Real cG, const Real cP,
+                                     SosT& smax)
 {
     Real* __restrict__ acc = A.acc;
     const int64_t fs = A.fs;
@@ -183,6 +201,7 @@ __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, cons
         const Real sP = in.acc[A_SUMP] + cP;
         const Real dU = in.acc[A_DIVU] + (F.vel - Fl.vel);
         const Real factor6 = Real(1.0f) / Real(6.0f);
+        Real un[NVAR];
 #pragma unroll
         for (int q = 0; q < NVAR; ++q) {
             // folded: R already holds a R - dtinvh (x+y part)
@@ -193,8 +212,14 @@ __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, cons
             else              rhs = dtinvh*t;
             const Real Rn = (kFold ? in.R[q] : A.a*in.R[q]) - rhs;
             A.R[q*fs + e] = Rn;
-            if (MODE == MODE_STAGE) A.uo[q*fs + e] = A.b*Rn + in.u[q];
+            if (MODE == MODE_STAGE) {
+                un[q] = A.b*Rn + in.u[q];
+                A.uo[q*fs + e] = un[q];
+            }
         }
+        // the next step's speed of sound, from the values just stored
+        if (MODE == MODE_STAGE && A.sos_next)
+            smax = mmax(smax, sos_cell(A.sos_cpu != 0, un[0], un[1], un[2], un[3], un[4], un[5], un[6]));
     }
 }
 
@@ -206,6 +231,7 @@ struct March
     Prim pm;                   // minus-state (face k-1/2) of cell k-1
     Real pGp, pPp;             // plus-state G,P (face k-3/2) of cell k-1
     FaceFlux Fl;               // flux at face k-3/2
+    SosT smax = 0;             // running max speed of sound of the emitted cells (folded step)
 
     __device__ __forceinline__ void push(const Prim& q) { w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = q; }
 
@@ -216,7 +242,7 @@ struct March
         const CellRec r = reconstruct<Real, kFast>(w0, w1, w2, w3, w4);
         if (k >= c0) {
             const FaceFlux F = riemann<D, Real, kFast>(pm, r.p);
-            if (k >= c0 + 1 && store) emit<MODE>(A, dtinvh, e_km1, in, F, Fl, pGp + pm.G, pPp + pm.P);
+            if (k >= c0 + 1 && store) emit<MODE>(A, dtinvh, e_km1, in, F, Fl, pGp + pm.G, pPp + pm.P, smax);
             Fl = F;
         }
         pm = r.m; pGp = r.p.G; pPp = r.p.P;
@@ -238,16 +264,14 @@ __device__ __forceinline__ Real stage_dtinvh(const StageArgs& A)
 // y / z sweeps: block of 256 lanes over the flattened (x, transverse) columns,
 // blockIdx.y = chunk of the marched axis.
 // ---------------------------------------------------------------------------
+// one column of a y / z sweep; returns the running max speed of sound of the
+// cells it wrote (zero unless A.sos_next)
 template <int D, int MODE>
-__device__ __forceinline__ void sweep_yz_body(const Geom& g, const StageArgs& A, const int chunk,
-                                              const int bcol, const int bz)
+__device__ __forceinline__ SosT sweep_yz_column(const Geom& g, const StageArgs& A, const int chunk, const int col,
+                                                const int bz)
 {
-    // columns (x, t) flattened x-fastest, 256 per block: consecutive lanes are
-    // consecutive x (coalesced) and no lane idles when nx is not a multiple of 64
     const int nt = D == 1 ? g.nz : g.ny;
     const int nd = D == 1 ? g.ny : g.nz;
-    const int col = bcol * 256 + threadIdx.x;
-    if (col >= g.nx * nt) return;   // no barriers in the body
     const int x = col % g.nx;
     const int t = col / g.nx;
     const int c0 = bz * chunk;
@@ -292,6 +316,28 @@ __device__ __forceinline__ void sweep_yz_body(const Geom& g, const StageArgs& A,
         m.push(q);
         m.step(A, dtinvh, k, c0, e0 + (k - 1) * s, ein);
         ein = ein_next;
+    }
+    return m.smax;
+}
+
+template <int D, int MODE>
+__device__ __forceinline__ void sweep_yz_body(const Geom& g, const StageArgs& A, const int chunk,
+                                              const int bcol, const int bz)
+{
+    // columns (x, t) flattened x-fastest, 256 per block: consecutive lanes are
+    // consecutive x (coalesced) and no lane idles when nx is not a multiple of 64
+    const int nt = D == 1 ? g.nz : g.ny;
+    const int col = bcol * 256 + threadIdx.x;
+    SosT smax = 0;
+    if (col < g.nx * nt) smax = sweep_yz_column<D, MODE>(g, A, chunk, col, bz);   // no barriers in the body
+    if (MODE == MODE_STAGE && A.sos_next) {
+        // one atomic max per wave; every lane of the wave takes part
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) smax = mmax(smax, __shfl_xor(smax, off, 64));
+        if ((threadIdx.x & 63) == 0 && smax > SosT(0)) {
+            if (kSosD) atomicMax(&A.sos_next->next_sos_bits_d, (unsigned long long)__double_as_longlong((double)smax));
+            else       atomicMax(&A.sos_next->next_sos_bits, __float_as_uint((float)smax));
+        }
     }
 }
 
@@ -638,9 +684,6 @@ void k_halo(const Geom g, Real* __restrict__ u, const int64_t fs, const int face
 // the reduction target and the counter back to zero for the next step.
 // ---------------------------------------------------------------------------
 
-// scalar type of the speed-of-sound reduction
-typedef std::conditional<kSosD, double, float>::type SosT;
-
 struct DtArgs
 {
     int with_dt;                  // 0: only the maximum (mpcf_max_sos)
@@ -664,6 +707,9 @@ __device__ __forceinline__ void sos_finalize(StepState* st, const DtArgs& d)
         atomicExch(&st->sos_bits, 0u);
     }
     st->sos_is_d = kSosD ? 1 : 0;
+    // the folded maximum of the next step starts from zero
+    atomicExch(&st->next_sos_bits, 0u);
+    atomicExch(&st->next_sos_bits_d, 0ull);
     if (d.with_dt) {
         // binary64 dt from the binary32 (or binary64) maximum
         double dt = d.cfl*d.h/sos;
@@ -686,13 +732,8 @@ __device__ __forceinline__ void sos_body(const Geom& g, const Real* __restrict__
         const int e0 = g.idx(0, row % g.ny, row / g.ny);
         for (int x = threadIdx.x; x < g.nx; x += 64) {
             const int e = e0 + x;
-            const SosT r = u[e], mu = u[fs + e], mv = u[2*fs + e], mw = u[3*fs + e];
-            const SosT E = u[4*fs + e], G = u[5*fs + e], P = u[6*fs + e];
-            SosT p;
-            if (CPU_FORM) p = (SosT)((E - (mu*mu + mv*mv + mw*mw)*(0.5/r) - P)/G);
-            else          p = (E - (mu*mu + mv*mv + mw*mw)*(SosT(0.5f)/r) - P) / G;
-            const SosT c = msqrt(((p + P) / G + p) / r);
-            sos = mmax(sos, c + mmax(mmax(fabs(mu), fabs(mv)), fabs(mw)) / r);
+            sos = mmax(sos, sos_cell(CPU_FORM != 0, u[e], u[fs + e], u[2*fs + e], u[3*fs + e], u[4*fs + e],
+                                     u[5*fs + e], u[6*fs + e]));
         }
     }
 #pragma unroll
@@ -709,6 +750,17 @@ __device__ __forceinline__ void sos_body(const Geom& g, const Real* __restrict__
             sos_finalize(st, d);
         }
     }
+}
+
+// dt from the maximum folded into the previous step's last z sweep
+__global__ __launch_bounds__(64)
+void k_sos_folded(StepState* st, const DtArgs d)
+{
+    if (threadIdx.x != 0) return;
+    if (kSosD) atomicExch(&st->sos_bits_d, atomicOr(&st->next_sos_bits_d, 0ull));
+    else       atomicExch(&st->sos_bits, atomicOr(&st->next_sos_bits, 0u));
+    __threadfence();
+    sos_finalize(st, d);
 }
 
 template <int CPU_FORM>
@@ -855,6 +907,7 @@ __device__ __forceinline__ int find_case(const int* __restrict__ off, const int 
 __device__ __forceinline__ StageArgs case_args(const CaseDesc& C, const int cur, const Real a, const Real b)
 {
     StageArgs A;
+    A.sos_next = nullptr; A.sos_cpu = 0;
     A.bcmask = 0;
     for (int f = 0; f < 6; ++f) A.bcmask |= (C.bc[f] == 1) << f;
     A.u = (const Real*)C.u[cur]; A.R = (Real*)C.R; A.acc = (Real*)C.acc; A.uo = (Real*)C.u[1 - cur];
@@ -958,6 +1011,12 @@ void launch_sos(const Geom& g, const void* uv, StepState* st, const int cpu_form
     else          hipLaunchKernelGGL(k_sos<0>, dim3(nb), dim3(64, 4), 0, s, g, u, g.fstride, st, d);
 }
 
+void launch_sos_folded(StepState* st, const double cfl, const double h, const double dt_max, hipStream_t s)
+{
+    const DtArgs d{1, cfl, h, dt_max};
+    hipLaunchKernelGGL(k_sos_folded, dim3(1), dim3(64), 0, s, st, d);
+}
+
 // one direction sweep.  dir 0 (x) assigns the accumulators, dir 1 (y)
 // accumulates, dir 3 does x and y fused (assigns), dir 2 (z) finishes the
 // register update (stage == 0: R only, as Convection; stage == 1: R and the
@@ -968,6 +1027,7 @@ void launch_sweep(const Geom& g, const int dir, const SweepParams& p, const int 
     A.u = (const Real*)p.u; A.R = (Real*)p.R; A.acc = (Real*)p.acc; A.uo = (Real*)p.uo; A.fs = g.fstride;
     A.a = (Real)p.a; A.b = (Real)p.b; A.dtinvh = (Real)p.dtinvh; A.st = p.st;
     A.bcmask = p.bcmask;
+    A.sos_next = p.sos_next; A.sos_cpu = p.sos_cpu;
     if (dir == 3) {
         const int chunk = pick_chunk(g.ny, (int64_t)g.nx * g.nz, 1);
         int dims[3];

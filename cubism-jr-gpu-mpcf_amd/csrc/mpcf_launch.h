@@ -19,6 +19,9 @@ struct SweepParams
     double a, b, dtinvh;   // stage coefficients, rounded to the storage type by
                            // the launcher; dtinvh ignored when st != NULL
     const StepState* st;   // device step state carrying dt/h
+    StepState* sos_next = nullptr;  // non-null: the z sweep of stage == 1 folds the max speed of
+                                    // sound of the state it writes into sos_next->next_sos_bits
+    int sos_cpu = 0;                // its form (MPCF_FLAG_SOS_CPU)
     int bcmask;            // bit f: face f reflecting; bit 8+f: face f a halo face
                            // (ghost band imported); else absorbing.  The sweeps
                            // apply the boundary conditions as they load
@@ -58,6 +61,7 @@ struct BatchDev
 namespace mpcf { namespace NS {                                                                            \
 void launch_ghosts(const Geom& g, void* u, const int bc[6], hipStream_t s);                                \
 void launch_halo(const Geom& g, void* u, void* buf, int face, int import, hipStream_t s);                  \
+void launch_sos_folded(StepState* st, double cfl, double h, double dt_max, hipStream_t s);                 \
 void launch_sos(const Geom& g, const void* u, StepState* st, int cpu_form, int with_dt, double cfl,          \
                 double h, double dt_max, hipStream_t s);                                                   \
 void launch_sweep(const Geom& g, int dir, const SweepParams& p, int stage, hipStream_t s);                \

@@ -229,3 +229,59 @@ def test_fresh_block_ignores_freed_memory(mpcf, oracle):
         u, R = b.data, b.tmp
     assert_equal(u, ob.data, "u after reuse")
     assert_equal(R, ob.tmp, "R after reuse")
+
+
+def test_folded_speed_of_sound_follows_every_write(mpcf, oracle):
+    """The last stage of a step folds the next step's max speed of sound into
+    its z sweep.  Anything else that writes u (upload, process_all, the full
+    reduction of mpcf_max_sos) must make the next step reduce the state again:
+    the dt sequence of a run interleaved with such calls equals the oracle's."""
+    n = 40
+    h = 1.0 / n
+    s1 = states.random_smooth(n, n, n, seed=31)
+    s2 = states.shock_tube(n, n, n, axis=1, seed=32)
+    ob = oracle.Block(n, n, n)
+    say("folded sos: steps, upload, max_sos, process_all", n)
+    with mpcf.Block(n, n, n, h) as b:
+        b.data = s1
+        ob.data = s1
+        for _ in range(2):
+            assert b.step(0.3) == ob.step(h, 0.3, 1e30, [0] * 6)[0]
+        # a new state: the folded maximum of the old one must not be used
+        b.data = s2
+        ob.data = s2
+        for _ in range(2):
+            assert b.step(0.3) == ob.step(h, 0.3, 1e30, [0] * 6)[0]
+        # the full reduction in between gives the same maximum
+        sos = b.max_sos()
+        assert b.step(0.3) == ob.step(h, 0.3, 1e30, [0] * 6)[0]
+        assert b.last_dt()[1] == sos
+        # one stage driven from the host, then steps again
+        dt = 0.3 * h / b.max_sos()
+        b.process_all(0.0, 0.25, float(np.float32(dt / h)))
+        ob.apply_bc([0] * 6)
+        ob.rhs(0.0, float(np.float32(dt / h)))
+        ob.update(0.25)
+        for _ in range(2):
+            assert b.step(0.3) == ob.step(h, 0.3, 1e30, [0] * 6)[0]
+        assert_equal(b.data, ob.data, "u after interleaved calls")
+
+
+@pytest.mark.parametrize("flags", [dict(fast=True), dict(p64=True), dict(fp64=True)])
+def test_folded_speed_of_sound_equals_full_reduction(mpcf, flags):
+    """In every build the folded maximum is the full reduction's: a run whose
+    every step starts from a fresh reduction (mpcf_max_sos clears the folded
+    value) takes the same dts to the same state."""
+    n = 48
+    s = states.random_smooth(n, n, n, seed=41)
+    say("folded vs full sos", flags)
+    with mpcf.Block(n, n, n, 1.0 / n, **flags) as a, mpcf.Block(n, n, n, 1.0 / n, **flags) as b:
+        a.data = s
+        b.data = s
+        for _ in range(3):
+            da = a.step(0.3)
+            b.max_sos()
+            assert da == b.step(0.3)
+        ua = a.download_f64() if flags.get("fp64") else a.data
+        ub = b.download_f64() if flags.get("fp64") else b.data
+        assert np.array_equal(ua, ub)
