@@ -709,7 +709,7 @@ void batch_enqueue_step(mpcf_batch* b, double cfl, double dt_max)
     b->L->b_sos(b->dev, b->tot_sos, cpu_form, cfl, dt_max, b->stream);
     for (int s = 0; s < 3; ++s) {
         b->L->b_xy(b->dev, b->tot_xy, kA[s], kB[s], b->stream);
-        b->L->b_z(b->dev, b->tot_z, kA[s], kB[s], b->stream);
+        b->L->b_z(b->dev, b->tot_z, kA[s], kB[s], s == 2, cpu_form, b->stream);
     }
 }
 
@@ -962,6 +962,8 @@ extern "C" int mpcf_batch_autotune(mpcf_batch* b, int steps, double cfl, int* be
             const size_t reg = 7 * (size_t)c->g.fstride * c->esize;
             hipMemcpyAsync(c->u[c->cur], save[i], reg, hipMemcpyDeviceToDevice, b->stream);
             hipMemcpyAsync(c->R, (char*)save[i] + reg, reg, hipMemcpyDeviceToDevice, b->stream);
+            // the folded speed of sound belongs to the trial state: reduce again
+            hipMemsetAsync(&c->st->folded, 0, sizeof(int), b->stream);
         }
         hipStreamSynchronize(b->stream);
         if (rc != MPCF_OK) break;

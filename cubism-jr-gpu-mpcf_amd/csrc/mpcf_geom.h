@@ -86,6 +86,10 @@ struct StepState
     // the next step turns it into dt instead of reading the state again
     unsigned int next_sos_bits;
     unsigned long long next_sos_bits_d;
+    // batched cases: set by the last z block of a step's last stage once
+    // next_sos_bits holds the maximum of the state it wrote (every batched
+    // step folds; a case's state is zeroed when it joins a batch)
+    int folded;
 };
 
 } // namespace mpcf

@@ -924,7 +924,19 @@ void kb_sos(const BatchDev B, const double cfl, const double dt_max)
     const int c = find_case(B.off_sos, B.n, blockIdx.x);
     const CaseDesc& C = B.cases[c];
     const DtArgs d{1, cfl, C.h, dt_max};
-    sos_body<CPU_FORM>(C.g, (const Real*)C.u[B.cur[c]], C.g.fstride, C.st, blockIdx.x - B.off_sos[c], C.nb_sos, wmax, d);
+    const int local = blockIdx.x - B.off_sos[c];
+    if (C.st->folded) {
+        // the previous step's last z sweep folded this state's maximum: only
+        // dt remains, done by the case's first block
+        if (local == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+            if (kSosD) atomicExch(&C.st->sos_bits_d, atomicOr(&C.st->next_sos_bits_d, 0ull));
+            else       atomicExch(&C.st->sos_bits, atomicOr(&C.st->next_sos_bits, 0u));
+            __threadfence();
+            sos_finalize(C.st, d);
+        }
+        return;
+    }
+    sos_body<CPU_FORM>(C.g, (const Real*)C.u[B.cur[c]], C.g.fstride, C.st, local, C.nb_sos, wmax, d);
 }
 
 __global__ __launch_bounds__(256)
@@ -939,13 +951,16 @@ void kb_sweep_xy(const BatchDev B, const Real a, const Real b)
 
 // the z sweep finishes the stage; the last block of each case to finish
 // flips the case's current-buffer index (every block has read it by then)
+// fold != 0 (a step's last stage): the sweep also folds the maximum speed of
+// sound of the state it writes, for the next step's kb_sos
 __global__ __launch_bounds__(256)
-void kb_sweep_z(const BatchDev B, const Real a, const Real b)
+void kb_sweep_z(const BatchDev B, const Real a, const Real b, const int fold, const int cpu_form)
 {
     const int c = find_case(B.off_z, B.n, blockIdx.x);
     const CaseDesc& C = B.cases[c];
     const int local = blockIdx.x - B.off_z[c];
-    const StageArgs A = case_args(C, B.cur[c], a, b);
+    StageArgs A = case_args(C, B.cur[c], a, b);
+    if (fold) { A.sos_next = C.st; A.sos_cpu = cpu_form; }
     sweep_yz_body<2, MODE_STAGE>(C.g, A, C.chunk_z, local % C.nbx_z, local / C.nbx_z);
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -953,6 +968,7 @@ void kb_sweep_z(const BatchDev B, const Real a, const Real b)
         const unsigned nblk = (unsigned)(B.off_z[c + 1] - B.off_z[c]);
         if (atomicAdd(&C.st->z_count, 1u) == nblk - 1u) {
             atomicExch(&B.cur[c], 1 - B.cur[c]);
+            if (fold) atomicExch(&C.st->folded, 1);
             atomicExch(&C.st->z_count, 0u);
         }
     }
@@ -1127,9 +1143,10 @@ void batch_xy(const BatchDev& B, const int blocks, const double a, const double 
     hipLaunchKernelGGL(kb_sweep_xy, dim3(blocks), dim3(64, 4), 0, s, B, (Real)a, (Real)b);
 }
 
-void batch_z(const BatchDev& B, const int blocks, const double a, const double b, hipStream_t s)
+void batch_z(const BatchDev& B, const int blocks, const double a, const double b, const int fold, const int cpu_form,
+             hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_sweep_z, dim3(blocks), dim3(256), 0, s, B, (Real)a, (Real)b);
+    hipLaunchKernelGGL(kb_sweep_z, dim3(blocks), dim3(256), 0, s, B, (Real)a, (Real)b, fold, cpu_form);
 }
 
 } // namespace MPCF_NS

@@ -72,7 +72,7 @@ void launch_test_weno(int n, const void* in5, void* m, void* p, hipStream_t s); 
 void launch_test_face(int n, int dir, const void* st, void* out, hipStream_t s);                           \
 void batch_sos(const BatchDev& B, int blocks, int cpu_form, double cfl, double dt_max, hipStream_t s);    \
 void batch_xy(const BatchDev& B, int blocks, double a, double b, hipStream_t s);                           \
-void batch_z(const BatchDev& B, int blocks, double a, double b, hipStream_t s);                            \
+void batch_z(const BatchDev& B, int blocks, double a, double b, int fold, int cpu_form, hipStream_t s);    \
 void xy_grid(const Geom& g, int chunk, int dims[3]);                                                       \
 void z_grid(const Geom& g, int chunk, int dims[3]);                                                        \
 int default_chunk(int nd, long long columns);                                                              \

@@ -497,7 +497,7 @@ void enqueue_step_kernels(mpcf_server* S)
     S->L->b_sos(S->dev, S->tot[2], cpu_form, S->cfl, 1e30, S->stream);
     for (int s = 0; s < 3; ++s) {
         S->L->b_xy(S->dev, S->tot[0], kA[s], kB[s], S->stream);
-        S->L->b_z(S->dev, S->tot[1], kA[s], kB[s], S->stream);
+        S->L->b_z(S->dev, S->tot[1], kA[s], kB[s], s == 2, cpu_form, S->stream);
     }
 }
 
