@@ -8,6 +8,7 @@ Objects:
   fastp.o   mpcf_kernels.hip, fast + MPCF_P64=1 (binary64 pressure recovery and reductions)
   f64.o     mpcf_kernels.hip, MPCF_F64=1, -ffp-contract=off (the scheme in binary64)
   reduce.o  mpcf_reduce.hip  (binary64 block comparisons and integrals)
+  wavelet.o mpcf_wavelet.hip (wavelet-compressed dumps; linked with zlib)
   capi.o    the C-ABI (include/mpcf_hip.h)
   ic.o      initial conditions;  sic.o  the shock-induced cloud input and IC
   sim_ic.o  the Sod, 2D shock-bubble and static-index drivers' ICs (-ffp-contract=off)
@@ -67,6 +68,7 @@ def build(verbose: bool = False) -> str:
         ("fastp.o", "mpcf_kernels.hip", ["-DMPCF_FAST=1", "-DMPCF_P64=1", "-ffp-contract=fast"]),
         ("f64.o", "mpcf_kernels.hip", ["-DMPCF_FAST=0", "-DMPCF_F64=1", "-ffp-contract=off"]),
         ("reduce.o", "mpcf_reduce.hip", ["-ffp-contract=off"]),
+        ("wavelet.o", "mpcf_wavelet.hip", ["-ffp-contract=off"]),
         ("capi.o", "mpcf_capi.cpp", ["-x", "hip"]),
         ("serve.o", "mpcf_serve.cpp", ["-x", "hip"]),
         ("ic.o", "ic.cpp", ["-x", "hip"]),
@@ -87,7 +89,7 @@ def build(verbose: bool = False) -> str:
                 print(" ".join(cmd))
     objs = [c[-1] for c in cmds]
     tmp = LIB + ".tmp"
-    _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lz"])
     os.replace(tmp, LIB)
     # the C++ driver over the C-ABI (mpcf.hpp), next to the library
     exe = os.path.join(PKG, "mpcf_run")

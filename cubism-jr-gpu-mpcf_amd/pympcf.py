@@ -44,7 +44,7 @@ EXPORTS = [
     "mpcf_read_save_info", "mpcf_dump", "mpcf_save", "mpcf_restart", "mpcf_post_shock_ratio",
     "mpcf_sod_defaults", "mpcf_ic_sod", "mpcf_sbi_defaults", "mpcf_sbi_config", "mpcf_read_bubbles",
     "mpcf_ic_sbi", "mpcf_ic_static", "mpcf_ic_steady", "mpcf_halo_count", "mpcf_halo_export",
-    "mpcf_halo_import",
+    "mpcf_halo_import", "mpcf_wavelet_dump", "mpcf_wavelet_read", "mpcf_wavelet_encode", "mpcf_wavelet_decode",
 ]
 
 # streamer kinds of the dumps (mpcf_stream_out)
@@ -178,6 +178,10 @@ def lib():
         L.mpcf_halo_count.argtypes = [P, I, P]
         L.mpcf_halo_export.argtypes = [P, I, P]
         L.mpcf_halo_import.argtypes = [P, I, P]
+        L.mpcf_wavelet_dump.argtypes = [P, I, C.c_char_p, F, I, P]
+        L.mpcf_wavelet_read.argtypes = [C.c_char_p, P, LL, P]
+        L.mpcf_wavelet_encode.argtypes = [P, I, F, I, P, LL, P]
+        L.mpcf_wavelet_decode.argtypes = [P, LL, I, I, P]
         S_ = C.c_char_p
         L.mpcf_stream_channels.argtypes = [I]
         L.mpcf_stream_out.argtypes = [P, I, P]
@@ -389,6 +393,15 @@ class Block:
         """The device buffer at dev_ptr into the ghost band of `face`
         (asynchronous on this block's stream)."""
         _check(lib().mpcf_halo_import(self._h, int(face), C.c_void_p(int(dev_ptr))))
+
+    def wavelet_dump(self, channel, path, threshold=0.0, half=False):
+        """Wavelet-compressed dump of StreamerGridPointIterative channel 0..6 to
+        <path>.StreamerGridPointIterative.channel<k>; returns (stream bytes,
+        file bytes, compression rate)."""
+        st = (C.c_double * 3)()
+        _check(lib().mpcf_wavelet_dump(self._h, int(channel), str(path).encode(), float(threshold), int(bool(half)),
+                                       C.cast(st, C.c_void_p)))
+        return tuple(st)
 
     @property
     def stream(self):
@@ -639,6 +652,37 @@ def ic_steady(nx, ny, nz, rho=1.0, u=0.0, v=0.0, w=0.0, p=1.0, g=1.4, pc=0.0):
     st = (C.c_double * 7)(*map(float, (rho, u, v, w, p, g, pc)))
     _check(lib().mpcf_ic_steady(nx, ny, nz, C.cast(st, C.c_void_p), _fields(f)))
     return f
+
+
+def wavelet_encode(cube, threshold=0.0, half=False):
+    """The compressor's stream (bitset bytes, then survivors) of a cube of edge
+    n (a power of two in 8..1024)."""
+    a = np.ascontiguousarray(cube, dtype=np.float32)
+    n = a.shape[0]
+    assert a.shape == (n, n, n)
+    nb = C.c_longlong()
+    _check(lib().mpcf_wavelet_encode(_ptr(a), n, float(threshold), int(bool(half)), None, 0,
+                                     C.cast(C.byref(nb), C.c_void_p)))
+    out = np.empty(nb.value, np.uint8)
+    _check(lib().mpcf_wavelet_encode(_ptr(a), n, float(threshold), int(bool(half)), _ptr(out), nb.value,
+                                     C.cast(C.byref(nb), C.c_void_p)))
+    return out.tobytes()
+
+
+def wavelet_decode(buf, n, half=False):
+    b = np.frombuffer(bytes(buf), np.uint8).copy()
+    out = np.empty((n, n, n), np.float32)
+    _check(lib().mpcf_wavelet_decode(_ptr(b), b.size, int(n), int(bool(half)), _ptr(out)))
+    return out
+
+
+def wavelet_read(path):
+    """A wavelet dump read back: the (n, n, n) binary32 channel."""
+    n = C.c_int()
+    _check(lib().mpcf_wavelet_read(str(path).encode(), None, 0, C.cast(C.byref(n), C.c_void_p)))
+    out = np.empty((n.value,) * 3, np.float32)
+    _check(lib().mpcf_wavelet_read(str(path).encode(), _ptr(out), out.size, C.cast(C.byref(n), C.c_void_p)))
+    return out
 
 
 def write_save_info(path, t, step, fcount):

@@ -405,6 +405,36 @@ int mpcf_ic_steady(int nx, int ny, int nz, const double state[7], float* const f
  * probe for dumps), rounded to binary32. */
 int mpcf_ic_static(int nx, int ny, int nz, const int dims[3], float* const fields[7]);
 
+/* ------------------------------------------------------------------------
+ * Wavelet-compressed dumps (SerializerIO_WaveletCompression_MPI_Simple,
+ * source/IO/SerializerIO_WaveletCompression_MPI_Simple.h:26-808, over
+ * source/WaveletCompression/): 4th-order interpolating wavelets on the whole
+ * block, survivors |c| > threshold as a bitset plus binary32 (or the
+ * reference's truncating binary16) values, zlib-encoded, in the reference's
+ * file layout.  The block must be a cube of power-of-two edge 8..1024.
+ * ------------------------------------------------------------------------ */
+
+/* Write channel 0..6 of StreamerGridPointIterative (rho, u, v, w, p, G, P;
+ * Types.h:157-186) of the current state to "<path>.StreamerGridPointIterative.
+ * channel<k>" (SerializerIO_...::Write<channel>, :785-794).  stats (may be
+ * NULL): stream bytes before zlib, file bytes, and the reference's
+ * "compr. rate" (binary32 bytes of the channel / ocean bytes). */
+int mpcf_wavelet_dump(mpcf_ctx* ctx, int channel, const char* path, float threshold, int halffloat,
+                      double stats[3]);
+
+/* Read such a file back (decompress and inverse transform): *n = the block
+ * edge; cube (n^3 binary32, x fastest) is filled when cap >= n^3, else only
+ * *n is set. */
+int mpcf_wavelet_read(const char* file, float* cube, long long cap, int* n);
+
+/* The compressor on a host cube of edge n (WaveletCompressorGeneric::compress
+ * / decompress, WaveletCompressor.cpp:117-180): the stream handed to the
+ * encoder, bitset bytes then survivors; *nbytes = its size, copied to out up
+ * to cap bytes.  Decode reconstructs the cube from such a stream. */
+int mpcf_wavelet_encode(const float* cube, int n, float threshold, int halffloat, unsigned char* out,
+                        long long cap, long long* nbytes);
+int mpcf_wavelet_decode(const unsigned char* in, long long nbytes, int n, int halffloat, float* cube);
+
 /* Operation count of the scheme as implemented here, from a host run of the
  * same per-cell code with a counting scalar (binary32/64 add, sub, mul, div,
  * sqrt = 1 FLOP each; min/max and conversions listed apart):
