@@ -64,11 +64,12 @@ struct Timed
     mpcf_ctx* c;
     int kind;
     hipEvent_t b = nullptr;
-    Timed(mpcf_ctx* c_, int k) : c(c_), kind(k)
+    hipStream_t s;
+    Timed(mpcf_ctx* c_, int k, hipStream_t s_ = nullptr) : c(c_), kind(k), s(s_ ? s_ : c_->stream)
     {
         if (c->timing) {
             b = take_event(c);
-            if (b) hipEventRecord(b, c->stream);
+            if (b) hipEventRecord(b, s);
         }
     }
     ~Timed()
@@ -76,7 +77,7 @@ struct Timed
         if (c->timing && b) {
             hipEvent_t e = take_event(c);
             if (e) {
-                hipEventRecord(e, c->stream);
+                hipEventRecord(e, s);
                 c->marks.push_back({kind, b, e});
             } else {
                 c->ev_pool.push_back(b);
@@ -107,6 +108,59 @@ void run_rhs(mpcf_ctx* c, const void* u, void* uo, double a, double b, double dt
         c->L->sweep(c->g, 3, p, stage, c->stream);
     }
     { Timed t(c, K_SWEEP_Z); c->L->sweep(c->g, 2, p, stage, c->stream); }
+}
+
+// z-slabs of a pipelined step (0: none): fused x+y sweeps and a block deep
+// enough that a quarter of it still fills the GPU
+int pipeline_slabs(const mpcf_ctx* c)
+{
+    if (c->flags & MPCF_FLAG_NO_PIPELINE) return 0;
+    const bool split = (c->flags & MPCF_FLAG_SPLIT_XY) ? true
+                     : (c->flags & MPCF_FLAG_FUSED_XY) ? false
+                     : !(c->flags & MPCF_FLAG_FAST);
+    if (split || c->g.nz < 128) return 0;
+    return 4;
+}
+
+// the three stages of a step over S z-slabs on two streams.  The x+y sweep of
+// slab j needs the previous stage's z sweep of slab j only (its stencil is in
+// x and y; it also rewrites that slab's R and accumulators), so it waits for
+// that event; the z sweep of slab j waits for the x+y sweep of slab j and,
+// by the order of the second stream, for every z sweep of the previous
+// stage (its stencil reaches into the neighbouring slabs, and its output
+// buffer is the one they read).  The x+y sweeps of the next slab run while a
+// z sweep runs: VALU-bound and bandwidth-bound work overlap.
+int run_stages_pipelined(mpcf_ctx* c, const int S)
+{
+    if (!c->stream2) {
+        HIP_TRY(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+        c->pipe_ev.assign(2 * 3 * S, nullptr);
+        for (hipEvent_t& e : c->pipe_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    }
+    auto exy = [&](int s, int j) { return c->pipe_ev[(s * S + j) * 2]; };
+    auto ez = [&](int s, int j) { return c->pipe_ev[(s * S + j) * 2 + 1]; };
+    for (int s = 0; s < 3; ++s) {
+        mpcf::SweepParams p;
+        p.u = c->u[c->cur]; p.R = c->R; p.acc = c->acc; p.uo = c->u[1 - c->cur];
+        p.a = kA[s]; p.b = kB[s]; p.dtinvh = 0.0; p.st = c->st;
+        p.sos_next = s == 2 ? c->st : nullptr; p.sos_cpu = (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0;
+        p.bcmask = mpcf::detail::bc_mask(c->bc);
+        for (int j = 0; j < S; ++j) {
+            p.z0 = (int)((int64_t)c->g.nz * j / S);
+            p.z1 = (int)((int64_t)c->g.nz * (j + 1) / S);
+            if (s > 0) HIP_TRY(hipStreamWaitEvent(c->stream, ez(s - 1, j), 0));
+            { Timed t(c, K_SWEEP_X); c->L->sweep(c->g, 3, p, 1, c->stream); }
+            HIP_TRY(hipEventRecord(exy(s, j), c->stream));
+            HIP_TRY(hipStreamWaitEvent(c->stream2, exy(s, j), 0));
+            { Timed t(c, K_SWEEP_Z, c->stream2); c->L->sweep(c->g, 2, p, 1, c->stream2); }
+            HIP_TRY(hipEventRecord(ez(s, j), c->stream2));
+        }
+        LAUNCH_CHECK("stage");
+        c->cur = 1 - c->cur;
+    }
+    // everything after the step is ordered after its last z sweep
+    HIP_TRY(hipStreamWaitEvent(c->stream, ez(2, S - 1), 0));
+    return MPCF_OK;
 }
 
 } // namespace
@@ -176,6 +230,11 @@ extern "C" int mpcf_destroy(mpcf_ctx* c)
 {
     if (!c) return MPCF_OK;
     hipStreamSynchronize(c->stream);
+    if (c->stream2) {
+        hipStreamSynchronize(c->stream2);
+        for (hipEvent_t e : c->pipe_ev) if (e) hipEventDestroy(e);
+        hipStreamDestroy(c->stream2);
+    }
     if (c->own_stream) hipStreamDestroy(c->stream);
     for (auto& m : c->marks) { hipEventDestroy(m.b); hipEventDestroy(m.e); }
     for (auto e : c->ev_pool) hipEventDestroy(e);
@@ -416,10 +475,16 @@ extern "C" int mpcf_step(mpcf_ctx* c, double cfl, double dt_max, double* dt)
     LAUNCH_CHECK("sos/dt");
     c->sos_folded = false;
     // the sweeps apply the boundary conditions as they load: no ghost fill
-    for (int s = 0; s < 3; ++s) {
-        run_rhs(c, c->u[c->cur], c->u[1 - c->cur], kA[s], kB[s], 0.0, c->st, 1, s == 2 ? c->st : nullptr);
-        LAUNCH_CHECK("stage");
-        c->cur = 1 - c->cur;
+    const int slabs = pipeline_slabs(c);
+    if (slabs) {
+        const int rc = run_stages_pipelined(c, slabs);
+        if (rc != MPCF_OK) return rc;
+    } else {
+        for (int s = 0; s < 3; ++s) {
+            run_rhs(c, c->u[c->cur], c->u[1 - c->cur], kA[s], kB[s], 0.0, c->st, 1, s == 2 ? c->st : nullptr);
+            LAUNCH_CHECK("stage");
+            c->cur = 1 - c->cur;
+        }
     }
     c->sos_folded = true;
     if (dt) {

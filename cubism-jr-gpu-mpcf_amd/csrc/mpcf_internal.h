@@ -72,7 +72,7 @@ inline const Launchers kF64 = MPCF_LAUNCHERS(f64);
 #undef MPCF_LAUNCHERS
 
 inline const unsigned kAllFlags = MPCF_FLAG_FAST | MPCF_FLAG_SOS_CPU | MPCF_FLAG_SPLIT_XY | MPCF_FLAG_FUSED_XY |
-                           MPCF_FLAG_P64 | MPCF_FLAG_FP64;
+                           MPCF_FLAG_P64 | MPCF_FLAG_FP64 | MPCF_FLAG_NO_PIPELINE;
 
 // arithmetic build selected by the flags
 inline const Launchers* pick_launchers(const unsigned flags)
@@ -128,6 +128,10 @@ struct mpcf_ctx
     // the last stage of the previous mpcf_step (cleared by anything else that
     // writes u or runs the full reduction)
     bool sos_folded;
+    // pipelined steps: a second stream for the z sweeps of the slabs and the
+    // events that order them against the x+y sweeps (created on first use)
+    hipStream_t stream2;
+    std::vector<hipEvent_t> pipe_ev;
     // optional per-kernel event timing (mpcf_timing_enable)
     bool timing;
     std::vector<hipEvent_t> ev_pool;

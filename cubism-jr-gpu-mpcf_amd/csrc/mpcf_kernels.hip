@@ -97,6 +97,9 @@ struct StageArgs
     StepState* sos_next;           // non-null: fold the max speed of sound of the written u into
                                    // sos_next->next_sos_bits[_d] (last stage of a step)
     int sos_cpu;                   // its form: MaxSpeedOfSound_CPP (1) or the GPU kernel's (0)
+    int z0, z1;                    // the z range of interior cells this launch covers: the planes
+                                   // of the x+y sweep, the marched rows of the z sweep (a slab of
+                                   // a pipelined step; 0, nz otherwise)
     int bcmask;                    // bit f: face f reflecting; bit 8+f: face f a halo
                                    // face (ghost band imported); else absorbing
 };
@@ -274,8 +277,10 @@ __device__ __forceinline__ SosT sweep_yz_column(const Geom& g, const StageArgs& 
     const int nd = D == 1 ? g.ny : g.nz;
     const int x = col % g.nx;
     const int t = col / g.nx;
-    const int c0 = bz * chunk;
-    const int c1 = min(c0 + chunk, nd);
+    // the z sweep marches its launch's rows [z0, z1) (boundary conditions still
+    // use the whole extent nd); the y sweep the whole axis
+    const int c0 = (D == 2 ? A.z0 : 0) + bz * chunk;
+    const int c1 = min(c0 + chunk, D == 2 ? A.z1 : nd);
     const int s = g.stride(D);
     const int e0 = D == 1 ? g.idx(x, 0, t) : g.idx(x, t, 0);
     const Real dtinvh = stage_dtinvh(A);
@@ -516,12 +521,13 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
     // computed but never stored
     const int lane = threadIdx.x;
     const int rowlen = g.nx + 6;
+    const int nzr = A.z1 - A.z0;                  // planes of this launch
     const int q = (bw * 4 + threadIdx.y) * XY_TILE + lane;
     int z = q / rowlen;
     const int x = q - z * rowlen - 3;
-    const bool store = lane >= 3 && lane <= 60 && x >= 0 && x < g.nx && z < g.nz;
-    if ((bw * 4 + threadIdx.y) * XY_TILE >= rowlen * g.nz) return;   // whole wave past the end
-    z = min(z, g.nz - 1);                         // lanes past the last plane read its last row
+    const bool store = lane >= 3 && lane <= 60 && x >= 0 && x < g.nx && z < nzr;
+    if ((bw * 4 + threadIdx.y) * XY_TILE >= rowlen * nzr) return;   // whole wave past the end
+    z = A.z0 + min(z, nzr - 1);                   // lanes past the last plane read its last row
     // lanes outside [0, nx) read the cell their x-ghost copies
     Real sgx;
     const int xm = bc_index(x, g.nx, A.bcmask, 0, sgx);
@@ -910,6 +916,7 @@ __device__ __forceinline__ StageArgs case_args(const CaseDesc& C, const int cur,
     A.sos_next = nullptr; A.sos_cpu = 0;
     A.bcmask = 0;
     for (int f = 0; f < 6; ++f) A.bcmask |= (C.bc[f] == 1) << f;
+    A.z0 = 0; A.z1 = C.g.nz;
     A.u = (const Real*)C.u[cur]; A.R = (Real*)C.R; A.acc = (Real*)C.acc; A.uo = (Real*)C.u[1 - cur];
     A.fs = C.g.fstride;
     A.a = a; A.b = b; A.dtinvh = Real(0.0f); A.st = C.st;
@@ -1044,11 +1051,13 @@ void launch_sweep(const Geom& g, const int dir, const SweepParams& p, const int 
     A.a = (Real)p.a; A.b = (Real)p.b; A.dtinvh = (Real)p.dtinvh; A.st = p.st;
     A.bcmask = p.bcmask;
     A.sos_next = p.sos_next; A.sos_cpu = p.sos_cpu;
+    A.z0 = p.z0; A.z1 = p.z1 < 0 ? g.nz : p.z1;
+    const int nzr = A.z1 - A.z0;
     if (dir == 3) {
-        const int chunk = pick_chunk(g.ny, (int64_t)g.nx * g.nz, 1);
-        int dims[3];
-        xy_grid(g, chunk, dims);
-        hipLaunchKernelGGL(k_sweep_xy, dim3(dims[0], dims[2]), dim3(64, 4), 0, s, g, A, chunk);
+        const int chunk = pick_chunk(g.ny, (int64_t)g.nx * nzr, 1);
+        const int waves = ((g.nx + 6) * nzr + XY_TILE - 1) / XY_TILE;
+        const dim3 grid((waves + 3) / 4, (g.ny + chunk - 1) / chunk);
+        hipLaunchKernelGGL(k_sweep_xy, grid, dim3(64, 4), 0, s, g, A, chunk);
     } else if (dir == 0) {
         const int npen = g.ny * g.nz;
         const int chunk = pick_chunk(g.nx, npen, 1);
@@ -1059,10 +1068,8 @@ void launch_sweep(const Geom& g, const int dir, const SweepParams& p, const int 
         dim3 grid((g.nx * g.nz + 255) / 256, (g.ny + chunk - 1) / chunk);
         hipLaunchKernelGGL((k_sweep_yz<1, MODE_ACC>), grid, dim3(256), 0, s, g, A, chunk);
     } else {
-        const int chunk = pick_chunk(g.nz, (int64_t)g.nx * g.ny, 1);
-        int dims[3];
-        z_grid(g, chunk, dims);
-        dim3 grid(dims[0], dims[2]);
+        const int chunk = pick_chunk(nzr, (int64_t)g.nx * g.ny, 1);
+        const dim3 grid((g.nx * g.ny + 255) / 256, (nzr + chunk - 1) / chunk);
         if (stage) hipLaunchKernelGGL((k_sweep_yz<2, MODE_STAGE>), grid, dim3(256), 0, s, g, A, chunk);
         else       hipLaunchKernelGGL((k_sweep_yz<2, MODE_CONV>), grid, dim3(256), 0, s, g, A, chunk);
     }

@@ -22,6 +22,8 @@ struct SweepParams
     StepState* sos_next = nullptr;  // non-null: the z sweep of stage == 1 folds the max speed of
                                     // sound of the state it writes into sos_next->next_sos_bits
     int sos_cpu = 0;                // its form (MPCF_FLAG_SOS_CPU)
+    int z0 = 0, z1 = -1;            // z range of the fused x+y sweep's planes and the z sweep's
+                                    // rows (a slab of a pipelined step); z1 < 0: the whole block
     int bcmask;            // bit f: face f reflecting; bit 8+f: face f a halo face
                            // (ghost band imported); else absorbing.  The sweeps
                            // apply the boundary conditions as they load

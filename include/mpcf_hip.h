@@ -55,6 +55,10 @@ enum {
     MPCF_FLAG_P64 = 16u,     /* selective binary64 on binary32 storage: the FAST arithmetic, with the
                                 pressure recovery p = (E - KE - P)/G and the speed-of-sound / dt
                                 reduction in binary64 (implies MPCF_FLAG_FAST)                      */
+    MPCF_FLAG_NO_PIPELINE = 128u, /* run every stage of mpcf_step as whole-block sweeps.  Default, for
+                                fused x+y sweeps and NZ >= 128: the block is cut into z-slabs and the
+                                z sweep of a slab runs on a second stream while the x+y sweep of the
+                                next slab runs on the context's (same results)                    */
     MPCF_FLAG_FP64 = 32u     /* the whole scheme in binary64 (storage, arithmetic, reductions): the
                                 high-precision run the binary32 builds are measured against.  Host
                                 arrays stay binary32 for mpcf_upload/mpcf_download (rounded on the way

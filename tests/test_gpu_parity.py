@@ -285,3 +285,34 @@ def test_folded_speed_of_sound_equals_full_reduction(mpcf, flags):
         ua = a.download_f64() if flags.get("fp64") else a.data
         ub = b.download_f64() if flags.get("fp64") else b.data
         assert np.array_equal(ua, ub)
+
+
+def test_pipelined_step_bit_exact(mpcf, oracle):
+    """A block deep enough for the z-slab pipeline (NZ >= 128, fused x+y
+    sweep) steps exactly as the oracle: slab boundaries move no result."""
+    nx, ny, nz = 24, 20, 130
+    s = states.random_smooth(nx, ny, nz, seed=51)
+    bc = [1, 0, 0, 1, 1, 0]
+    ob = oracle.Block(nx, ny, nz)
+    ob.data = s
+    say("pipelined parity step", (nx, ny, nz))
+    with mpcf.Block(nx, ny, nz, 1.0 / nz, bc=bc, split_xy=False) as b:
+        b.data = s
+        for _ in range(3):
+            assert b.step(0.3) == ob.step(1.0 / nz, 0.3, 1e30, bc)[0]
+        assert_equal(b.data, ob.data, "u after pipelined steps")
+        assert_equal(b.tmp, ob.tmp, "R after pipelined steps")
+
+
+@pytest.mark.parametrize("flags", [dict(fast=True), dict(p64=True)])
+def test_pipelined_step_equals_whole_block(mpcf, flags):
+    n, nz = 48, 192
+    s = states.random_smooth(n, n, nz, seed=52)
+    say("pipelined vs whole-block", flags)
+    with mpcf.Block(n, n, nz, 1.0 / nz, **flags) as a, \
+            mpcf.Block(n, n, nz, 1.0 / nz, pipeline=False, **flags) as b:
+        a.data = s
+        b.data = s
+        for _ in range(3):
+            assert a.step(0.3) == b.step(0.3)
+        assert np.array_equal(a.data, b.data) and np.array_equal(a.tmp, b.tmp)
