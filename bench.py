@@ -69,8 +69,8 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_512.json")
 def parse():
     p = argparse.ArgumentParser()
     p.add_argument("--gpus", type=int, default=1)
-    p.add_argument("--steps", type=int, default=5)
-    p.add_argument("--warmup", type=int, default=2)
+    p.add_argument("--steps", type=int, default=20)
+    p.add_argument("--warmup", type=int, default=3)
     p.add_argument("--n", type=int, default=512)
     p.add_argument("--mode", choices=("parity", "fast"), default="fast")
     p.add_argument("--no-pipeline", action="store_true",

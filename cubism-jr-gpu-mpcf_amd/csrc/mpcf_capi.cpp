@@ -10,6 +10,7 @@
 // 703-793) and its per-stage PCIe traffic: after mpcf_upload nothing crosses
 // PCIe until mpcf_download, and dt is produced and consumed on the device.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -86,6 +87,14 @@ struct Timed
     }
 };
 
+// the fused x+y sweep wins when the arithmetic is cheap (FAST); the split
+// pair wins when it is VALU-bound (bit-exact build), unless forced
+bool split_xy(const mpcf_ctx* c)
+{
+    return (c->flags & MPCF_FLAG_SPLIT_XY) ? true : (c->flags & MPCF_FLAG_FUSED_XY) ? false
+                                                   : !(c->flags & MPCF_FLAG_FAST);
+}
+
 // R <- a R - dtinvh L(u) (stage == 0), or that and u_out <- u + b R (stage == 1)
 void run_rhs(mpcf_ctx* c, const void* u, void* uo, double a, double b, double dtinvh, const StepState* st, int stage,
              StepState* sos_next = nullptr)
@@ -95,12 +104,7 @@ void run_rhs(mpcf_ctx* c, const void* u, void* uo, double a, double b, double dt
     p.a = a; p.b = b; p.dtinvh = dtinvh; p.st = st;
     p.sos_next = sos_next; p.sos_cpu = (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0;
     p.bcmask = mpcf::detail::bc_mask(c->bc);
-    // the fused x+y sweep wins when the arithmetic is cheap (FAST); the split
-    // pair wins when it is VALU-bound (bit-exact build), unless forced
-    const bool split = (c->flags & MPCF_FLAG_SPLIT_XY) ? true
-                     : (c->flags & MPCF_FLAG_FUSED_XY) ? false
-                     : !(c->flags & MPCF_FLAG_FAST);
-    if (split) {
+    if (split_xy(c)) {
         { Timed t(c, K_SWEEP_X); c->L->sweep(c->g, 0, p, stage, c->stream); }
         { Timed t(c, K_SWEEP_Y); c->L->sweep(c->g, 1, p, stage, c->stream); }
     } else {
@@ -110,16 +114,19 @@ void run_rhs(mpcf_ctx* c, const void* u, void* uo, double a, double b, double dt
     { Timed t(c, K_SWEEP_Z); c->L->sweep(c->g, 2, p, stage, c->stream); }
 }
 
-// z-slabs of a pipelined step (0: none): fused x+y sweeps and a block deep
-// enough that a quarter of it still fills the GPU
+// z-slabs of a pipelined step (0: none): fused x+y sweeps only, the count
+// mpcf_set_pipeline asked for, else one from the block's size
 int pipeline_slabs(const mpcf_ctx* c)
 {
-    if (c->flags & MPCF_FLAG_NO_PIPELINE) return 0;
-    const bool split = (c->flags & MPCF_FLAG_SPLIT_XY) ? true
-                     : (c->flags & MPCF_FLAG_FUSED_XY) ? false
-                     : !(c->flags & MPCF_FLAG_FAST);
-    if (split || c->g.nz < 128) return 0;
-    return 4;
+    if (split_xy(c)) return 0;
+    if (c->pipe_req >= 0) return c->pipe_req;
+    // a slab needs about 64 z-planes of a 256^2-or-larger section to fill the
+    // chip on its own stream (measured on MI355X, fast build: 128^3 runs best
+    // whole, 256^3 gains 3-8% with 3-4 slabs, 384^3 20% and 512^3 14% with 8);
+    // more than 8 slabs gain nothing more at 512^3
+    const int64_t cells = (int64_t)c->g.nx * c->g.ny * c->g.nz;
+    const int S = std::min(8, c->g.nz / 64);
+    return (cells >= (int64_t(1) << 24) && S >= 2) ? S : 0;
 }
 
 // the three stages of a step over S z-slabs on two streams.  The x+y sweep of
@@ -132,10 +139,11 @@ int pipeline_slabs(const mpcf_ctx* c)
 // z sweep runs: VALU-bound and bandwidth-bound work overlap.
 int run_stages_pipelined(mpcf_ctx* c, const int S)
 {
-    if (!c->stream2) {
-        HIP_TRY(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
-        c->pipe_ev.assign(2 * 3 * S, nullptr);
-        for (hipEvent_t& e : c->pipe_ev) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    if (!c->stream2) HIP_TRY(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+    while (c->pipe_ev.size() < (size_t)(2 * 3 * S)) {
+        hipEvent_t e = nullptr;
+        HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        c->pipe_ev.push_back(e);
     }
     auto exy = [&](int s, int j) { return c->pipe_ev[(s * S + j) * 2]; };
     auto ez = [&](int s, int j) { return c->pipe_ev[(s * S + j) * 2 + 1]; };
@@ -187,6 +195,7 @@ extern "C" int mpcf_create(mpcf_ctx** out, int nx, int ny, int nz, double h, uns
 
     mpcf_ctx* c = new mpcf_ctx();
     c->g = g; c->h = h; c->flags = flags; c->cur = 0; c->sos_folded = false;
+    c->pipe_req = (flags & MPCF_FLAG_NO_PIPELINE) ? 0 : -1;
     for (int i = 0; i < 6; ++i) c->bc[i] = MPCF_BC_ABSORBING;
     c->L = pick_launchers(flags);
     c->esize = elem_bytes(flags);
@@ -241,6 +250,17 @@ extern "C" int mpcf_destroy(mpcf_ctx* c)
     hipFree(c->base);
     hipHostFree(c->st_host);
     delete c;
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_set_pipeline(mpcf_ctx* c, int slabs)
+{
+    if (!c) return fail(MPCF_EINVAL, "mpcf_set_pipeline: NULL context");
+    if (slabs < -1 || slabs == 1 || slabs > std::min(64, c->g.nz))
+        return fail(MPCF_EINVAL, "mpcf_set_pipeline: slabs must be -1 (automatic), 0 (off) or 2..min(64, NZ)");
+    if (slabs >= 2 && split_xy(c))
+        return fail(MPCF_EINVAL, "mpcf_set_pipeline: slabs need the fused x+y sweep");
+    c->pipe_req = slabs;
     return MPCF_OK;
 }
 

@@ -132,6 +132,7 @@ struct mpcf_ctx
     // events that order them against the x+y sweeps (created on first use)
     hipStream_t stream2;
     std::vector<hipEvent_t> pipe_ev;
+    int pipe_req;   // -1 automatic, 0 whole-block stages, k >= 2: k slabs (mpcf_set_pipeline)
     // optional per-kernel event timing (mpcf_timing_enable)
     bool timing;
     std::vector<hipEvent_t> ev_pool;

@@ -28,7 +28,7 @@ LSRK3_A = (0.0, -17.0 / 32, -32.0 / 27)
 LSRK3_B = (1.0 / 4, 8.0 / 9, 3.0 / 4)
 
 EXPORTS = [
-    "mpcf_last_error", "mpcf_version", "mpcf_create", "mpcf_destroy", "mpcf_set_stream",
+    "mpcf_last_error", "mpcf_version", "mpcf_create", "mpcf_destroy", "mpcf_set_pipeline", "mpcf_set_stream",
     "mpcf_get_stream", "mpcf_set_bc", "mpcf_upload", "mpcf_download", "mpcf_load_ghosts",
     "mpcf_max_sos", "mpcf_convection", "mpcf_update", "mpcf_process_all", "mpcf_step",
     "mpcf_last_dt", "mpcf_synchronize", "mpcf_query", "mpcf_test_weno", "mpcf_test_face",
@@ -107,6 +107,7 @@ def lib():
         L.mpcf_version.restype = C.c_char_p
         L.mpcf_create.argtypes = [C.POINTER(P), I, I, I, D, C.c_uint]
         L.mpcf_destroy.argtypes = [P]
+        L.mpcf_set_pipeline.argtypes = [P, I]
         L.mpcf_set_stream.argtypes = [P, P]
         L.mpcf_get_stream.argtypes = [P]
         L.mpcf_get_stream.restype = P
@@ -229,12 +230,14 @@ class Block:
         """split_xy: None = the library's default for the build, True/False = force.
         p64: binary32 storage with binary64 pressure recovery and speed-of-sound
         reduction (FAST arithmetic otherwise); fp64: the whole scheme in binary64.
-        pipeline=False: whole-block sweeps in every stage (MPCF_FLAG_NO_PIPELINE)."""
+        pipeline: True = the library's choice of z-slabs for mpcf_step, False =
+        whole-block sweeps in every stage (MPCF_FLAG_NO_PIPELINE), an int k >= 2
+        = k slabs (mpcf_set_pipeline)."""
         self.shape = (nz, ny, nx)
         self.h = float(h)
         fast = fast or p64
         flags = (FLAG_FAST if fast else 0) | (FLAG_SOS_CPU if sos_cpu else 0)
-        flags |= (FLAG_P64 if p64 else 0) | (FLAG_FP64 if fp64 else 0) | (0 if pipeline else FLAG_NO_PIPELINE)
+        flags |= (FLAG_P64 if p64 else 0) | (FLAG_FP64 if fp64 else 0) | (0 if pipeline is not False else FLAG_NO_PIPELINE)
         if split_xy is True:
             flags |= FLAG_SPLIT_XY
         elif split_xy is False:
@@ -243,6 +246,8 @@ class Block:
         h_ = C.c_void_p()
         _check(lib().mpcf_create(C.byref(h_), nx, ny, nz, float(h), flags))
         self._h = h_
+        if pipeline is not True and pipeline is not False:
+            self.set_pipeline(pipeline)
         if bc is not None:
             self.set_bc(bc)
 
@@ -259,6 +264,10 @@ class Block:
 
     def __exit__(self, *a):
         self.close()
+
+    def set_pipeline(self, slabs):
+        """-1: automatic, 0: whole-block sweeps, k >= 2: k z-slabs on two streams."""
+        _check(lib().mpcf_set_pipeline(self._h, int(slabs)))
 
     def set_bc(self, bc):
         b = np.ascontiguousarray(bc, dtype=np.int32)

@@ -55,10 +55,8 @@ enum {
     MPCF_FLAG_P64 = 16u,     /* selective binary64 on binary32 storage: the FAST arithmetic, with the
                                 pressure recovery p = (E - KE - P)/G and the speed-of-sound / dt
                                 reduction in binary64 (implies MPCF_FLAG_FAST)                      */
-    MPCF_FLAG_NO_PIPELINE = 128u, /* run every stage of mpcf_step as whole-block sweeps.  Default, for
-                                fused x+y sweeps and NZ >= 128: the block is cut into z-slabs and the
-                                z sweep of a slab runs on a second stream while the x+y sweep of the
-                                next slab runs on the context's (same results)                    */
+    MPCF_FLAG_NO_PIPELINE = 128u, /* start with mpcf_set_pipeline(ctx, 0): every stage of mpcf_step as
+                                whole-block sweeps                                                */
     MPCF_FLAG_FP64 = 32u     /* the whole scheme in binary64 (storage, arithmetic, reductions): the
                                 high-precision run the binary32 builds are measured against.  Host
                                 arrays stay binary32 for mpcf_upload/mpcf_download (rounded on the way
@@ -83,6 +81,18 @@ int mpcf_create(mpcf_ctx** out, int nx, int ny, int nz, double h, unsigned flags
 
 /* Release a context (GPUlab::~GPUlab -> GPU::dealloc, GPUlab.h:251). */
 int mpcf_destroy(mpcf_ctx* ctx);
+
+/* How mpcf_step runs its stages (same results either way).  With slabs >= 2
+ * the block is cut into that many z-slabs and the z sweep of a slab runs on
+ * a second stream of the context while the x+y sweep of the next slab runs on
+ * the context's stream (the VALU-bound and the bandwidth-bound sweep
+ * overlap); the context's stream is ordered after the step's last sweep.
+ * 0: whole-block sweeps.  -1 (default; 0 under MPCF_FLAG_NO_PIPELINE):
+ * min(8, NZ/64) slabs for blocks of 2^24 cells or more, whole-block below.
+ * Needs the fused x+y sweep (MPCF_FLAG_FAST or MPCF_FLAG_FUSED_XY) for
+ * slabs >= 2; slabs <= min(64, NZ).  No counterpart in the reference, whose
+ * stages run chunk by chunk on the host (GPUlab.cpp:703-793). */
+int mpcf_set_pipeline(mpcf_ctx* ctx, int slabs);
 
 /* Use the given hipStream_t (NULL = create and own one).  Default: own stream. */
 int mpcf_set_stream(mpcf_ctx* ctx, void* hip_stream);

@@ -287,16 +287,18 @@ def test_folded_speed_of_sound_equals_full_reduction(mpcf, flags):
         assert np.array_equal(ua, ub)
 
 
-def test_pipelined_step_bit_exact(mpcf, oracle):
-    """A block deep enough for the z-slab pipeline (NZ >= 128, fused x+y
-    sweep) steps exactly as the oracle: slab boundaries move no result."""
+@pytest.mark.parametrize("slabs", [4, 5, 33])
+def test_pipelined_step_bit_exact(mpcf, oracle, slabs):
+    """mpcf_step over z-slabs on two streams (fused x+y sweep; even, uneven
+    and 3-4-plane-thin slabs) steps exactly as the oracle: slab boundaries
+    move no result."""
     nx, ny, nz = 24, 20, 130
     s = states.random_smooth(nx, ny, nz, seed=51)
     bc = [1, 0, 0, 1, 1, 0]
     ob = oracle.Block(nx, ny, nz)
     ob.data = s
-    say("pipelined parity step", (nx, ny, nz))
-    with mpcf.Block(nx, ny, nz, 1.0 / nz, bc=bc, split_xy=False) as b:
+    say("pipelined parity step", (nx, ny, nz), slabs)
+    with mpcf.Block(nx, ny, nz, 1.0 / nz, bc=bc, split_xy=False, pipeline=slabs) as b:
         b.data = s
         for _ in range(3):
             assert b.step(0.3) == ob.step(1.0 / nz, 0.3, 1e30, bc)[0]
@@ -304,15 +306,54 @@ def test_pipelined_step_bit_exact(mpcf, oracle):
         assert_equal(b.tmp, ob.tmp, "R after pipelined steps")
 
 
-@pytest.mark.parametrize("flags", [dict(fast=True), dict(p64=True)])
-def test_pipelined_step_equals_whole_block(mpcf, flags):
+@pytest.mark.parametrize("flags,slabs", [(dict(fast=True), 3), (dict(fast=True), 8), (dict(p64=True), 4)])
+def test_pipelined_step_equals_whole_block(mpcf, flags, slabs):
     n, nz = 48, 192
     s = states.random_smooth(n, n, nz, seed=52)
-    say("pipelined vs whole-block", flags)
-    with mpcf.Block(n, n, nz, 1.0 / nz, **flags) as a, \
+    say("pipelined vs whole-block", flags, slabs)
+    with mpcf.Block(n, n, nz, 1.0 / nz, pipeline=slabs, **flags) as a, \
             mpcf.Block(n, n, nz, 1.0 / nz, pipeline=False, **flags) as b:
         a.data = s
         b.data = s
         for _ in range(3):
             assert a.step(0.3) == b.step(0.3)
         assert np.array_equal(a.data, b.data) and np.array_equal(a.tmp, b.tmp)
+
+
+def sweep_launches(mpcf, b, steps=2):
+    b.timing(True)
+    b.timing_read()
+    for _ in range(steps):
+        b.step(0.3, sync=False)
+    b.synchronize()
+    t = b.timing_read()
+    b.timing(False)
+    return t["sweep_x"][1] // (3 * steps), t["sweep_z"][1] // (3 * steps)
+
+
+def test_pipeline_slab_choice(mpcf):
+    """The automatic choice: min(8, NZ/64) slabs for blocks of 2^24 cells or
+    more with the fused x+y sweep; whole-block sweeps below that, for the
+    split sweeps and under MPCF_FLAG_NO_PIPELINE; forced counts as asked."""
+    cases = [((128, 128, 128), dict(fast=True), 1), ((256, 256, 256), dict(fast=True), 4),
+             ((512, 512, 64), dict(fast=True), 1), ((256, 256, 256), dict(), 1),
+             ((256, 256, 256), dict(fast=True, pipeline=False), 1), ((64, 64, 96), dict(fast=True, pipeline=6), 6)]
+    for (nx, ny, nz), kw, want in cases:
+        say("slab choice", (nx, ny, nz), kw)
+        with mpcf.Block(nx, ny, nz, 1.0 / nx, **kw) as b:
+            b.data = states.uniform(nx, ny, nz)
+            assert sweep_launches(mpcf, b) == (want, want), (nx, ny, nz, kw)
+
+
+def test_pipeline_arguments(mpcf):
+    with mpcf.Block(16, 16, 40, 1.0 / 16, fast=True) as b:
+        for bad in (-2, 1, 41, 65):
+            with pytest.raises(mpcf.MpcfError, match="slabs"):
+                b.set_pipeline(bad)
+        for ok in (-1, 0, 2, 40):
+            b.set_pipeline(ok)
+    with mpcf.Block(16, 16, 40, 1.0 / 16) as b:          # split x, y sweeps (bit-exact build default)
+        with pytest.raises(mpcf.MpcfError, match="fused"):
+            b.set_pipeline(4)
+        b.set_pipeline(0)
+        b.set_pipeline(-1)
