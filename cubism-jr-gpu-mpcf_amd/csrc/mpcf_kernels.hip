@@ -478,12 +478,15 @@ void k_sweep_x(const Geom g, const StageArgs A, const int chunk)
 // ---------------------------------------------------------------------------
 enum { XY_TILE = 58 };
 
-// value of lane l-1 / lane l+1; the wave's edge lanes keep their own value
+// value of lane l-1 / lane l+1.  The wave's edge lanes (no source lane) read
+// zero: they are halo positions whose results are never stored, and with no
+// old value to keep the shift is one v_mov_b32_dpp (no copy of the operand),
+// which the compiler can also fold into the consuming VALU instruction
 template <int CTRL>
 __device__ __forceinline__ float dpp_move(const float v)
 {
     const int b = __builtin_bit_cast(int, v);
-    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(b, b, CTRL, 0xf, 0xf, false));
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(b, CTRL, 0xf, 0xf, true));
 }
 template <int CTRL>
 __device__ __forceinline__ double dpp_move(const double v)
@@ -491,8 +494,8 @@ __device__ __forceinline__ double dpp_move(const double v)
     // the two 32-bit halves move with the same lane shift
     const long long b = __builtin_bit_cast(long long, v);
     const int lo = (int)(b & 0xffffffffLL), hi = (int)(b >> 32);
-    const int lo2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, 0xf, 0xf, false);
-    const int hi2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, 0xf, 0xf, false);
+    const int lo2 = __builtin_amdgcn_mov_dpp(lo, CTRL, 0xf, 0xf, true);
+    const int hi2 = __builtin_amdgcn_mov_dpp(hi, CTRL, 0xf, 0xf, true);
     return __builtin_bit_cast(double, ((long long)hi2 << 32) | (long long)(unsigned)lo2);
 }
 __device__ __forceinline__ Real from_left(const Real v) { return dpp_move<0x138>(v); }    // wave_shr:1
