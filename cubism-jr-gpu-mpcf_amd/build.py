@@ -4,7 +4,10 @@
 
 Objects:
   parity.o  mpcf_kernels.hip, MPCF_FAST=0, -ffp-contract=off  (bit-exact scheme)
-  fast.o    mpcf_kernels.hip, MPCF_FAST=1, -ffp-contract=fast (FMA-contracted)
+  fast.o    mpcf_kernels.hip, MPCF_FAST=1, -ffp-contract=on (a*b+c inside one
+            expression becomes an FMA; the same in every copy of the code, so
+            results do not depend on which unrolled or peeled step, chunk or
+            batch computes a cell)
   fastp.o   mpcf_kernels.hip, fast + MPCF_P64=1 (binary64 pressure recovery and reductions)
   f64.o     mpcf_kernels.hip, MPCF_F64=1, -ffp-contract=off (the scheme in binary64)
   reduce.o  mpcf_reduce.hip  (binary64 block comparisons and integrals)
@@ -64,8 +67,8 @@ def build(verbose: bool = False) -> str:
     cc = hipcc()
     units = [
         ("parity.o", "mpcf_kernels.hip", ["-DMPCF_FAST=0", "-ffp-contract=off"]),
-        ("fast.o", "mpcf_kernels.hip", ["-DMPCF_FAST=1", "-ffp-contract=fast"]),
-        ("fastp.o", "mpcf_kernels.hip", ["-DMPCF_FAST=1", "-DMPCF_P64=1", "-ffp-contract=fast"]),
+        ("fast.o", "mpcf_kernels.hip", ["-DMPCF_FAST=1", "-ffp-contract=on"]),
+        ("fastp.o", "mpcf_kernels.hip", ["-DMPCF_FAST=1", "-DMPCF_P64=1", "-ffp-contract=on"]),
         ("f64.o", "mpcf_kernels.hip", ["-DMPCF_FAST=0", "-DMPCF_F64=1", "-ffp-contract=off"]),
         ("reduce.o", "mpcf_reduce.hip", ["-ffp-contract=off"]),
         ("wavelet.o", "mpcf_wavelet.hip", ["-ffp-contract=off"]),

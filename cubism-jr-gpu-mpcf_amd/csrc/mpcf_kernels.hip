@@ -2,7 +2,7 @@
 //
 // Built four times (see build.py):
 //   MPCF_FAST=0, -ffp-contract=off  -> namespace mpcf::parity  (bit-exact scheme)
-//   MPCF_FAST=1, -ffp-contract=fast -> namespace mpcf::fast
+//   MPCF_FAST=1, -ffp-contract=on   -> namespace mpcf::fast
 //   MPCF_FAST=1, MPCF_P64=1         -> namespace mpcf::fastp   (fast + binary64
 //                                      pressure recovery and speed-of-sound reduction)
 //   MPCF_F64=1,  -ffp-contract=off  -> namespace mpcf::f64     (the scheme in binary64:
@@ -18,6 +18,7 @@
 // Accumulators carry the partial flux sums between sweeps in the reference's
 // summation order (((((fxp-fxm)+fyp)-fym)+fzp)-fzm) (GPUkernels.cu:1111).
 #include <hip/hip_runtime.h>
+#include <type_traits>
 #include <type_traits>
 #include "mpcf_scheme.h"
 #include "mpcf_launch.h"
@@ -82,6 +83,9 @@ constexpr bool kFast = MPCF_FAST != 0;
 // bandwidth to spare, and drops 7 accumulator fields from the z sweep's
 // reads.  The bit-exact builds keep the reference's single update.
 constexpr bool kFold = kFast;
+// the fused x+y march unrolled by five in the binary32 FAST builds (in the
+// bit-exact and binary64 builds the unrolled march spills)
+constexpr bool kUnrollXY = kFast && !MPCF_F64;
 
 enum { MODE_FIRST = 0, MODE_ACC = 1, MODE_CONV = 2, MODE_STAGE = 3 };
 
@@ -557,22 +561,25 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
     Real nxt[NVAR];
     Real nsg = load_row(A, e0, c0 + 1, g.ny, s, 2, nxt);
     const Real dtinvh = stage_dtinvh(A);
-    for (int k = c0 - 1; k <= c1; ++k) {
+    // one march step at row k.  FLUX: the face below row k is solved
+    // (k >= c0); EMIT: row k-1 is finished and written (k >= c0 + 1); PF: row
+    // k+3 is prefetched (k < c1)
+    auto step = [&](auto FLUX, auto EMIT, auto PF, const int k) {
         fix(nxt, nsg);
         const Prim q = to_prim<Real, kFast, kP64>(nxt[0], nxt[1], nxt[2], nxt[3], nxt[4], nxt[5], nxt[6]);  // row k+2
-        if (k + 3 <= c1 + 2) nsg = load_row(A, e0, k + 3, g.ny, s, 2, nxt);
+        if (PF) nsg = load_row(A, e0, k + 3, g.ny, s, 2, nxt);
         // folded builds: R of row k-1, emitted below (every lane's address is
         // an interior cell, stored or not)
         Real Ro[NVAR];
-        if (kFold && k >= c0 + 1) {
+        if (kFold && EMIT) {
 #pragma unroll
             for (int f = 0; f < NVAR; ++f) Ro[f] = A.R[f*fs + e0 + (k - 1) * s];
         }
         m.push(q);
         const CellRec ry = reconstruct<Real, kFast>(m.w0, m.w1, m.w2, m.w3, m.w4);   // row k, y-direction
-        if (k >= c0) {
+        if (FLUX) {
             const FaceFlux Fy = riemann<1, Real, kFast>(m.pm, ry.p);                // face k-1/2
-            if (k >= c0 + 1) {
+            if (EMIT) {
                 // x-direction of row k-1 (window slot w1)
                 const Prim B = prim_from_left(m.w1), D = prim_from_right(m.w1);
                 const Prim Aa = prim_from_left(B), E = prim_from_right(D);
@@ -600,10 +607,37 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
             m.Fl = Fy;
         }
         m.pm = ry.m; m.pGp = ry.p.G; m.pPp = ry.p.P;
+    };
+    if constexpr (kUnrollXY) {
+        using T_ = std::true_type;
+        using F_ = std::false_type;
+        step(F_{}, F_{}, T_{}, c0 - 1);
+        step(T_{}, F_{}, T_{}, c0);
+        int k = c0 + 1;
+        // five steps per trip, the period of the window of primitives: the
+        // window then rotates by register renaming instead of ~50 moves a step
+        for (; k + 5 <= c1; k += 5) {
+            step(T_{}, T_{}, T_{}, k);
+            step(T_{}, T_{}, T_{}, k + 1);
+            step(T_{}, T_{}, T_{}, k + 2);
+            step(T_{}, T_{}, T_{}, k + 3);
+            step(T_{}, T_{}, T_{}, k + 4);
+        }
+        for (; k < c1; ++k) step(T_{}, T_{}, T_{}, k);
+        step(T_{}, T_{}, F_{}, c1);      // c1 >= c0 + 1: every chunk has a row
+    } else {
+        for (int k = c0 - 1; k <= c1; ++k) step(k >= c0, k >= c0 + 1, k < c1, k);
     }
 }
 
-__global__ __launch_bounds__(256)
+// the unrolled FAST march could take more than 256 VGPRs (one wave per SIMD);
+// two waves per SIMD with a few spills outside the march run faster
+#if MPCF_FAST && !MPCF_F64
+#define MPCF_XY_WAVES __attribute__((amdgpu_waves_per_eu(2, 2)))
+#else
+#define MPCF_XY_WAVES
+#endif
+__global__ __launch_bounds__(256) MPCF_XY_WAVES
 void k_sweep_xy(const Geom g, const StageArgs A, const int chunk)
 {
     sweep_xy_body(g, A, chunk, blockIdx.x, blockIdx.y);
@@ -949,7 +983,7 @@ void kb_sos(const BatchDev B, const double cfl, const double dt_max)
     sos_body<CPU_FORM>(C.g, (const Real*)C.u[B.cur[c]], C.g.fstride, C.st, local, C.nb_sos, wmax, d);
 }
 
-__global__ __launch_bounds__(256)
+__global__ __launch_bounds__(256) MPCF_XY_WAVES
 void kb_sweep_xy(const BatchDev B, const Real a, const Real b)
 {
     const int c = find_case(B.off_xy, B.n, blockIdx.x);
