@@ -86,6 +86,8 @@ constexpr bool kFold = kFast;
 // the fused x+y march unrolled by five in the binary32 FAST builds (in the
 // bit-exact and binary64 builds the unrolled march spills)
 constexpr bool kUnrollXY = kFast && !MPCF_F64;
+// likewise the y / z marches (a separate switch: their register budget differs)
+constexpr bool kUnrollMarch = kFast && !MPCF_F64;
 
 enum { MODE_FIRST = 0, MODE_ACC = 1, MODE_CONV = 2, MODE_STAGE = 3 };
 
@@ -242,14 +244,17 @@ struct March
 
     __device__ __forceinline__ void push(const Prim& q) { w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = q; }
 
-    // reconstruct cell k, solve face k-1/2, emit cell k-1 (operands in `in`)
-    __device__ __forceinline__ void step(const StageArgs& A, const Real dtinvh, const int k, const int c0, const int e_km1,
-                                         const EmitIn<MODE>& in, const bool store = true)
+    // reconstruct cell k, solve face k-1/2 (FLUX: k >= c0), emit cell k-1
+    // (EMIT: k >= c0 + 1; operands in `in`).  FLUX and EMIT are bools or
+    // std::bool_constant (the peeled steps of an unrolled march)
+    template <class FLUX, class EMIT>
+    __device__ __forceinline__ void step(const StageArgs& A, const Real dtinvh, const FLUX flux, const EMIT em,
+                                         const int e_km1, const EmitIn<MODE>& in)
     {
         const CellRec r = reconstruct<Real, kFast>(w0, w1, w2, w3, w4);
-        if (k >= c0) {
+        if (flux) {
             const FaceFlux F = riemann<D, Real, kFast>(pm, r.p);
-            if (k >= c0 + 1 && store) emit<MODE>(A, dtinvh, e_km1, in, F, Fl, pGp + pm.G, pPp + pm.P, smax);
+            if (em) emit<MODE>(A, dtinvh, e_km1, in, F, Fl, pGp + pm.G, pPp + pm.P, smax);
             Fl = F;
         }
         pm = r.m; pGp = r.p.G; pPp = r.p.P;
@@ -310,21 +315,43 @@ __device__ __forceinline__ SosT sweep_yz_column(const Geom& g, const StageArgs& 
     Real nxt[NVAR];
     Real nsg = load_row(A, e0, c0 + 1, nd, s, FLO, nxt);
     EmitIn<MODE> ein, ein_next;
-    for (int k = c0 - 1; k <= c1; ++k) {
+    // one march step at row k.  FLUX: face k-1/2 is solved (k >= c0); EMIT:
+    // cell k-1 is written (k >= c0 + 1); PF: row k+3 is prefetched and the
+    // emit operands of cell k are loaded (k < c1; the latter from k >= c0)
+    auto step = [&](auto FLUX, auto EMIT, auto PF, const int k) {
         nxt[NEGQ] *= nsg;   // boundary sign of the prefetched row
         // rotate the raw ring: r0 = cell k-1, r3 = cell k+2
 #pragma unroll
         for (int q = 0; q < NVAR; ++q) { r0[q] = r1[q]; r1[q] = r2[q]; r2[q] = r3[q]; r3[q] = nxt[q]; }
         const Prim q = to_prim<Real, kFast, kP64>(nxt[0], nxt[1], nxt[2], nxt[3], nxt[4], nxt[5], nxt[6]);  // cell k+2
-        if (k + 3 <= c1 + 2) nsg = load_row(A, e0, k + 3, nd, s, FLO, nxt);
-        if (k >= c0 && k < c1) emit_load<MODE>(A, e0 + k * s, ein_next);   // emitted at step k+1
+        if (PF) nsg = load_row(A, e0, k + 3, nd, s, FLO, nxt);
+        if (PF && FLUX) emit_load<MODE>(A, e0 + k * s, ein_next);   // emitted at step k+1
         if (kRaw) {
 #pragma unroll
             for (int f = 0; f < NVAR; ++f) ein.u[f] = r0[f];                 // cell k-1
         }
         m.push(q);
-        m.step(A, dtinvh, k, c0, e0 + (k - 1) * s, ein);
+        m.step(A, dtinvh, FLUX, EMIT, e0 + (k - 1) * s, ein);
         ein = ein_next;
+    };
+    if constexpr (kUnrollMarch) {
+        using T_ = std::true_type;
+        using F_ = std::false_type;
+        step(F_{}, F_{}, T_{}, c0 - 1);
+        step(T_{}, F_{}, T_{}, c0);
+        int k = c0 + 1;
+        // five steps per trip, the period of the window of primitives
+        for (; k + 5 <= c1; k += 5) {
+            step(T_{}, T_{}, T_{}, k);
+            step(T_{}, T_{}, T_{}, k + 1);
+            step(T_{}, T_{}, T_{}, k + 2);
+            step(T_{}, T_{}, T_{}, k + 3);
+            step(T_{}, T_{}, T_{}, k + 4);
+        }
+        for (; k < c1; ++k) step(T_{}, T_{}, T_{}, k);
+        step(T_{}, T_{}, F_{}, c1);      // c1 >= c0 + 1: every chunk has a row
+    } else {
+        for (int k = c0 - 1; k <= c1; ++k) step(k >= c0, k >= c0 + 1, k < c1, k);
     }
     return m.smax;
 }
