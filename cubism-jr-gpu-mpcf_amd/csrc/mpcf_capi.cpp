@@ -114,12 +114,15 @@ void run_rhs(mpcf_ctx* c, const void* u, void* uo, double a, double b, double dt
     { Timed t(c, K_SWEEP_Z); c->L->sweep(c->g, 2, p, stage, c->stream); }
 }
 
-// z-slabs of a pipelined step (0: none): fused x+y sweeps only, the count
-// mpcf_set_pipeline asked for, else one from the block's size
+// z-slabs of a pipelined step (0: none): the count mpcf_set_pipeline asked
+// for, else one from the block's size
 int pipeline_slabs(const mpcf_ctx* c)
 {
-    if (split_xy(c)) return 0;
     if (c->pipe_req >= 0) return c->pipe_req;
+    // the split sweeps (bit-exact build) are all VALU-bound: slabs give them
+    // nothing to overlap (measured on MI355X at 512^3: whole-block 59.4 ms a
+    // step, 4 slabs 60.1, 8 slabs 61.4)
+    if (split_xy(c)) return 0;
     // a slab needs about 64 z-planes of a 256^2-or-larger section to fill the
     // chip on its own stream (measured on MI355X, fast build: 128^3 runs best
     // whole, 256^3 gains 3-8% with 3-4 slabs, 384^3 20% and 512^3 14% with 8);
@@ -129,8 +132,8 @@ int pipeline_slabs(const mpcf_ctx* c)
     return (cells >= (int64_t(1) << 24) && S >= 2) ? S : 0;
 }
 
-// the three stages of a step over S z-slabs on two streams.  The x+y sweep of
-// slab j needs the previous stage's z sweep of slab j only (its stencil is in
+// the three stages of a step over S z-slabs on two streams.  The x+y sweep
+// (the fused one, or the x and the y sweep in turn) of slab j needs the previous stage's z sweep of slab j only (its stencil is in
 // x and y; it also rewrites that slab's R and accumulators), so it waits for
 // that event; the z sweep of slab j waits for the x+y sweep of slab j and,
 // by the order of the second stream, for every z sweep of the previous
@@ -145,6 +148,7 @@ int run_stages_pipelined(mpcf_ctx* c, const int S)
         HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
         c->pipe_ev.push_back(e);
     }
+    const bool split = split_xy(c);   // x and y as two sweeps per slab
     auto exy = [&](int s, int j) { return c->pipe_ev[(s * S + j) * 2]; };
     auto ez = [&](int s, int j) { return c->pipe_ev[(s * S + j) * 2 + 1]; };
     for (int s = 0; s < 3; ++s) {
@@ -157,7 +161,13 @@ int run_stages_pipelined(mpcf_ctx* c, const int S)
             p.z0 = (int)((int64_t)c->g.nz * j / S);
             p.z1 = (int)((int64_t)c->g.nz * (j + 1) / S);
             if (s > 0) HIP_TRY(hipStreamWaitEvent(c->stream, ez(s - 1, j), 0));
-            { Timed t(c, K_SWEEP_X); c->L->sweep(c->g, 3, p, 1, c->stream); }
+            if (split) {
+                { Timed t(c, K_SWEEP_X); c->L->sweep(c->g, 0, p, 1, c->stream); }
+                { Timed t(c, K_SWEEP_Y); c->L->sweep(c->g, 1, p, 1, c->stream); }
+            } else {
+                Timed t(c, K_SWEEP_X);
+                c->L->sweep(c->g, 3, p, 1, c->stream);
+            }
             HIP_TRY(hipEventRecord(exy(s, j), c->stream));
             HIP_TRY(hipStreamWaitEvent(c->stream2, exy(s, j), 0));
             { Timed t(c, K_SWEEP_Z, c->stream2); c->L->sweep(c->g, 2, p, 1, c->stream2); }
@@ -258,8 +268,6 @@ extern "C" int mpcf_set_pipeline(mpcf_ctx* c, int slabs)
     if (!c) return fail(MPCF_EINVAL, "mpcf_set_pipeline: NULL context");
     if (slabs < -1 || slabs == 1 || slabs > std::min(64, c->g.nz))
         return fail(MPCF_EINVAL, "mpcf_set_pipeline: slabs must be -1 (automatic), 0 (off) or 2..min(64, NZ)");
-    if (slabs >= 2 && split_xy(c))
-        return fail(MPCF_EINVAL, "mpcf_set_pipeline: slabs need the fused x+y sweep");
     c->pipe_req = slabs;
     return MPCF_OK;
 }

@@ -104,8 +104,8 @@ struct StageArgs
                                    // sos_next->next_sos_bits[_d] (last stage of a step)
     int sos_cpu;                   // its form: MaxSpeedOfSound_CPP (1) or the GPU kernel's (0)
     int z0, z1;                    // the z range of interior cells this launch covers: the planes
-                                   // of the x+y sweep, the marched rows of the z sweep (a slab of
-                                   // a pipelined step; 0, nz otherwise)
+                                   // of the x, y and x+y sweeps, the marched rows of the z sweep
+                                   // (a slab of a pipelined step; 0, nz otherwise)
     int bcmask;                    // bit f: face f reflecting; bit 8+f: face f a halo
                                    // face (ghost band imported); else absorbing
 };
@@ -282,12 +282,12 @@ template <int D, int MODE>
 __device__ __forceinline__ SosT sweep_yz_column(const Geom& g, const StageArgs& A, const int chunk, const int col,
                                                 const int bz)
 {
-    const int nt = D == 1 ? g.nz : g.ny;
     const int nd = D == 1 ? g.ny : g.nz;
     const int x = col % g.nx;
-    const int t = col / g.nx;
-    // the z sweep marches its launch's rows [z0, z1) (boundary conditions still
-    // use the whole extent nd); the y sweep the whole axis
+    // the y sweep's columns lie in the launch's planes [z0, z1); the z sweep
+    // marches its launch's rows [z0, z1) (boundary conditions still use the
+    // whole extent nd)
+    const int t = (D == 1 ? A.z0 : 0) + col / g.nx;
     const int c0 = (D == 2 ? A.z0 : 0) + bz * chunk;
     const int c1 = min(c0 + chunk, D == 2 ? A.z1 : nd);
     const int s = g.stride(D);
@@ -362,7 +362,7 @@ __device__ __forceinline__ void sweep_yz_body(const Geom& g, const StageArgs& A,
 {
     // columns (x, t) flattened x-fastest, 256 per block: consecutive lanes are
     // consecutive x (coalesced) and no lane idles when nx is not a multiple of 64
-    const int nt = D == 1 ? g.nz : g.ny;
+    const int nt = D == 1 ? A.z1 - A.z0 : g.ny;
     const int col = bcol * 256 + threadIdx.x;
     SosT smax = 0;
     if (col < g.nx * nt) smax = sweep_yz_column<D, MODE>(g, A, chunk, col, bz);   // no barriers in the body
@@ -417,7 +417,7 @@ void k_sweep_x(const Geom g, const StageArgs A, const int chunk)
 {
     __shared__ XStage S;
     const int lane = threadIdx.x;
-    const int npen = g.ny * g.nz;
+    const int npen = g.ny * (A.z1 - A.z0);     // pencils (y, z) of the launch's planes [z0, z1)
     const int pb = blockIdx.x * 64;
     const int c0 = blockIdx.y * chunk;
     const int c1 = min(c0 + chunk, g.nx);
@@ -432,7 +432,7 @@ void k_sweep_x(const Geom g, const StageArgs A, const int chunk)
         const int p = pb + i * 8 + lane / SX;
         sok[i] = p < npen;
         const int pc = sok[i] ? p : npen - 1;
-        sbase[i] = g.idx(0, pc % g.ny, pc / g.ny);
+        sbase[i] = g.idx(0, pc % g.ny, A.z0 + pc / g.ny);
     }
     const bool active = pb + lane < npen;
 
@@ -1123,13 +1123,13 @@ void launch_sweep(const Geom& g, const int dir, const SweepParams& p, const int 
         const dim3 grid((waves + 3) / 4, (g.ny + chunk - 1) / chunk);
         hipLaunchKernelGGL(k_sweep_xy, grid, dim3(64, 4), 0, s, g, A, chunk);
     } else if (dir == 0) {
-        const int npen = g.ny * g.nz;
+        const int npen = g.ny * nzr;
         const int chunk = pick_chunk(g.nx, npen, 1);
         dim3 grid((npen + 63) / 64, (g.nx + chunk - 1) / chunk);
         hipLaunchKernelGGL(k_sweep_x, grid, dim3(64), 0, s, g, A, chunk);
     } else if (dir == 1) {
-        const int chunk = pick_chunk(g.ny, (int64_t)g.nx * g.nz, 1);
-        dim3 grid((g.nx * g.nz + 255) / 256, (g.ny + chunk - 1) / chunk);
+        const int chunk = pick_chunk(g.ny, (int64_t)g.nx * nzr, 1);
+        dim3 grid((g.nx * nzr + 255) / 256, (g.ny + chunk - 1) / chunk);
         hipLaunchKernelGGL((k_sweep_yz<1, MODE_ACC>), grid, dim3(256), 0, s, g, A, chunk);
     } else {
         const int chunk = pick_chunk(nzr, (int64_t)g.nx * g.ny, 1);

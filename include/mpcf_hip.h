@@ -84,14 +84,13 @@ int mpcf_destroy(mpcf_ctx* ctx);
 
 /* How mpcf_step runs its stages (same results either way).  With slabs >= 2
  * the block is cut into that many z-slabs and the z sweep of a slab runs on
- * a second stream of the context while the x+y sweep of the next slab runs on
- * the context's stream (the VALU-bound and the bandwidth-bound sweep
- * overlap); the context's stream is ordered after the step's last sweep.
- * 0: whole-block sweeps.  -1 (default; 0 under MPCF_FLAG_NO_PIPELINE):
- * min(8, NZ/64) slabs for blocks of 2^24 cells or more, whole-block below.
- * Needs the fused x+y sweep (MPCF_FLAG_FAST or MPCF_FLAG_FUSED_XY) for
- * slabs >= 2; slabs <= min(64, NZ).  No counterpart in the reference, whose
- * stages run chunk by chunk on the host (GPUlab.cpp:703-793). */
+ * a second stream of the context while the x and y sweeps (fused or split) of
+ * the next slab run on the context's stream; the context's stream is ordered
+ * after the step's last sweep.  0: whole-block sweeps.  -1 (default; 0 under
+ * MPCF_FLAG_NO_PIPELINE): with the fused x+y sweep, min(8, NZ/64) slabs for
+ * blocks of 2^24 cells or more, whole-block below; with the split sweeps,
+ * whole-block.  slabs <= min(64, NZ).  No counterpart in the
+ * reference, whose stages run chunk by chunk on the host (GPUlab.cpp:703-793). */
 int mpcf_set_pipeline(mpcf_ctx* ctx, int slabs);
 
 /* Use the given hipStream_t (NULL = create and own one).  Default: own stream. */

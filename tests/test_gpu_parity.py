@@ -287,18 +287,18 @@ def test_folded_speed_of_sound_equals_full_reduction(mpcf, flags):
         assert np.array_equal(ua, ub)
 
 
-@pytest.mark.parametrize("slabs", [4, 5, 33])
-def test_pipelined_step_bit_exact(mpcf, oracle, slabs):
-    """mpcf_step over z-slabs on two streams (fused x+y sweep; even, uneven
-    and 3-4-plane-thin slabs) steps exactly as the oracle: slab boundaries
-    move no result."""
+@pytest.mark.parametrize("slabs,split", [(4, False), (5, False), (33, False), (4, True), (7, True)])
+def test_pipelined_step_bit_exact(mpcf, oracle, slabs, split):
+    """mpcf_step over z-slabs on two streams (fused or split x and y sweeps;
+    even, uneven and 3-4-plane-thin slabs) steps exactly as the oracle: slab
+    boundaries move no result."""
     nx, ny, nz = 24, 20, 130
     s = states.random_smooth(nx, ny, nz, seed=51)
     bc = [1, 0, 0, 1, 1, 0]
     ob = oracle.Block(nx, ny, nz)
     ob.data = s
-    say("pipelined parity step", (nx, ny, nz), slabs)
-    with mpcf.Block(nx, ny, nz, 1.0 / nz, bc=bc, split_xy=False, pipeline=slabs) as b:
+    say("pipelined parity step", (nx, ny, nz), slabs, "split" if split else "fused")
+    with mpcf.Block(nx, ny, nz, 1.0 / nz, bc=bc, split_xy=split, pipeline=slabs) as b:
         b.data = s
         for _ in range(3):
             assert b.step(0.3) == ob.step(1.0 / nz, 0.3, 1e30, bc)[0]
@@ -334,9 +334,11 @@ def sweep_launches(mpcf, b, steps=2):
 def test_pipeline_slab_choice(mpcf):
     """The automatic choice: min(8, NZ/64) slabs for blocks of 2^24 cells or
     more with the fused x+y sweep; whole-block sweeps below that, for the
-    split sweeps and under MPCF_FLAG_NO_PIPELINE; forced counts as asked."""
+    split sweeps and under MPCF_FLAG_NO_PIPELINE; forced counts as asked,
+    split sweeps included."""
     cases = [((128, 128, 128), dict(fast=True), 1), ((256, 256, 256), dict(fast=True), 4),
              ((512, 512, 64), dict(fast=True), 1), ((256, 256, 256), dict(), 1),
+             ((64, 64, 96), dict(pipeline=3), 3),
              ((256, 256, 256), dict(fast=True, pipeline=False), 1), ((64, 64, 96), dict(fast=True, pipeline=6), 6)]
     for (nx, ny, nz), kw, want in cases:
         say("slab choice", (nx, ny, nz), kw)
@@ -353,7 +355,5 @@ def test_pipeline_arguments(mpcf):
         for ok in (-1, 0, 2, 40):
             b.set_pipeline(ok)
     with mpcf.Block(16, 16, 40, 1.0 / 16) as b:          # split x, y sweeps (bit-exact build default)
-        with pytest.raises(mpcf.MpcfError, match="fused"):
-            b.set_pipeline(4)
-        b.set_pipeline(0)
-        b.set_pipeline(-1)
+        for ok in (4, 0, -1):
+            b.set_pipeline(ok)
