@@ -265,6 +265,11 @@ class Block:
     def __exit__(self, *a):
         self.close()
 
+    def set_stream(self, stream):
+        """Run on a caller's hipStream_t (its address as an int); None: the
+        block's own stream again."""
+        _check(lib().mpcf_set_stream(self._h, C.c_void_p(stream) if stream else None))
+
     def set_pipeline(self, slabs):
         """-1: automatic, 0: whole-block sweeps, k >= 2: k z-slabs on two streams."""
         _check(lib().mpcf_set_pipeline(self._h, int(slabs)))

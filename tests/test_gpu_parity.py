@@ -357,3 +357,38 @@ def test_pipeline_arguments(mpcf):
     with mpcf.Block(16, 16, 40, 1.0 / 16) as b:          # split x, y sweeps (bit-exact build default)
         for ok in (4, 0, -1):
             b.set_pipeline(ok)
+
+
+def test_pipeline_on_caller_stream(mpcf):
+    """A pipelined step on a stream the caller made: the caller's stream is
+    ordered after the step's last sweep (synchronizing only that stream
+    finishes the step), and the results equal whole-block stepping."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipStreamCreate.argtypes = [ctypes.POINTER(ctypes.c_void_p)]
+    hip.hipStreamSynchronize.argtypes = [ctypes.c_void_p]
+    hip.hipStreamDestroy.argtypes = [ctypes.c_void_p]
+    n, nz = 40, 96
+    s = states.random_smooth(n, n, nz, seed=53)
+    st = ctypes.c_void_p()
+    assert hip.hipStreamCreate(ctypes.byref(st)) == 0
+    say("pipelined step on a caller stream")
+    try:
+        with mpcf.Block(n, n, nz, 1.0 / nz, fast=True, pipeline=4) as a, \
+                mpcf.Block(n, n, nz, 1.0 / nz, fast=True, pipeline=False) as b:
+            a.set_stream(st.value)
+            assert a.stream == st.value
+            a.data = s
+            b.data = s
+            for _ in range(4):
+                a.step(0.3, sync=False)
+                b.step(0.3)
+            assert hip.hipStreamSynchronize(st) == 0
+            assert np.array_equal(a.data, b.data) and np.array_equal(a.tmp, b.tmp)
+            a.set_stream(None)
+            assert a.stream != st.value
+            a.step(0.3)
+            b.step(0.3)
+            assert np.array_equal(a.data, b.data)
+    finally:
+        hip.hipStreamDestroy(st)
