@@ -83,6 +83,9 @@ constexpr bool kFast = MPCF_FAST != 0;
 // bandwidth to spare, and drops 7 accumulator fields from the z sweep's
 // reads.  The bit-exact builds keep the reference's single update.
 constexpr bool kFold = kFast;
+// accelerator complex dies of an MI355X; workgroups are dispatched to them in
+// turn by linear workgroup id
+constexpr int kXCDs = 8;
 // the fused x+y march unrolled by five in the binary32 FAST builds (in the
 // bit-exact and binary64 builds the unrolled march spills)
 constexpr bool kUnrollXY = kFast && !MPCF_F64;
@@ -667,7 +670,16 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
 __global__ __launch_bounds__(256) MPCF_XY_WAVES
 void k_sweep_xy(const Geom g, const StageArgs A, const int chunk)
 {
-    sweep_xy_body(g, A, chunk, blockIdx.x, blockIdx.y);
+    // workgroups go to the 8 XCDs round-robin by linear id; give each XCD a
+    // contiguous run of tiles instead, so that neighbouring tiles (which
+    // overlap by six positions and share partial cache lines) meet in the
+    // same L2
+    const int gx = gridDim.x, n = gx * gridDim.y;
+    const int lin = blockIdx.x + blockIdx.y * gx;
+    const int xcd = lin % kXCDs, k = lin / kXCDs;
+    const int per = n / kXCDs, extra = n % kXCDs;
+    const int t = xcd * per + min(xcd, extra) + k;
+    sweep_xy_body(g, A, chunk, t % gx, t / gx);
 }
 
 // ---------------------------------------------------------------------------
