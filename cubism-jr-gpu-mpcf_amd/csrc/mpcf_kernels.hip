@@ -549,6 +549,18 @@ __device__ __forceinline__ Prim prim_from_right(const Prim& q)
     return o;
 }
 
+// workgroups go to the XCDs round-robin by linear id; this maps linear id
+// `lin` of a grid of n to a tile index such that each XCD takes a contiguous
+// run of tiles, so that neighbouring tiles of the fused x+y sweep (which
+// overlap by six positions and share partial cache lines) meet in one L2.
+// For grids of equal-cost tiles (one block)
+__device__ __forceinline__ int xcd_order(const int lin, const int n)
+{
+    const int xcd = lin % kXCDs, k = lin / kXCDs;
+    const int per = n / kXCDs, extra = n % kXCDs;
+    return xcd * per + min(xcd, extra) + k;
+}
+
 __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A, const int chunk,
                                               const int bw, const int bz)
 {
@@ -670,15 +682,8 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
 __global__ __launch_bounds__(256) MPCF_XY_WAVES
 void k_sweep_xy(const Geom g, const StageArgs A, const int chunk)
 {
-    // workgroups go to the 8 XCDs round-robin by linear id; give each XCD a
-    // contiguous run of tiles instead, so that neighbouring tiles (which
-    // overlap by six positions and share partial cache lines) meet in the
-    // same L2
-    const int gx = gridDim.x, n = gx * gridDim.y;
-    const int lin = blockIdx.x + blockIdx.y * gx;
-    const int xcd = lin % kXCDs, k = lin / kXCDs;
-    const int per = n / kXCDs, extra = n % kXCDs;
-    const int t = xcd * per + min(xcd, extra) + k;
+    const int gx = gridDim.x;
+    const int t = xcd_order(blockIdx.x + blockIdx.y * gx, gx * gridDim.y);
     sweep_xy_body(g, A, chunk, t % gx, t / gx);
 }
 
@@ -1025,6 +1030,9 @@ void kb_sos(const BatchDev B, const double cfl, const double dt_max)
 __global__ __launch_bounds__(256) MPCF_XY_WAVES
 void kb_sweep_xy(const BatchDev B, const Real a, const Real b)
 {
+    // round-robin over the XCDs as dispatched: the tiles of different cases
+    // differ in cost, and an even mix per XCD balances them (contiguous runs
+    // per XCD, as in k_sweep_xy, were slower here)
     const int c = find_case(B.off_xy, B.n, blockIdx.x);
     const CaseDesc& C = B.cases[c];
     const int local = blockIdx.x - B.off_xy[c];
