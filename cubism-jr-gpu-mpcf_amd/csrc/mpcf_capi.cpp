@@ -691,12 +691,17 @@ struct mpcf_batch
     // device tables
     mpcf::CaseDesc* d_cases = nullptr;
     int* d_cur = nullptr;
-    int* d_off = nullptr;                    // 3 x (cap + 1)
+    int* d_off = nullptr;                    // 2 groups x 3 x (cap + 1)
     int cap = 0;
     bool dirty = true;
     bool cur_on_device = false;              // d_cur holds the truth (after steps)
-    int tot_xy = 0, tot_z = 0, tot_sos = 0;
-    mpcf::BatchDev dev{};
+    // the cases in two groups of about equal cells, stepped on two streams
+    // (group 1 on stream2, forked from and joined back to stream), so that
+    // one group's x+y sweep runs beside the other's z sweep
+    int tot_xy[2] = {}, tot_z[2] = {}, tot_sos[2] = {};
+    mpcf::BatchDev dev[2]{};
+    hipStream_t stream2 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     // graph of one step
     bool use_graph = false;
     bool graph_valid = false;
@@ -756,28 +761,43 @@ int batch_rebuild(mpcf_batch* b)
         const int cap = std::max(16, 2 * n);
         hipError_t e = hipMalloc((void**)&b->d_cases, cap * sizeof(mpcf::CaseDesc));
         if (e == hipSuccess) e = hipMalloc((void**)&b->d_cur, cap * sizeof(int));
-        if (e == hipSuccess) e = hipMalloc((void**)&b->d_off, 3 * (cap + 1) * sizeof(int));
+        if (e == hipSuccess) e = hipMalloc((void**)&b->d_off, 2 * 3 * (cap + 1) * sizeof(int));
         if (e != hipSuccess) return fail(MPCF_ENOMEM, std::string("batch tables: ") + hipGetErrorString(e));
         b->cap = cap;
     }
     std::vector<mpcf::CaseDesc> desc(n);
-    std::vector<int> cur(n), off(3 * (b->cap + 1), 0);
-    int* oxy = &off[0];
-    int* oz = &off[b->cap + 1];
-    int* os = &off[2 * (b->cap + 1)];
-    long long cols_xy = 0, cols_z = 0;
+    const int tab = 3 * (b->cap + 1);
+    std::vector<int> cur(n), off(2 * tab, 0);
+    long long cols_xy = 0, cols_z = 0, cells = 0;
     for (mpcf_ctx* c : b->cases) {
         cols_xy += (long long)(c->g.nx + 6) * c->g.nz;
         cols_z += (long long)c->g.nx * c->g.ny;
+        cells += (long long)c->g.nx * c->g.ny * c->g.nz;
+    }
+    // group 0: cases [0, split), group 1: [split, n), split where the cells
+    // reach half (one group when there is one case)
+    int split = n;
+    if (n >= 2) {
+        long long acc = 0;
+        split = 1;
+        for (int i = 0; i < n - 1; ++i) {
+            acc += (long long)b->cases[i]->g.nx * b->cases[i]->g.ny * b->cases[i]->g.nz;
+            split = i + 1;
+            if (2 * acc >= cells) break;
+        }
     }
     for (int i = 0; i < n; ++i) {
         mpcf_ctx* c = b->cases[i];
+        const int grp = i < split ? 0 : 1, j = grp ? i - split : i;   // index within the group
+        int* oxy = &off[grp * tab];
+        int* oz = oxy + (b->cap + 1);
+        int* os = oxy + 2 * (b->cap + 1);
         int blocks[3];
         mpcf::detail::case_desc(c, c->cur, march_len(b, i, c->g.ny, 0, cols_xy), march_len(b, i, c->g.nz, 1, cols_z),
                                 desc[i], blocks);
-        oxy[i + 1] = oxy[i] + blocks[0];
-        oz[i + 1] = oz[i] + blocks[1];
-        os[i + 1] = os[i] + blocks[2];
+        oxy[j + 1] = oxy[j] + blocks[0];
+        oz[j + 1] = oz[j] + blocks[1];
+        os[j + 1] = os[j] + blocks[2];
         cur[i] = c->cur;
     }
     if (n > 0) {
@@ -786,24 +806,48 @@ int batch_rebuild(mpcf_batch* b)
         HIP_TRY(hipMemcpyAsync(b->d_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
         HIP_TRY(hipStreamSynchronize(b->stream));
     }
-    b->tot_xy = oxy[n]; b->tot_z = oz[n]; b->tot_sos = os[n];
-    b->dev.cases = b->d_cases; b->dev.cur = b->d_cur; b->dev.n = n;
-    b->dev.off_xy = b->d_off; b->dev.off_z = b->d_off + (b->cap + 1); b->dev.off_sos = b->d_off + 2 * (b->cap + 1);
+    for (int grp = 0; grp < 2; ++grp) {
+        const int first = grp ? split : 0, count = grp ? n - split : split;
+        const int* ho = &off[grp * tab];
+        b->tot_xy[grp] = ho[count]; b->tot_z[grp] = ho[(b->cap + 1) + count]; b->tot_sos[grp] = ho[2 * (b->cap + 1) + count];
+        mpcf::BatchDev& d = b->dev[grp];
+        d.cases = b->d_cases + first; d.cur = b->d_cur + first; d.n = count;
+        d.off_xy = b->d_off + grp * tab; d.off_z = d.off_xy + (b->cap + 1); d.off_sos = d.off_xy + 2 * (b->cap + 1);
+    }
     b->dirty = false;
     batch_drop_graph(b);
     return MPCF_OK;
 }
 
-void batch_enqueue_step(mpcf_batch* b, double cfl, double dt_max)
+int batch_enqueue_step(mpcf_batch* b, double cfl, double dt_max)
 {
     // the sweeps apply each case's boundary conditions as they load: a step is
-    // seven launches
+    // seven launches per group; group 1 runs on the second stream
     const int cpu_form = (b->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0;
-    b->L->b_sos(b->dev, b->tot_sos, cpu_form, cfl, dt_max, b->stream);
-    for (int s = 0; s < 3; ++s) {
-        b->L->b_xy(b->dev, b->tot_xy, kA[s], kB[s], b->stream);
-        b->L->b_z(b->dev, b->tot_z, kA[s], kB[s], s == 2, cpu_form, b->stream);
+    const bool two = b->dev[1].n > 0;
+    if (two) {
+        if (!b->stream2) {
+            HIP_TRY(hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking));
+            HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+            HIP_TRY(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
+        }
+        HIP_TRY(hipEventRecord(b->fork, b->stream));
+        HIP_TRY(hipStreamWaitEvent(b->stream2, b->fork, 0));
     }
+    for (int grp = two ? 1 : 0; grp >= 0; --grp) {
+        hipStream_t st = grp ? b->stream2 : b->stream;
+        const mpcf::BatchDev& d = b->dev[grp];
+        b->L->b_sos(d, b->tot_sos[grp], cpu_form, cfl, dt_max, st);
+        for (int s = 0; s < 3; ++s) {
+            b->L->b_xy(d, b->tot_xy[grp], kA[s], kB[s], st);
+            b->L->b_z(d, b->tot_z[grp], kA[s], kB[s], s == 2, cpu_form, st);
+        }
+    }
+    if (two) {
+        HIP_TRY(hipEventRecord(b->join, b->stream2));
+        HIP_TRY(hipStreamWaitEvent(b->stream, b->join, 0));
+    }
+    return MPCF_OK;
 }
 
 int find_case_index(const mpcf_batch* b, int id)
@@ -850,6 +894,12 @@ extern "C" int mpcf_batch_destroy(mpcf_batch* b)
     if (b->d_cases) hipFree(b->d_cases);
     if (b->d_cur) hipFree(b->d_cur);
     if (b->d_off) hipFree(b->d_off);
+    if (b->stream2) {
+        hipStreamSynchronize(b->stream2);
+        hipStreamDestroy(b->stream2);
+        hipEventDestroy(b->fork);
+        hipEventDestroy(b->join);
+    }
     hipStreamDestroy(b->stream);
     delete b;
     return MPCF_OK;
@@ -942,9 +992,10 @@ extern "C" int mpcf_batch_step(mpcf_batch* b, double cfl, double dt_max)
         if (!b->graph_valid || b->g_cfl != cfl || b->g_dtmax != dt_max) {
             batch_drop_graph(b);
             HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-            batch_enqueue_step(b, cfl, dt_max);
+            const int rc = batch_enqueue_step(b, cfl, dt_max);
             hipGraph_t g = nullptr;
             HIP_TRY(hipStreamEndCapture(b->stream, &g));
+            if (rc != MPCF_OK) { if (g) hipGraphDestroy(g); return rc; }
             b->graph = g;
             HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
             b->graph_valid = true;
@@ -954,7 +1005,8 @@ extern "C" int mpcf_batch_step(mpcf_batch* b, double cfl, double dt_max)
         }
         HIP_TRY(hipGraphLaunch(b->exec, b->stream));
     } else {
-        batch_enqueue_step(b, cfl, dt_max);
+        const int rc = batch_enqueue_step(b, cfl, dt_max);
+        if (rc != MPCF_OK) return rc;
         LAUNCH_CHECK("batch step");
     }
     b->cur_on_device = true;

@@ -175,8 +175,13 @@ struct mpcf_server
     int slot = 0;
     std::vector<Req*> active;
     std::vector<long long> active_seq;
-    mpcf::BatchDev dev{};
-    int tot[3] = {0, 0, 0};   // blocks of the x+y sweep, z sweep, speed of sound
+    // the live cases in two groups of about equal cells, stepped on two
+    // streams (group 1 on stream2, forked from and joined back to stream), so
+    // that one group's x+y sweep runs beside the other's z sweep
+    mpcf::BatchDev dev[2]{};
+    int tot[2][3] = {};       // per group: blocks of the x+y sweep, z sweep, speed of sound
+    hipStream_t stream2 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<std::pair<hipGraphExec_t, hipEvent_t>> retired;
     // statistics (engine-owned, read under mu via snapshot)
@@ -453,52 +458,84 @@ bool rebuild(mpcf_server* S)
     int* hcur = S->h_cur[sl];
     const int n = (int)S->active.size();
     const int cap = S->max_live;
-    int* oxy = S->h_off[sl];
-    int* oz = oxy + (cap + 1);
-    int* os = oxy + 2 * (cap + 1);
-    oxy[0] = oz[0] = os[0] = 0;
+    const int tab = 3 * (cap + 1);
+    int* hoff = S->h_off[sl];
     long long cells = 0, cols_xy = 0, cols_z = 0;
     for (Req* r : S->active) {
         cols_xy += (long long)(r->c->g.nx + 6) * r->c->g.nz;
         cols_z += (long long)r->c->g.nx * r->c->g.ny;
+        cells += r->c->g.nx * (long long)r->c->g.ny * r->c->g.nz;
     }
+    // group 0: cases [0, split), group 1: [split, n), split where the cells
+    // reach half (one group when there is one case)
+    int split = n;
+    if (n >= 2) {
+        long long acc = 0;
+        for (int i = 0; i < n - 1; ++i) {
+            const mpcf::Geom& g = S->active[i]->c->g;
+            acc += g.nx * (long long)g.ny * g.nz;
+            split = i + 1;
+            if (2 * acc >= cells) break;
+        }
+    }
+    for (int grp = 0; grp < 2; ++grp) hoff[grp * tab] = hoff[grp * tab + (cap + 1)] = hoff[grp * tab + 2 * (cap + 1)] = 0;
     for (int i = 0; i < n; ++i) {
         mpcf_ctx* c = S->active[i]->c;
+        const int grp = i < split ? 0 : 1, j = grp ? i - split : i;   // index within the group
+        int* oxy = hoff + grp * tab;
+        int* oz = oxy + (cap + 1);
+        int* os = oxy + 2 * (cap + 1);
         int blocks[3];
         mpcf::detail::case_desc(c, c->cur, mpcf::detail::adaptive_march(c->g.ny, cols_xy),
                                 mpcf::detail::adaptive_march(c->g.nz, cols_z), hc[i], blocks);
-        oxy[i + 1] = oxy[i] + blocks[0];
-        oz[i + 1] = oz[i] + blocks[1];
-        os[i + 1] = os[i] + blocks[2];
+        oxy[j + 1] = oxy[j] + blocks[0];
+        oz[j + 1] = oz[j] + blocks[1];
+        os[j + 1] = os[j] + blocks[2];
         hcur[i] = c->cur;
-        cells += c->g.nx * (long long)c->g.ny * c->g.nz;
     }
     if (n > 0) {
         if (hipMemcpyAsync(S->d_cases, hc, n * sizeof(mpcf::CaseDesc), hipMemcpyHostToDevice, S->stream) != hipSuccess ||
             hipMemcpyAsync(S->d_cur, hcur, n * sizeof(int), hipMemcpyHostToDevice, S->stream) != hipSuccess ||
-            hipMemcpyAsync(S->d_off, oxy, 3 * (cap + 1) * sizeof(int), hipMemcpyHostToDevice, S->stream) != hipSuccess)
+            hipMemcpyAsync(S->d_off, hoff, 2 * tab * sizeof(int), hipMemcpyHostToDevice, S->stream) != hipSuccess)
             return false;
         if (hipEventRecord(S->slot_ev[sl], S->stream) != hipSuccess) return false;
     }
-    S->tot[0] = oxy[n]; S->tot[1] = oz[n]; S->tot[2] = os[n];
-    S->dev.cases = S->d_cases; S->dev.cur = S->d_cur; S->dev.n = n;
-    S->dev.off_xy = S->d_off; S->dev.off_z = S->d_off + (cap + 1); S->dev.off_sos = S->d_off + 2 * (cap + 1);
+    for (int grp = 0; grp < 2; ++grp) {
+        const int first = grp ? split : 0, count = grp ? n - split : split;
+        const int* ho = hoff + grp * tab;
+        S->tot[grp][0] = ho[count]; S->tot[grp][1] = ho[(cap + 1) + count]; S->tot[grp][2] = ho[2 * (cap + 1) + count];
+        mpcf::BatchDev& d = S->dev[grp];
+        d.cases = S->d_cases + first; d.cur = S->d_cur + first; d.n = count;
+        d.off_xy = S->d_off + grp * tab; d.off_z = d.off_xy + (cap + 1); d.off_sos = d.off_xy + 2 * (cap + 1);
+    }
     S->st_active_cells = cells;
     retire_graph(S);
     ++S->st_rebuilds;
     return true;
 }
 
-void enqueue_step_kernels(mpcf_server* S)
+bool enqueue_step_kernels(mpcf_server* S)
 {
     // the sweeps apply each case's boundary conditions as they load: seven
-    // launches per step
+    // launches per step and group; group 1 runs on the second stream
     const int cpu_form = (S->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0;
-    S->L->b_sos(S->dev, S->tot[2], cpu_form, S->cfl, 1e30, S->stream);
-    for (int s = 0; s < 3; ++s) {
-        S->L->b_xy(S->dev, S->tot[0], kA[s], kB[s], S->stream);
-        S->L->b_z(S->dev, S->tot[1], kA[s], kB[s], s == 2, cpu_form, S->stream);
+    const bool two = S->dev[1].n > 0;
+    if (two && (hipEventRecord(S->fork, S->stream) != hipSuccess ||
+                hipStreamWaitEvent(S->stream2, S->fork, 0) != hipSuccess))
+        return false;
+    for (int grp = two ? 1 : 0; grp >= 0; --grp) {
+        hipStream_t st = grp ? S->stream2 : S->stream;
+        const mpcf::BatchDev& d = S->dev[grp];
+        S->L->b_sos(d, S->tot[grp][2], cpu_form, S->cfl, 1e30, st);
+        for (int s = 0; s < 3; ++s) {
+            S->L->b_xy(d, S->tot[grp][0], kA[s], kB[s], st);
+            S->L->b_z(d, S->tot[grp][1], kA[s], kB[s], s == 2, cpu_form, st);
+        }
     }
+    if (two && (hipEventRecord(S->join, S->stream2) != hipSuccess ||
+                hipStreamWaitEvent(S->stream, S->join, 0) != hipSuccess))
+        return false;
+    return true;
 }
 
 bool enqueue_step(mpcf_server* S)
@@ -506,9 +543,10 @@ bool enqueue_step(mpcf_server* S)
     if (S->use_graph) {
         if (!S->exec) {
             if (hipStreamBeginCapture(S->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return false;
-            enqueue_step_kernels(S);
+            const bool ok = enqueue_step_kernels(S);
             hipGraph_t g = nullptr;
             if (hipStreamEndCapture(S->stream, &g) != hipSuccess) return false;
+            if (!ok) { if (g) hipGraphDestroy(g); return false; }
             const hipError_t e = hipGraphInstantiate(&S->exec, g, nullptr, nullptr, 0);
             hipGraphDestroy(g);
             if (e != hipSuccess) return false;
@@ -516,7 +554,7 @@ bool enqueue_step(mpcf_server* S)
         }
         if (hipGraphLaunch(S->exec, S->stream) != hipSuccess) return false;
     } else {
-        enqueue_step_kernels(S);
+        if (!enqueue_step_kernels(S)) return false;
         if (hipGetLastError() != hipSuccess) return false;
     }
     ++S->st_steps;
@@ -648,12 +686,18 @@ extern "C" int mpcf_server_create(mpcf_server** out, unsigned flags, long long h
             if (S->slot_ev[k]) hipEventDestroy(S->slot_ev[k]);
         }
         if (S->stream) hipStreamDestroy(S->stream);
+        if (S->stream2) hipStreamDestroy(S->stream2);
+        if (S->fork) hipEventDestroy(S->fork);
+        if (S->join) hipEventDestroy(S->join);
         delete S;
         return fail(code, m);
     };
     if (hipGetDevice(&S->device) != hipSuccess) return bail(MPCF_EHIP, "mpcf_server_create: hipGetDevice");
-    if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess)
-        return bail(MPCF_EHIP, "mpcf_server_create: stream");
+    if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&S->stream2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&S->fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&S->join, hipEventDisableTiming) != hipSuccess)
+        return bail(MPCF_EHIP, "mpcf_server_create: streams");
     S->arena.size = round256((size_t)hbm_bytes);
     if (hipMalloc((void**)&S->arena.base, S->arena.size) != hipSuccess)
         return bail(MPCF_ENOMEM, "mpcf_server_create: cannot allocate the HBM budget");
@@ -661,12 +705,12 @@ extern "C" int mpcf_server_create(mpcf_server** out, unsigned flags, long long h
     const int cap = max_live;
     if (hipMalloc((void**)&S->d_cases, cap * sizeof(mpcf::CaseDesc)) != hipSuccess ||
         hipMalloc((void**)&S->d_cur, cap * sizeof(int)) != hipSuccess ||
-        hipMalloc((void**)&S->d_off, 3 * (cap + 1) * sizeof(int)) != hipSuccess)
+        hipMalloc((void**)&S->d_off, 2 * 3 * (cap + 1) * sizeof(int)) != hipSuccess)
         return bail(MPCF_ENOMEM, "mpcf_server_create: tables");
     for (int k = 0; k < mpcf_server::NSLOT; ++k)
         if (hipHostMalloc((void**)&S->h_cases[k], cap * sizeof(mpcf::CaseDesc), hipHostMallocDefault) != hipSuccess ||
             hipHostMalloc((void**)&S->h_cur[k], cap * sizeof(int), hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void**)&S->h_off[k], 3 * (cap + 1) * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void**)&S->h_off[k], 2 * 3 * (cap + 1) * sizeof(int), hipHostMallocDefault) != hipSuccess ||
             hipEventCreateWithFlags(&S->slot_ev[k], hipEventDisableTiming) != hipSuccess)
             return bail(MPCF_ENOMEM, "mpcf_server_create: tables");
     S->copier = std::thread(copier_main, S);
@@ -789,6 +833,10 @@ extern "C" int mpcf_server_destroy(mpcf_server* S)
         hipHostFree(S->h_off[k]);
         hipEventDestroy(S->slot_ev[k]);
     }
+    hipStreamSynchronize(S->stream2);
+    hipStreamDestroy(S->stream2);
+    hipEventDestroy(S->fork);
+    hipEventDestroy(S->join);
     hipStreamDestroy(S->stream);
     delete S;
     return MPCF_OK;
