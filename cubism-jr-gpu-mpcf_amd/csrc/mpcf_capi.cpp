@@ -695,13 +695,13 @@ struct mpcf_batch
     int cap = 0;
     bool dirty = true;
     bool cur_on_device = false;              // d_cur holds the truth (after steps)
-    // the cases in two groups of about equal cells, stepped on two streams
-    // (group 1 on stream2, forked from and joined back to stream), so that
-    // one group's x+y sweep runs beside the other's z sweep
-    int tot_xy[2] = {}, tot_z[2] = {}, tot_sos[2] = {};
-    mpcf::BatchDev dev[2]{};
-    hipStream_t stream2 = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    // the cases in kBatchGroups groups of about equal cells, each stepped on
+    // its own stream (groups 1.. on side streams forked from and joined back
+    // to `stream`), so that one group's x+y sweep runs beside another's z sweep
+    int tot_xy[kBatchGroups] = {}, tot_z[kBatchGroups] = {}, tot_sos[kBatchGroups] = {};
+    mpcf::BatchDev dev[kBatchGroups]{};
+    hipStream_t side[kBatchGroups] = {};
+    hipEvent_t fork = nullptr, join[kBatchGroups] = {};
     // graph of one step
     bool use_graph = false;
     bool graph_valid = false;
@@ -761,34 +761,29 @@ int batch_rebuild(mpcf_batch* b)
         const int cap = std::max(16, 2 * n);
         hipError_t e = hipMalloc((void**)&b->d_cases, cap * sizeof(mpcf::CaseDesc));
         if (e == hipSuccess) e = hipMalloc((void**)&b->d_cur, cap * sizeof(int));
-        if (e == hipSuccess) e = hipMalloc((void**)&b->d_off, 2 * 3 * (cap + 1) * sizeof(int));
+        if (e == hipSuccess) e = hipMalloc((void**)&b->d_off, kBatchGroups * 3 * (cap + 1) * sizeof(int));
         if (e != hipSuccess) return fail(MPCF_ENOMEM, std::string("batch tables: ") + hipGetErrorString(e));
         b->cap = cap;
     }
     std::vector<mpcf::CaseDesc> desc(n);
     const int tab = 3 * (b->cap + 1);
-    std::vector<int> cur(n), off(2 * tab, 0);
-    long long cols_xy = 0, cols_z = 0, cells = 0;
+    std::vector<int> cur(n), off(kBatchGroups * tab, 0);
+    long long cols_xy = 0, cols_z = 0;
     for (mpcf_ctx* c : b->cases) {
         cols_xy += (long long)(c->g.nx + 6) * c->g.nz;
         cols_z += (long long)c->g.nx * c->g.ny;
-        cells += (long long)c->g.nx * c->g.ny * c->g.nz;
     }
-    // group 0: cases [0, split), group 1: [split, n), split where the cells
-    // reach half (one group when there is one case)
-    int split = n;
-    if (n >= 2) {
-        long long acc = 0;
-        split = 1;
-        for (int i = 0; i < n - 1; ++i) {
-            acc += (long long)b->cases[i]->g.nx * b->cases[i]->g.ny * b->cases[i]->g.nz;
-            split = i + 1;
-            if (2 * acc >= cells) break;
-        }
+    int first[kBatchGroups + 1];
+    {
+        std::vector<long long> cc(n);
+        for (int i = 0; i < n; ++i) cc[i] = (long long)b->cases[i]->g.nx * b->cases[i]->g.ny * b->cases[i]->g.nz;
+        split_groups(cc.data(), n, kBatchGroups, first);
     }
     for (int i = 0; i < n; ++i) {
         mpcf_ctx* c = b->cases[i];
-        const int grp = i < split ? 0 : 1, j = grp ? i - split : i;   // index within the group
+        int grp = 0;
+        while (i >= first[grp + 1]) ++grp;
+        const int j = i - first[grp];                                      // index within the group
         int* oxy = &off[grp * tab];
         int* oz = oxy + (b->cap + 1);
         int* os = oxy + 2 * (b->cap + 1);
@@ -806,12 +801,12 @@ int batch_rebuild(mpcf_batch* b)
         HIP_TRY(hipMemcpyAsync(b->d_off, off.data(), off.size() * sizeof(int), hipMemcpyHostToDevice, b->stream));
         HIP_TRY(hipStreamSynchronize(b->stream));
     }
-    for (int grp = 0; grp < 2; ++grp) {
-        const int first = grp ? split : 0, count = grp ? n - split : split;
+    for (int grp = 0; grp < kBatchGroups; ++grp) {
+        const int count = first[grp + 1] - first[grp];
         const int* ho = &off[grp * tab];
         b->tot_xy[grp] = ho[count]; b->tot_z[grp] = ho[(b->cap + 1) + count]; b->tot_sos[grp] = ho[2 * (b->cap + 1) + count];
         mpcf::BatchDev& d = b->dev[grp];
-        d.cases = b->d_cases + first; d.cur = b->d_cur + first; d.n = count;
+        d.cases = b->d_cases + first[grp]; d.cur = b->d_cur + first[grp]; d.n = count;
         d.off_xy = b->d_off + grp * tab; d.off_z = d.off_xy + (b->cap + 1); d.off_sos = d.off_xy + 2 * (b->cap + 1);
     }
     b->dirty = false;
@@ -822,31 +817,33 @@ int batch_rebuild(mpcf_batch* b)
 int batch_enqueue_step(mpcf_batch* b, double cfl, double dt_max)
 {
     // the sweeps apply each case's boundary conditions as they load: a step is
-    // seven launches per group; group 1 runs on the second stream
+    // seven launches per group; groups 1.. run on side streams
     const int cpu_form = (b->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0;
-    const bool two = b->dev[1].n > 0;
-    if (two) {
-        if (!b->stream2) {
-            HIP_TRY(hipStreamCreateWithFlags(&b->stream2, hipStreamNonBlocking));
-            HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
-            HIP_TRY(hipEventCreateWithFlags(&b->join, hipEventDisableTiming));
-        }
-        HIP_TRY(hipEventRecord(b->fork, b->stream));
-        HIP_TRY(hipStreamWaitEvent(b->stream2, b->fork, 0));
-    }
-    for (int grp = two ? 1 : 0; grp >= 0; --grp) {
-        hipStream_t st = grp ? b->stream2 : b->stream;
+    bool forked = false;
+    for (int grp = kBatchGroups - 1; grp >= 0; --grp) {
         const mpcf::BatchDev& d = b->dev[grp];
+        if (d.n == 0) continue;
+        hipStream_t st = b->stream;
+        if (grp > 0) {
+            if (!b->side[grp]) {
+                HIP_TRY(hipStreamCreateWithFlags(&b->side[grp], hipStreamNonBlocking));
+                HIP_TRY(hipEventCreateWithFlags(&b->join[grp], hipEventDisableTiming));
+            }
+            if (!b->fork) HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
+            if (!forked) HIP_TRY(hipEventRecord(b->fork, b->stream));
+            forked = true;
+            st = b->side[grp];
+            HIP_TRY(hipStreamWaitEvent(st, b->fork, 0));
+        }
         b->L->b_sos(d, b->tot_sos[grp], cpu_form, cfl, dt_max, st);
         for (int s = 0; s < 3; ++s) {
             b->L->b_xy(d, b->tot_xy[grp], kA[s], kB[s], st);
             b->L->b_z(d, b->tot_z[grp], kA[s], kB[s], s == 2, cpu_form, st);
         }
+        if (grp > 0) HIP_TRY(hipEventRecord(b->join[grp], st));
     }
-    if (two) {
-        HIP_TRY(hipEventRecord(b->join, b->stream2));
-        HIP_TRY(hipStreamWaitEvent(b->stream, b->join, 0));
-    }
+    for (int grp = 1; grp < kBatchGroups; ++grp)
+        if (b->dev[grp].n > 0) HIP_TRY(hipStreamWaitEvent(b->stream, b->join[grp], 0));
     return MPCF_OK;
 }
 
@@ -894,12 +891,13 @@ extern "C" int mpcf_batch_destroy(mpcf_batch* b)
     if (b->d_cases) hipFree(b->d_cases);
     if (b->d_cur) hipFree(b->d_cur);
     if (b->d_off) hipFree(b->d_off);
-    if (b->stream2) {
-        hipStreamSynchronize(b->stream2);
-        hipStreamDestroy(b->stream2);
-        hipEventDestroy(b->fork);
-        hipEventDestroy(b->join);
-    }
+    for (int grp = 1; grp < kBatchGroups; ++grp)
+        if (b->side[grp]) {
+            hipStreamSynchronize(b->side[grp]);
+            hipStreamDestroy(b->side[grp]);
+            hipEventDestroy(b->join[grp]);
+        }
+    if (b->fork) hipEventDestroy(b->fork);
     hipStreamDestroy(b->stream);
     delete b;
     return MPCF_OK;

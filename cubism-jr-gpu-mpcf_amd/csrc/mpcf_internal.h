@@ -71,6 +71,32 @@ inline const Launchers kFastP = MPCF_LAUNCHERS(fastp);
 inline const Launchers kF64 = MPCF_LAUNCHERS(f64);
 #undef MPCF_LAUNCHERS
 
+// batches and the server step their cases as this many groups of about equal
+// cells, each on its own stream
+#ifndef MPCF_BATCH_GROUPS
+#define MPCF_BATCH_GROUPS 2
+#endif
+constexpr int kBatchGroups = MPCF_BATCH_GROUPS;
+
+// group g of the n cases (cells[i] each) is [first[g], first[g + 1]): every
+// group gets one case while cases remain, then the cases whose midpoint lies
+// below g + 1 parts of G of the total cells, keeping one case for each later
+// group
+inline void split_groups(const long long* cells, const int n, const int G, int* first)
+{
+    long long total = 0, acc = 0;
+    for (int i = 0; i < n; ++i) total += cells[i];
+    int i = 0;
+    first[0] = 0;
+    for (int g = 1; g < G; ++g) {
+        const long long target2 = 2 * (total / G * g + total % G * g / G);
+        if (i < n) acc += cells[i++];
+        while (i < n - (G - g) && 2 * acc + cells[i] < target2) acc += cells[i++];
+        first[g] = i;
+    }
+    first[G] = n;
+}
+
 inline const unsigned kAllFlags = MPCF_FLAG_FAST | MPCF_FLAG_SOS_CPU | MPCF_FLAG_SPLIT_XY | MPCF_FLAG_FUSED_XY |
                            MPCF_FLAG_P64 | MPCF_FLAG_FP64 | MPCF_FLAG_NO_PIPELINE;
 
@@ -97,6 +123,8 @@ inline const double kB[3] = {1. / 4, 8. / 9, 3. / 4};
 
 using mpcf::detail::Launchers;
 using mpcf::detail::pick_launchers;
+using mpcf::detail::kBatchGroups;
+using mpcf::detail::split_groups;
 using mpcf::detail::elem_bytes;
 using mpcf::detail::kA;
 using mpcf::detail::kB;

@@ -175,13 +175,14 @@ struct mpcf_server
     int slot = 0;
     std::vector<Req*> active;
     std::vector<long long> active_seq;
-    // the live cases in two groups of about equal cells, stepped on two
-    // streams (group 1 on stream2, forked from and joined back to stream), so
-    // that one group's x+y sweep runs beside the other's z sweep
-    mpcf::BatchDev dev[2]{};
-    int tot[2][3] = {};       // per group: blocks of the x+y sweep, z sweep, speed of sound
-    hipStream_t stream2 = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    // the live cases in kBatchGroups groups of about equal cells, each stepped
+    // on its own stream (groups 1.. on side streams forked from and joined
+    // back to `stream`), so that one group's x+y sweep runs beside another's
+    // z sweep
+    mpcf::BatchDev dev[kBatchGroups]{};
+    int tot[kBatchGroups][3] = {};   // per group: blocks of the x+y sweep, z sweep, speed of sound
+    hipStream_t side[kBatchGroups] = {};
+    hipEvent_t fork = nullptr, join[kBatchGroups] = {};
     hipGraphExec_t exec = nullptr;
     std::vector<std::pair<hipGraphExec_t, hipEvent_t>> retired;
     // statistics (engine-owned, read under mu via snapshot)
@@ -461,27 +462,23 @@ bool rebuild(mpcf_server* S)
     const int tab = 3 * (cap + 1);
     int* hoff = S->h_off[sl];
     long long cells = 0, cols_xy = 0, cols_z = 0;
-    for (Req* r : S->active) {
-        cols_xy += (long long)(r->c->g.nx + 6) * r->c->g.nz;
-        cols_z += (long long)r->c->g.nx * r->c->g.ny;
-        cells += r->c->g.nx * (long long)r->c->g.ny * r->c->g.nz;
+    std::vector<long long> cc(n);
+    for (int i = 0; i < n; ++i) {
+        const mpcf::Geom& g = S->active[i]->c->g;
+        cols_xy += (long long)(g.nx + 6) * g.nz;
+        cols_z += (long long)g.nx * g.ny;
+        cc[i] = g.nx * (long long)g.ny * g.nz;
+        cells += cc[i];
     }
-    // group 0: cases [0, split), group 1: [split, n), split where the cells
-    // reach half (one group when there is one case)
-    int split = n;
-    if (n >= 2) {
-        long long acc = 0;
-        for (int i = 0; i < n - 1; ++i) {
-            const mpcf::Geom& g = S->active[i]->c->g;
-            acc += g.nx * (long long)g.ny * g.nz;
-            split = i + 1;
-            if (2 * acc >= cells) break;
-        }
-    }
-    for (int grp = 0; grp < 2; ++grp) hoff[grp * tab] = hoff[grp * tab + (cap + 1)] = hoff[grp * tab + 2 * (cap + 1)] = 0;
+    int first[kBatchGroups + 1];
+    split_groups(cc.data(), n, kBatchGroups, first);
+    for (int grp = 0; grp < kBatchGroups; ++grp)
+        hoff[grp * tab] = hoff[grp * tab + (cap + 1)] = hoff[grp * tab + 2 * (cap + 1)] = 0;
     for (int i = 0; i < n; ++i) {
         mpcf_ctx* c = S->active[i]->c;
-        const int grp = i < split ? 0 : 1, j = grp ? i - split : i;   // index within the group
+        int grp = 0;
+        while (i >= first[grp + 1]) ++grp;
+        const int j = i - first[grp];                                      // index within the group
         int* oxy = hoff + grp * tab;
         int* oz = oxy + (cap + 1);
         int* os = oxy + 2 * (cap + 1);
@@ -496,16 +493,16 @@ bool rebuild(mpcf_server* S)
     if (n > 0) {
         if (hipMemcpyAsync(S->d_cases, hc, n * sizeof(mpcf::CaseDesc), hipMemcpyHostToDevice, S->stream) != hipSuccess ||
             hipMemcpyAsync(S->d_cur, hcur, n * sizeof(int), hipMemcpyHostToDevice, S->stream) != hipSuccess ||
-            hipMemcpyAsync(S->d_off, hoff, 2 * tab * sizeof(int), hipMemcpyHostToDevice, S->stream) != hipSuccess)
+            hipMemcpyAsync(S->d_off, hoff, kBatchGroups * tab * sizeof(int), hipMemcpyHostToDevice, S->stream) != hipSuccess)
             return false;
         if (hipEventRecord(S->slot_ev[sl], S->stream) != hipSuccess) return false;
     }
-    for (int grp = 0; grp < 2; ++grp) {
-        const int first = grp ? split : 0, count = grp ? n - split : split;
+    for (int grp = 0; grp < kBatchGroups; ++grp) {
+        const int count = first[grp + 1] - first[grp];
         const int* ho = hoff + grp * tab;
         S->tot[grp][0] = ho[count]; S->tot[grp][1] = ho[(cap + 1) + count]; S->tot[grp][2] = ho[2 * (cap + 1) + count];
         mpcf::BatchDev& d = S->dev[grp];
-        d.cases = S->d_cases + first; d.cur = S->d_cur + first; d.n = count;
+        d.cases = S->d_cases + first[grp]; d.cur = S->d_cur + first[grp]; d.n = count;
         d.off_xy = S->d_off + grp * tab; d.off_z = d.off_xy + (cap + 1); d.off_sos = d.off_xy + 2 * (cap + 1);
     }
     S->st_active_cells = cells;
@@ -517,24 +514,28 @@ bool rebuild(mpcf_server* S)
 bool enqueue_step_kernels(mpcf_server* S)
 {
     // the sweeps apply each case's boundary conditions as they load: seven
-    // launches per step and group; group 1 runs on the second stream
+    // launches per step and group; groups 1.. run on side streams
     const int cpu_form = (S->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0;
-    const bool two = S->dev[1].n > 0;
-    if (two && (hipEventRecord(S->fork, S->stream) != hipSuccess ||
-                hipStreamWaitEvent(S->stream2, S->fork, 0) != hipSuccess))
-        return false;
-    for (int grp = two ? 1 : 0; grp >= 0; --grp) {
-        hipStream_t st = grp ? S->stream2 : S->stream;
+    bool forked = false;
+    for (int grp = kBatchGroups - 1; grp >= 0; --grp) {
         const mpcf::BatchDev& d = S->dev[grp];
+        if (d.n == 0) continue;
+        hipStream_t st = S->stream;
+        if (grp > 0) {
+            if (!forked && hipEventRecord(S->fork, S->stream) != hipSuccess) return false;
+            forked = true;
+            st = S->side[grp];
+            if (hipStreamWaitEvent(st, S->fork, 0) != hipSuccess) return false;
+        }
         S->L->b_sos(d, S->tot[grp][2], cpu_form, S->cfl, 1e30, st);
         for (int s = 0; s < 3; ++s) {
             S->L->b_xy(d, S->tot[grp][0], kA[s], kB[s], st);
             S->L->b_z(d, S->tot[grp][1], kA[s], kB[s], s == 2, cpu_form, st);
         }
+        if (grp > 0 && hipEventRecord(S->join[grp], st) != hipSuccess) return false;
     }
-    if (two && (hipEventRecord(S->join, S->stream2) != hipSuccess ||
-                hipStreamWaitEvent(S->stream, S->join, 0) != hipSuccess))
-        return false;
+    for (int grp = 1; grp < kBatchGroups; ++grp)
+        if (S->dev[grp].n > 0 && hipStreamWaitEvent(S->stream, S->join[grp], 0) != hipSuccess) return false;
     return true;
 }
 
@@ -686,18 +687,22 @@ extern "C" int mpcf_server_create(mpcf_server** out, unsigned flags, long long h
             if (S->slot_ev[k]) hipEventDestroy(S->slot_ev[k]);
         }
         if (S->stream) hipStreamDestroy(S->stream);
-        if (S->stream2) hipStreamDestroy(S->stream2);
+        for (int grp = 1; grp < kBatchGroups; ++grp) {
+            if (S->side[grp]) hipStreamDestroy(S->side[grp]);
+            if (S->join[grp]) hipEventDestroy(S->join[grp]);
+        }
         if (S->fork) hipEventDestroy(S->fork);
-        if (S->join) hipEventDestroy(S->join);
         delete S;
         return fail(code, m);
     };
     if (hipGetDevice(&S->device) != hipSuccess) return bail(MPCF_EHIP, "mpcf_server_create: hipGetDevice");
     if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&S->stream2, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&S->fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&S->join, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&S->fork, hipEventDisableTiming) != hipSuccess)
         return bail(MPCF_EHIP, "mpcf_server_create: streams");
+    for (int grp = 1; grp < kBatchGroups; ++grp)
+        if (hipStreamCreateWithFlags(&S->side[grp], hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&S->join[grp], hipEventDisableTiming) != hipSuccess)
+            return bail(MPCF_EHIP, "mpcf_server_create: streams");
     S->arena.size = round256((size_t)hbm_bytes);
     if (hipMalloc((void**)&S->arena.base, S->arena.size) != hipSuccess)
         return bail(MPCF_ENOMEM, "mpcf_server_create: cannot allocate the HBM budget");
@@ -705,12 +710,12 @@ extern "C" int mpcf_server_create(mpcf_server** out, unsigned flags, long long h
     const int cap = max_live;
     if (hipMalloc((void**)&S->d_cases, cap * sizeof(mpcf::CaseDesc)) != hipSuccess ||
         hipMalloc((void**)&S->d_cur, cap * sizeof(int)) != hipSuccess ||
-        hipMalloc((void**)&S->d_off, 2 * 3 * (cap + 1) * sizeof(int)) != hipSuccess)
+        hipMalloc((void**)&S->d_off, kBatchGroups * 3 * (cap + 1) * sizeof(int)) != hipSuccess)
         return bail(MPCF_ENOMEM, "mpcf_server_create: tables");
     for (int k = 0; k < mpcf_server::NSLOT; ++k)
         if (hipHostMalloc((void**)&S->h_cases[k], cap * sizeof(mpcf::CaseDesc), hipHostMallocDefault) != hipSuccess ||
             hipHostMalloc((void**)&S->h_cur[k], cap * sizeof(int), hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void**)&S->h_off[k], 2 * 3 * (cap + 1) * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void**)&S->h_off[k], kBatchGroups * 3 * (cap + 1) * sizeof(int), hipHostMallocDefault) != hipSuccess ||
             hipEventCreateWithFlags(&S->slot_ev[k], hipEventDisableTiming) != hipSuccess)
             return bail(MPCF_ENOMEM, "mpcf_server_create: tables");
     S->copier = std::thread(copier_main, S);
@@ -833,10 +838,12 @@ extern "C" int mpcf_server_destroy(mpcf_server* S)
         hipHostFree(S->h_off[k]);
         hipEventDestroy(S->slot_ev[k]);
     }
-    hipStreamSynchronize(S->stream2);
-    hipStreamDestroy(S->stream2);
+    for (int grp = 1; grp < kBatchGroups; ++grp) {
+        hipStreamSynchronize(S->side[grp]);
+        hipStreamDestroy(S->side[grp]);
+        hipEventDestroy(S->join[grp]);
+    }
     hipEventDestroy(S->fork);
-    hipEventDestroy(S->join);
     hipStreamDestroy(S->stream);
     delete S;
     return MPCF_OK;
