@@ -133,9 +133,10 @@ int pipeline_slabs(const mpcf_ctx* c)
 }
 
 // the three stages of a step over S z-slabs on two streams.  The x+y sweep
-// (the fused one, or the x and the y sweep in turn) of slab j needs the previous stage's z sweep of slab j only (its stencil is in
-// x and y; it also rewrites that slab's R and accumulators), so it waits for
-// that event; the z sweep of slab j waits for the x+y sweep of slab j and,
+// (the fused one, or the x and the y sweep in turn) of slab j needs the
+// previous stage's z sweep of slab j only (its stencil is in x and y; it also
+// rewrites that slab's R and accumulators), so it waits for that event; the
+// z sweep of slab j waits for the x+y sweep of slab j and,
 // by the order of the second stream, for every z sweep of the previous
 // stage (its stencil reaches into the neighbouring slabs, and its output
 // buffer is the one they read).  The x+y sweeps of the next slab run while a
@@ -691,7 +692,7 @@ struct mpcf_batch
     // device tables
     mpcf::CaseDesc* d_cases = nullptr;
     int* d_cur = nullptr;
-    int* d_off = nullptr;                    // 2 groups x 3 x (cap + 1)
+    int* d_off = nullptr;                    // kBatchGroups x 3 x (cap + 1)
     int cap = 0;
     bool dirty = true;
     bool cur_on_device = false;              // d_cur holds the truth (after steps)
