@@ -128,6 +128,40 @@ MPCF_HD PrimT<T> to_prim(T r, T ru, T rv, T rw, T E, T G, T P)
 // (GPUonly.cuh:99-194; WENOEPS = 1e-6, Makefile.config:15)
 template <typename T> struct RecT { T m, p; };
 
+// the FAST reconstruction from the window's centre three values and its four
+// first differences dab = b-a, dbc = c-b, dcd = d-c, dde = e-d (weno_cell
+// with FAST == true is this on the differences it forms; a caller that has
+// the differences already, e.g. from neighbouring lanes, passes them in)
+template <typename T>
+MPCF_HD RecT<T> weno_fast_diff(const T b, const T c, const T d, const T dab, const T dbc, const T dcd, const T dde)
+{
+    RecT<T> out;
+    const T eps = T(1e-6);
+    const T Db = dbc - dab, Dc = dcd - dbc, Dd = dde - dcd;   // second differences
+    const T tA = Db + 2.0f*dbc;                               // a - 4b + 3c
+    const T tB = dbc + dcd;                                   // d - b
+    const T tC = Dd - 2.0f*dcd;                               // 3c - 4d + e
+    const T k13_12 = T(13./12.);
+    const T iA = Db*(k13_12*Db) + ((0.25f*tA)*tA + eps);
+    const T iB = Dc*(k13_12*Dc) + ((0.25f*tB)*tB + eps);
+    const T iC = Dd*(k13_12*Dd) + ((0.25f*tC)*tC + eps);
+    const T rA = mrcp(iA*iA), rB = mrcp(iB*iB), rC = mrcp(iC*iC);
+    const T aB = 6.0f*rB, mC = 3.0f*rC, pA = 3.0f*rA;
+    // minus-state candidates (window a..e), 6 (q_k - c)
+    const T nm0 = 5.0f*dbc - 2.0f*dab, nm1 = dbc + 2.0f*dcd, nm2 = 4.0f*dcd - dde;
+    // plus-state candidates (the mirror), 6 (q_k - c); the middle one as
+    // -(its value)
+    const T np0 = 2.0f*dde - 5.0f*dcd, np1n = dcd + 2.0f*dbc, np2 = dab - 4.0f*dbc;
+    const T k1_6 = T(1./6.);
+    const T om = (rA*nm0 + aB*nm1 + mC*nm2) * mrcp(rA + aB + mC);
+    const T op = (rC*np0 - aB*np1n + pA*np2) * mrcp(rC + aB + pA);
+    const T mn = mmin(mmin(b, c), d);
+    const T mx = mmax(mmax(b, c), d);
+    out.m = mclamp(om*k1_6 + c, mn, mx);
+    out.p = mclamp(op*k1_6 + c, mn, mx);
+    return out;
+}
+
 template <typename T, bool FAST>
 MPCF_HD RecT<T> weno_cell(const T a, const T b, const T c, const T d, const T e)
 {
@@ -151,30 +185,7 @@ MPCF_HD RecT<T> weno_cell(const T a, const T b, const T c, const T d, const T e)
         //    c + (-2 (b-a) + 5 (c-b))/6, so a state is
         //    c + sum(alpha_k n_k) / (6 sum(alpha));
         //  * the clip to the three central cells as one median-of-three.
-        const T dab = b - a, dbc = c - b, dcd = d - c, dde = e - d;
-        const T Db = dbc - dab, Dc = dcd - dbc, Dd = dde - dcd;   // second differences
-        const T tA = Db + 2.0f*dbc;                               // a - 4b + 3c
-        const T tB = dbc + dcd;                                   // d - b
-        const T tC = Dd - 2.0f*dcd;                               // 3c - 4d + e
-        const T k13_12 = T(13./12.);
-        const T iA = Db*(k13_12*Db) + ((0.25f*tA)*tA + eps);
-        const T iB = Dc*(k13_12*Dc) + ((0.25f*tB)*tB + eps);
-        const T iC = Dd*(k13_12*Dd) + ((0.25f*tC)*tC + eps);
-        const T rA = mrcp(iA*iA), rB = mrcp(iB*iB), rC = mrcp(iC*iC);
-        const T aB = 6.0f*rB, mC = 3.0f*rC, pA = 3.0f*rA;
-        // minus-state candidates (window a..e), 6 (q_k - c)
-        const T nm0 = 5.0f*dbc - 2.0f*dab, nm1 = dbc + 2.0f*dcd, nm2 = 4.0f*dcd - dde;
-        // plus-state candidates (the mirror), 6 (q_k - c); the middle one as
-        // -(its value)
-        const T np0 = 2.0f*dde - 5.0f*dcd, np1n = dcd + 2.0f*dbc, np2 = dab - 4.0f*dbc;
-        const T k1_6 = T(1./6.);
-        const T om = (rA*nm0 + aB*nm1 + mC*nm2) * mrcp(rA + aB + mC);
-        const T op = (rC*np0 - aB*np1n + pA*np2) * mrcp(rC + aB + pA);
-        const T mn = mmin(mmin(b, c), d);
-        const T mx = mmax(mmax(b, c), d);
-        out.m = mclamp(om*k1_6 + c, mn, mx);
-        out.p = mclamp(op*k1_6 + c, mn, mx);
-        return out;
+        return weno_fast_diff<T>(b, c, d, b - a, c - b, d - c, e - d);
     } else {
         // the three smoothness indicators (minus is0/is1/is2 == plus is2/is1/is0)
         const T isA = a*(a*k4_3 - b*k19_3 + c*k11_3) + b*(b*k25_3 - c*k31_3) + c*c*k10_3;
