@@ -63,6 +63,25 @@ def test_batch_equals_alone(mpcf, fast, graph):
         assert np.array_equal(got[j][1], R), j
 
 
+@pytest.mark.parametrize("count", [1, 2, 3])
+def test_batch_group_splits_equal_alone(mpcf, count):
+    """A batch steps its cases as two groups on two streams: with one case
+    (one group), two (one each) and three (uneven), every case still equals
+    the same case stepped alone, through graph replays."""
+    cs = cases(mpcf)[2:2 + count]
+    steps = 3
+    say("group split", count, "cases")
+    with mpcf.Batch(fast=True, graph=True) as B:
+        ids = [B.add(s, bc=bc) for s, bc in cs]
+        for _ in range(steps):
+            B.step(0.3)
+        got = [(B.download(i), [B.last_dt(i)[0]]) for i in ids]
+    for j, (s, bc) in enumerate(cs):
+        u, _, dts = alone(mpcf, s, bc, steps, fast=True)
+        assert got[j][1] == dts[-1:], j
+        assert np.array_equal(got[j][0], u), j
+
+
 def test_graph_replays_match_eager_each_step(mpcf):
     cs = cases(mpcf)[:4]
     say("graph replay vs eager, step by step")
