@@ -86,7 +86,7 @@ def parse():
     p.add_argument("--no-serving", action="store_true", help="skip the served-stream comparison")
     p.add_argument("--serving-requests", type=int, default=48)
     p.add_argument("--serving-policy", default="fifo_slack", choices=("all", "fifo_fill", "srpt_fill", "hybrid", "fifo_slack"))
-    p.add_argument("--serving-fill", type=int, default=4 << 20, help="active-subset cell budget (floor) of the server's policy")
+    p.add_argument("--serving-fill", type=int, default=16 << 20, help="active-subset cell budget (floor) of the server's policy")
     p.add_argument("--no-precision", action="store_true", help="skip the binary32-vs-binary64 leg")
     p.add_argument("--precision-n", type=int, default=128)
     p.add_argument("--precision-steps", type=int, default=50)
