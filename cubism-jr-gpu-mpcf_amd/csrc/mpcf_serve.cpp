@@ -175,14 +175,14 @@ struct mpcf_server
     int slot = 0;
     std::vector<Req*> active;
     std::vector<long long> active_seq;
-    // the live cases in kBatchGroups groups of about equal cells, each stepped
+    // the live cases in kServerGroups groups of about equal cells, each stepped
     // on its own stream (groups 1.. on side streams forked from and joined
     // back to `stream`), so that one group's x+y sweep runs beside another's
     // z sweep
-    mpcf::BatchDev dev[kBatchGroups]{};
-    int tot[kBatchGroups][3] = {};   // per group: blocks of the x+y sweep, z sweep, speed of sound
-    hipStream_t side[kBatchGroups] = {};
-    hipEvent_t fork = nullptr, join[kBatchGroups] = {};
+    mpcf::BatchDev dev[kServerGroups]{};
+    int tot[kServerGroups][3] = {};   // per group: blocks of the x+y sweep, z sweep, speed of sound
+    hipStream_t side[kServerGroups] = {};
+    hipEvent_t fork = nullptr, join[kServerGroups] = {};
     hipGraphExec_t exec = nullptr;
     std::vector<std::pair<hipGraphExec_t, hipEvent_t>> retired;
     // statistics (engine-owned, read under mu via snapshot)
@@ -470,9 +470,9 @@ bool rebuild(mpcf_server* S)
         cc[i] = g.nx * (long long)g.ny * g.nz;
         cells += cc[i];
     }
-    int first[kBatchGroups + 1];
-    split_groups(cc.data(), n, kBatchGroups, first);
-    for (int grp = 0; grp < kBatchGroups; ++grp)
+    int first[kServerGroups + 1];
+    split_groups(cc.data(), n, kServerGroups, first);
+    for (int grp = 0; grp < kServerGroups; ++grp)
         hoff[grp * tab] = hoff[grp * tab + (cap + 1)] = hoff[grp * tab + 2 * (cap + 1)] = 0;
     for (int i = 0; i < n; ++i) {
         mpcf_ctx* c = S->active[i]->c;
@@ -493,11 +493,11 @@ bool rebuild(mpcf_server* S)
     if (n > 0) {
         if (hipMemcpyAsync(S->d_cases, hc, n * sizeof(mpcf::CaseDesc), hipMemcpyHostToDevice, S->stream) != hipSuccess ||
             hipMemcpyAsync(S->d_cur, hcur, n * sizeof(int), hipMemcpyHostToDevice, S->stream) != hipSuccess ||
-            hipMemcpyAsync(S->d_off, hoff, kBatchGroups * tab * sizeof(int), hipMemcpyHostToDevice, S->stream) != hipSuccess)
+            hipMemcpyAsync(S->d_off, hoff, kServerGroups * tab * sizeof(int), hipMemcpyHostToDevice, S->stream) != hipSuccess)
             return false;
         if (hipEventRecord(S->slot_ev[sl], S->stream) != hipSuccess) return false;
     }
-    for (int grp = 0; grp < kBatchGroups; ++grp) {
+    for (int grp = 0; grp < kServerGroups; ++grp) {
         const int count = first[grp + 1] - first[grp];
         const int* ho = hoff + grp * tab;
         S->tot[grp][0] = ho[count]; S->tot[grp][1] = ho[(cap + 1) + count]; S->tot[grp][2] = ho[2 * (cap + 1) + count];
@@ -517,7 +517,7 @@ bool enqueue_step_kernels(mpcf_server* S)
     // launches per step and group; groups 1.. run on side streams
     const int cpu_form = (S->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0;
     bool forked = false;
-    for (int grp = kBatchGroups - 1; grp >= 0; --grp) {
+    for (int grp = kServerGroups - 1; grp >= 0; --grp) {
         const mpcf::BatchDev& d = S->dev[grp];
         if (d.n == 0) continue;
         hipStream_t st = S->stream;
@@ -534,7 +534,7 @@ bool enqueue_step_kernels(mpcf_server* S)
         }
         if (grp > 0 && hipEventRecord(S->join[grp], st) != hipSuccess) return false;
     }
-    for (int grp = 1; grp < kBatchGroups; ++grp)
+    for (int grp = 1; grp < kServerGroups; ++grp)
         if (S->dev[grp].n > 0 && hipStreamWaitEvent(S->stream, S->join[grp], 0) != hipSuccess) return false;
     return true;
 }
@@ -687,7 +687,7 @@ extern "C" int mpcf_server_create(mpcf_server** out, unsigned flags, long long h
             if (S->slot_ev[k]) hipEventDestroy(S->slot_ev[k]);
         }
         if (S->stream) hipStreamDestroy(S->stream);
-        for (int grp = 1; grp < kBatchGroups; ++grp) {
+        for (int grp = 1; grp < kServerGroups; ++grp) {
             if (S->side[grp]) hipStreamDestroy(S->side[grp]);
             if (S->join[grp]) hipEventDestroy(S->join[grp]);
         }
@@ -699,7 +699,7 @@ extern "C" int mpcf_server_create(mpcf_server** out, unsigned flags, long long h
     if (hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&S->fork, hipEventDisableTiming) != hipSuccess)
         return bail(MPCF_EHIP, "mpcf_server_create: streams");
-    for (int grp = 1; grp < kBatchGroups; ++grp)
+    for (int grp = 1; grp < kServerGroups; ++grp)
         if (hipStreamCreateWithFlags(&S->side[grp], hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&S->join[grp], hipEventDisableTiming) != hipSuccess)
             return bail(MPCF_EHIP, "mpcf_server_create: streams");
@@ -710,12 +710,12 @@ extern "C" int mpcf_server_create(mpcf_server** out, unsigned flags, long long h
     const int cap = max_live;
     if (hipMalloc((void**)&S->d_cases, cap * sizeof(mpcf::CaseDesc)) != hipSuccess ||
         hipMalloc((void**)&S->d_cur, cap * sizeof(int)) != hipSuccess ||
-        hipMalloc((void**)&S->d_off, kBatchGroups * 3 * (cap + 1) * sizeof(int)) != hipSuccess)
+        hipMalloc((void**)&S->d_off, kServerGroups * 3 * (cap + 1) * sizeof(int)) != hipSuccess)
         return bail(MPCF_ENOMEM, "mpcf_server_create: tables");
     for (int k = 0; k < mpcf_server::NSLOT; ++k)
         if (hipHostMalloc((void**)&S->h_cases[k], cap * sizeof(mpcf::CaseDesc), hipHostMallocDefault) != hipSuccess ||
             hipHostMalloc((void**)&S->h_cur[k], cap * sizeof(int), hipHostMallocDefault) != hipSuccess ||
-            hipHostMalloc((void**)&S->h_off[k], kBatchGroups * 3 * (cap + 1) * sizeof(int), hipHostMallocDefault) != hipSuccess ||
+            hipHostMalloc((void**)&S->h_off[k], kServerGroups * 3 * (cap + 1) * sizeof(int), hipHostMallocDefault) != hipSuccess ||
             hipEventCreateWithFlags(&S->slot_ev[k], hipEventDisableTiming) != hipSuccess)
             return bail(MPCF_ENOMEM, "mpcf_server_create: tables");
     S->copier = std::thread(copier_main, S);
@@ -838,7 +838,7 @@ extern "C" int mpcf_server_destroy(mpcf_server* S)
         hipHostFree(S->h_off[k]);
         hipEventDestroy(S->slot_ev[k]);
     }
-    for (int grp = 1; grp < kBatchGroups; ++grp) {
+    for (int grp = 1; grp < kServerGroups; ++grp) {
         hipStreamSynchronize(S->side[grp]);
         hipStreamDestroy(S->side[grp]);
         hipEventDestroy(S->join[grp]);

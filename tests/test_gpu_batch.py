@@ -63,11 +63,12 @@ def test_batch_equals_alone(mpcf, fast, graph):
         assert np.array_equal(got[j][1], R), j
 
 
-@pytest.mark.parametrize("count", [1, 2, 3])
+@pytest.mark.parametrize("count", [1, 2, 3, 4])
 def test_batch_group_splits_equal_alone(mpcf, count):
-    """A batch steps its cases as two groups on two streams: with one case
-    (one group), two (one each) and three (uneven), every case still equals
-    the same case stepped alone, through graph replays."""
+    """A batch steps its cases as up to four groups of about equal cells on as
+    many streams: with one to four cases (one case a group; the six cases of
+    test_batch_equals_alone split unevenly), every case still equals the same
+    case stepped alone, through graph replays."""
     cs = cases(mpcf)[2:2 + count]
     steps = 3
     say("group split", count, "cases")
