@@ -1094,11 +1094,14 @@ void kb_sweep_z(const BatchDev B, const Real a, const Real b, const int fold, co
 // ---------------------------------------------------------------------------
 // host launchers
 // ---------------------------------------------------------------------------
-static int pick_chunk(const int nd, const int64_t columns, const int waves_per_col_group)
+static int pick_chunk(const int nd, const int64_t columns, const int mult)
 {
-    // enough marching threads for ~4 waves per SIMD (256 CUs x 4 SIMDs)
-    (void)waves_per_col_group;
-    const int64_t target_threads = 256LL * 4 * 4 * 64;
+    // enough marching threads for ~4 waves per SIMD (256 CUs x 4 SIMDs), times
+    // mult: a launch that runs alone wants more (its sweeps hold two waves per
+    // SIMD, and the last partial round of waves otherwise idles most of the
+    // chip); a slab of a pipelined step shares the chip with the other
+    // stream's sweep, which fills that tail
+    const int64_t target_threads = 256LL * 4 * 4 * 64 * mult;
     int nchunks = (int)((target_threads + columns - 1) / columns);
     if (nchunks < 1) nchunks = 1;
     int chunk = (nd + nchunks - 1) / nchunks;
@@ -1163,8 +1166,9 @@ void launch_sweep(const Geom& g, const int dir, const SweepParams& p, const int 
     A.sos_next = p.sos_next; A.sos_cpu = p.sos_cpu;
     A.z0 = p.z0; A.z1 = p.z1 < 0 ? g.nz : p.z1;
     const int nzr = A.z1 - A.z0;
+    const int alone = (A.z0 == 0 && A.z1 == g.nz) ? 2 : 1;   // a whole-block launch (not a slab)
     if (dir == 3) {
-        const int chunk = pick_chunk(g.ny, (int64_t)g.nx * nzr, 1);
+        const int chunk = pick_chunk(g.ny, (int64_t)g.nx * nzr, alone);
         const int waves = ((g.nx + 6) * nzr + XY_TILE - 1) / XY_TILE;
         const dim3 grid((waves + 3) / 4, (g.ny + chunk - 1) / chunk);
         hipLaunchKernelGGL(k_sweep_xy, grid, dim3(64, 4), 0, s, g, A, chunk);
@@ -1178,7 +1182,7 @@ void launch_sweep(const Geom& g, const int dir, const SweepParams& p, const int 
         dim3 grid((g.nx * nzr + 255) / 256, (g.ny + chunk - 1) / chunk);
         hipLaunchKernelGGL((k_sweep_yz<1, MODE_ACC>), grid, dim3(256), 0, s, g, A, chunk);
     } else {
-        const int chunk = pick_chunk(nzr, (int64_t)g.nx * g.ny, 1);
+        const int chunk = pick_chunk(nzr, (int64_t)g.nx * g.ny, alone);
         const dim3 grid((g.nx * g.ny + 255) / 256, (nzr + chunk - 1) / chunk);
         if (stage) hipLaunchKernelGGL((k_sweep_yz<2, MODE_STAGE>), grid, dim3(256), 0, s, g, A, chunk);
         else       hipLaunchKernelGGL((k_sweep_yz<2, MODE_CONV>), grid, dim3(256), 0, s, g, A, chunk);
