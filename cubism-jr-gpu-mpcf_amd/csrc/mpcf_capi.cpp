@@ -229,6 +229,8 @@ extern "C" int mpcf_create(mpcf_ctx** out, int nx, int ny, int nz, double h, uns
     if (e == hipSuccess) e = hipMemsetAsync(c->base, 0, c->bytes, c->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
     if (e != hipSuccess) {
+        if (c->stream) hipStreamDestroy(c->stream);
+        if (c->st_host) hipHostFree(c->st_host);
         hipFree(c->base);
         delete c;
         return fail(MPCF_EHIP, std::string("mpcf_create: ") + hipGetErrorString(e));
@@ -277,7 +279,17 @@ extern "C" int mpcf_set_stream(mpcf_ctx* c, void* stream)
 {
     if (!c) return fail(MPCF_EINVAL, "mpcf_set_stream: NULL context");
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (c->own_stream) HIP_TRY(hipStreamDestroy(c->stream));
+    if (c->own_stream) {
+        // a new stream first: on failure the context keeps its old one
+        if (!stream) {
+            hipStream_t fresh = nullptr;
+            HIP_TRY(hipStreamCreateWithFlags(&fresh, hipStreamNonBlocking));
+            hipStreamDestroy(c->stream);
+            c->stream = fresh;
+            return MPCF_OK;
+        }
+        HIP_TRY(hipStreamDestroy(c->stream));
+    }
     if (stream) {
         c->stream = (hipStream_t)stream;
         c->own_stream = false;
@@ -826,10 +838,8 @@ int batch_enqueue_step(mpcf_batch* b, double cfl, double dt_max)
         if (d.n == 0) continue;
         hipStream_t st = b->stream;
         if (grp > 0) {
-            if (!b->side[grp]) {
-                HIP_TRY(hipStreamCreateWithFlags(&b->side[grp], hipStreamNonBlocking));
-                HIP_TRY(hipEventCreateWithFlags(&b->join[grp], hipEventDisableTiming));
-            }
+            if (!b->side[grp]) HIP_TRY(hipStreamCreateWithFlags(&b->side[grp], hipStreamNonBlocking));
+            if (!b->join[grp]) HIP_TRY(hipEventCreateWithFlags(&b->join[grp], hipEventDisableTiming));
             if (!b->fork) HIP_TRY(hipEventCreateWithFlags(&b->fork, hipEventDisableTiming));
             if (!forked) HIP_TRY(hipEventRecord(b->fork, b->stream));
             forked = true;
@@ -893,11 +903,13 @@ extern "C" int mpcf_batch_destroy(mpcf_batch* b)
     if (b->d_cur) hipFree(b->d_cur);
     if (b->d_off) hipFree(b->d_off);
     for (int grp = 1; grp < kBatchGroups; ++grp)
+    {
         if (b->side[grp]) {
             hipStreamSynchronize(b->side[grp]);
             hipStreamDestroy(b->side[grp]);
-            hipEventDestroy(b->join[grp]);
         }
+        if (b->join[grp]) hipEventDestroy(b->join[grp]);
+    }
     if (b->fork) hipEventDestroy(b->fork);
     hipStreamDestroy(b->stream);
     delete b;

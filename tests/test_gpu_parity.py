@@ -376,6 +376,9 @@ def test_pipeline_on_caller_stream(mpcf):
     try:
         with mpcf.Block(n, n, nz, 1.0 / nz, fast=True, pipeline=4) as a, \
                 mpcf.Block(n, n, nz, 1.0 / nz, fast=True, pipeline=False) as b:
+            own = a.stream
+            a.set_stream(None)              # a fresh stream of its own, made before the old one goes
+            assert a.stream and a.stream != own
             a.set_stream(st.value)
             assert a.stream == st.value
             a.data = s
