@@ -17,9 +17,14 @@ recorded around every launch on the context's stream (mpcf_timing_*); the
 roofline line is for the dominant kernel (the most expensive direction sweep)
 with FLOPs from the library's own operation count of the scheme.
 
-cpu_baseline: the CPU oracle (the restated reference scheme, OpenMP over
-z-slices) timed on this host on a bounded sample (a 160^3 block of the same
-workload, one LSRK3 step), scaled to the same unit.
+cpu_baseline: the CPU oracle (the restated reference scheme in the
+reference's binary32 arithmetic, OpenMP over z-slices) timed on this host on
+bounded samples of the bubble-cloud workload: 64^3 for 10 LSRK3 steps and
+256^3 for one step, each the median of 3 repetitions after a warm-up step,
+and the max-speed-of-sound reduction alone at 256^3; `value` is the 256^3
+rate.  --cpu-full runs the whole plan instead (64^3 x 10, 256^3 x 5 and 512^3
+x 2 steps, medians of 3).  The host's CPU model, nproc, the physical cores of
+the GPU's NUMA node and the threads used are recorded with it.
 
 ensemble / serving (the metric's second half: p50/p99 against the plain
 solver): 64 uneven bubble-collapse cases batched vs one at a time, and
@@ -79,7 +84,8 @@ def parse():
                    help="skip the extra timing of the bit-exact build reported under 'parity_build'")
     p.add_argument("--cfl", type=float, default=0.3)
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    p.add_argument("--cpu-n", type=int, default=160)
+    p.add_argument("--cpu-full", action="store_true",
+                   help="CPU baseline over the whole plan: 64^3 x 10, 256^3 x 5 and 512^3 x 2 steps (minutes)")
     p.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble comparison")
     p.add_argument("--ensemble-cases", type=int, default=64)
     p.add_argument("--ensemble-steps", type=int, default=100)
@@ -95,27 +101,109 @@ def parse():
     return p.parse_args()
 
 
-def cpu_baseline(n: int):
-    """The restated reference scheme (oracle) on this host's cores: one LSRK3
-    step of an n^3 bubble-cloud block.  Returns cell-updates/s (cell x stage)."""
+def host_info():
+    """CPU model, logical CPUs, and the physical cores of the NUMA node the
+    GPU is attached to (sysfs; None where the host does not say)."""
+    info = {"cpu_model": None, "nproc": os.cpu_count(), "gpu_numa_node": None, "socket_physical_cores": None,
+            "node_physical_cores": None}
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                info["cpu_model"] = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+
+    def cores_of(cpus):
+        ids = set()
+        for c in cpus:
+            base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            try:
+                ids.add((open(base + "physical_package_id").read().strip(), open(base + "core_id").read().strip()))
+            except OSError:
+                return None
+        return ids
+
+    def cpulist(text):
+        out = []
+        for part in text.strip().split(","):
+            if "-" in part:
+                a_, b_ = part.split("-")
+                out.extend(range(int(a_), int(b_) + 1))
+            elif part:
+                out.append(int(part))
+        return out
+
+    try:
+        import glob
+        for dev in sorted(glob.glob("/sys/class/drm/card*/device")):
+            vendor = open(os.path.join(dev, "vendor")).read().strip()
+            node = int(open(os.path.join(dev, "numa_node")).read().strip())
+            if vendor == "0x1002" and node >= 0:
+                info["gpu_numa_node"] = node
+                cores = cores_of(cpulist(open(f"/sys/devices/system/node/node{node}/cpulist").read()))
+                if cores is not None:
+                    info["node_physical_cores"] = len(cores)
+                    pkgs = {p for p, _ in cores}
+                    allc = cores_of(range(os.cpu_count() or 0))
+                    if allc is not None:
+                        info["socket_physical_cores"] = len({c for c in allc if c[0] in pkgs})
+                break
+    except (OSError, ValueError):
+        pass
+    return info
+
+
+def cpu_baseline(full: bool):
+    """The restated reference scheme (the CPU oracle: binary32, the
+    reference's operation order, OpenMP over z-slices) on this host's cores.
+    Per size: one warm-up step, then the median of 3 timed repetitions of
+    `steps` LSRK3 steps (3 stages + the max speed of sound each); the
+    speed-of-sound reduction is also timed alone.  Rates are cell-updates/s
+    (cells x stages / s), the unit of the GPU figure."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import statistics
     import pyoracle
     import pympcf
-    ncores = os.cpu_count() or 1
-    nthreads = int(os.environ.get("OMP_NUM_THREADS", ncores))
+    nthreads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     pyoracle.set_threads(nthreads)
-    f, _ = pympcf.ic_bubble_cloud(n, n, n, 50, 0)
-    b = pyoracle.Block(n, n, n)
-    b.data = f
-    h = 1.0 / n
-    b.step(h, 0.3, 1e30, [0] * 6)          # warm-up step
-    t0 = time.perf_counter()
-    b.step(h, 0.3, 1e30, [0] * 6)
-    el = time.perf_counter() - t0
-    return {"value": n ** 3 * 3 / el, "unit": "cell-updates/s", "cores": pyoracle.threads(), "kind": "port",
-            "sample": f"one LSRK3 step (3 stages + max-SOS) of a {n}^3 bubble-cloud block, "
-                      f"oracle (restated reference scheme, binary32, OpenMP) after one warm-up step; "
-                      f"{el:.2f} s"}
+    plan = [(64, 10), (256, 5), (512, 2)] if full else [(64, 10), (256, 1)]
+    sizes = {}
+    sos = None
+    for n, steps in plan:
+        f, _ = pympcf.ic_bubble_cloud(n, n, n, 50, 0)
+        b = pyoracle.Block(n, n, n)
+        b.data = f
+        del f
+        h = 1.0 / n
+        b.step(h, 0.3, 1e30, [0] * 6)          # warm-up step
+        reps = []
+        for _ in range(3):
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                b.step(h, 0.3, 1e30, [0] * 6)
+            reps.append(time.perf_counter() - t0)
+        med = statistics.median(reps)
+        sizes[str(n)] = {"steps": steps, "repetitions_s": reps, "median_s_per_step": med / steps,
+                         "cell_updates_per_s": n ** 3 * 3 * steps / med}
+        log(f"CPU baseline {n}^3 x {steps} steps: median {med / steps:.3f} s/step "
+            f"({n ** 3 * 3 * steps / med:.3e} cell-updates/s)")
+        if n == 256:
+            t = []
+            for _ in range(3):
+                t0 = time.perf_counter()
+                b.max_sos()
+                t.append(time.perf_counter() - t0)
+            sos = {"n": n, "median_s": statistics.median(t), "cells_per_s": n ** 3 / statistics.median(t)}
+        del b
+    top = max(sizes, key=int)
+    return {"value": sizes[top]["cell_updates_per_s"], "unit": "cell-updates/s", "cores": pyoracle.threads(),
+            "kind": "port",
+            "sample": (f"the oracle (restated reference scheme, binary32 in the reference's operation order, "
+                       f"OpenMP over z-slices) on bubble-cloud blocks: "
+                       + ", ".join(f"{n}^3 x {v['steps']} LSRK3 steps" for n, v in sizes.items())
+                       + f", each the median of 3 repetitions after a warm-up step; value = the {top}^3 rate"),
+            "sizes": sizes, "max_sos": sos, "threads": pyoracle.threads(), **host_info()}
 
 
 def ensemble_leg(a):
@@ -214,16 +302,14 @@ def main():
                            pipeline=not a.no_pipeline)
         blk.upload(pympcf.DATA, f)
         blk.upload(pympcf.TMP, np.zeros_like(f))
-        for _ in range(warmup):
-            blk.step(a.cfl, sync=False)
+        blk.steps(warmup, a.cfl)
         blk.synchronize()
         log(f"{warmup} warm-up steps done; timing {steps} steps ({'fast' if fast_build else 'parity'} build)")
         blk.timing(True)
         blk.timing_read()                          # clear
         blk.synchronize()
         t0 = time.perf_counter()
-        for _ in range(steps):
-            blk.step(a.cfl, sync=False)
+        blk.steps(steps, a.cfl)              # K LSRK3 steps, queued at once
         blk.synchronize()
         wall = time.perf_counter() - t0
         ktimes = blk.timing_read()
@@ -273,11 +359,17 @@ def main():
     hbm_stage = cells * BYTES_PER_CELL_STAGE / stage_s / 1e9
 
     kernels = {k: {"avg_ms": (v[0] / v[1] if v[1] else 0.0), "launches": v[1]} for k, v in ktimes.items() if v[1]}
+    # measured bytes per cell come from the committed rocprofv3 summary of the
+    # 512^3 workload (counters cannot be read inside this run); scaled to the
+    # cells of one launch, and only for that size
     traffic = None
-    if os.path.exists(TRAFFIC_FILE):
+    traffic_source = None
+    if n == 512 and os.path.exists(TRAFFIC_FILE):
         tr = json.load(open(TRAFFIC_FILE)).get(a.mode, {}).get(kname[dom])
         if tr is not None:
             traffic = tr * cells_launch
+            traffic_source = ("profiles/traffic_512.json: rocprofv3 FETCH_SIZE + WRITE_SIZE per cell of this "
+                              "kernel at 512^3, calibrated on a copy kernel, times the cells of one launch")
     alg_bytes = KERNEL_BYTES_PER_CELL[a.mode][kname[dom]] * cells_launch
     # the FAST z sweep moves its design bytes at the copy bandwidth: when it is
     # the longer sweep (768^3 and up) it is priced against HBM, every other
@@ -288,7 +380,8 @@ def main():
     else:
         peak, unit = PEAK_FP32_TFLOPS, "TFLOP/s"
     out = {
-        "metric": "RHS cell-updates/s at 512^3 FP32",
+        "metric": ("RHS cell-updates/s at 512^3 FP32; ensemble p50/p99 latency vs plain solver" if n == 512
+                   else f"RHS cell-updates/s at {n}^3 FP32 (the metric is quoted at 512^3)"),
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
@@ -317,6 +410,7 @@ def main():
             "unit": unit,
             "frac": achieved / peak,
             "traffic": traffic,
+            "traffic_source": traffic_source,
             "algorithmic_bytes": alg_bytes,
             "kernel_hbm_GBs": alg_bytes / dom_avg_s / 1e9,
             "flop_per_cell": kflops[dom],
@@ -342,8 +436,8 @@ def main():
     if not a.no_precision:
         out["precision"] = precision_leg(a)
     if not a.no_cpu:
-        log(f"CPU baseline: oracle on a {a.cpu_n}^3 sample")
-        out["cpu_baseline"] = cpu_baseline(a.cpu_n)
+        log("CPU baseline: the oracle on this host" + (" (whole plan)" if a.cpu_full else ""))
+        out["cpu_baseline"] = cpu_baseline(a.cpu_full)
     print(json.dumps(out))
 
 

@@ -132,6 +132,13 @@ int pipeline_slabs(const mpcf_ctx* c)
     return (cells >= (int64_t(1) << 24) && S >= 2) ? S : 0;
 }
 
+// the x+y sweeps of a step's first stage need no dt (the FAST builds store
+// their part unscaled when a = 0, and the z sweeps scale it)
+bool dt_deferred(const mpcf_ctx* c)
+{
+    return (c->flags & MPCF_FLAG_FAST) && !(c->flags & MPCF_FLAG_FP64);
+}
+
 // the three stages of a step over S z-slabs on two streams.  The x+y sweep
 // (the fused one, or the x and the y sweep in turn) of slab j needs the
 // previous stage's z sweep of slab j only (its stencil is in x and y; it also
@@ -141,7 +148,11 @@ int pipeline_slabs(const mpcf_ctx* c)
 // stage (its stencil reaches into the neighbouring slabs, and its output
 // buffer is the one they read).  The x+y sweeps of the next slab run while a
 // z sweep runs: VALU-bound and bandwidth-bound work overlap.
-int run_stages_pipelined(mpcf_ctx* c, const int S)
+// prev_open: the previous step was pipelined with S slabs and left open, so
+// the first stage's x+y sweep of slab j waits for that step's last z sweep of
+// slab j (which wrote the slab's u) instead of the whole step.  open: leave
+// this step open the same way (the caller's stream has not joined it).
+int run_stages_pipelined(mpcf_ctx* c, const int S, const bool prev_open, const bool open)
 {
     if (!c->stream2) HIP_TRY(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
     while (c->pipe_ev.size() < (size_t)(2 * 3 * S)) {
@@ -162,6 +173,7 @@ int run_stages_pipelined(mpcf_ctx* c, const int S)
             p.z0 = (int)((int64_t)c->g.nz * j / S);
             p.z1 = (int)((int64_t)c->g.nz * (j + 1) / S);
             if (s > 0) HIP_TRY(hipStreamWaitEvent(c->stream, ez(s - 1, j), 0));
+            else if (prev_open) HIP_TRY(hipStreamWaitEvent(c->stream, ez(2, j), 0));
             if (split) {
                 { Timed t(c, K_SWEEP_X); c->L->sweep(c->g, 0, p, 1, c->stream); }
                 { Timed t(c, K_SWEEP_Y); c->L->sweep(c->g, 1, p, 1, c->stream); }
@@ -177,12 +189,29 @@ int run_stages_pipelined(mpcf_ctx* c, const int S)
         LAUNCH_CHECK("stage");
         c->cur = 1 - c->cur;
     }
-    // everything after the step is ordered after its last z sweep
-    HIP_TRY(hipStreamWaitEvent(c->stream, ez(2, S - 1), 0));
+    // everything after the step is ordered after its last z sweep (now, or
+    // when the next entry point joins)
+    if (open) c->pipe_open = S;
+    else HIP_TRY(hipStreamWaitEvent(c->stream, ez(2, S - 1), 0));
+    return MPCF_OK;
+}
+
+// the context's stream joins the z sweeps a pipelined step left open
+int join_pipeline(mpcf_ctx* c)
+{
+    if (c->pipe_open) {
+        const int S = c->pipe_open;
+        c->pipe_open = 0;
+        HIP_TRY(hipStreamWaitEvent(c->stream, c->pipe_ev[(2 * S + S - 1) * 2 + 1], 0));   // last z sweep
+    }
     return MPCF_OK;
 }
 
 } // namespace
+
+// message of MPCF_ENUMERIC for a flagged cell (the reference's assertions,
+// GPUkernels.cu:1171-1174,1218, MaxSpeedOfSound.cpp:31-48)
+const char* const kBadCell = "has a non-finite or non-physical cell (NaN / inf, rho <= 0, E <= 0, G <= 0, P < 0 or c <= 0)";
 
 extern "C" const char* mpcf_last_error(void) { return mpcf::detail::last_error(); }
 
@@ -255,6 +284,7 @@ extern "C" int mpcf_destroy(mpcf_ctx* c)
     if (c->stream2) {
         hipStreamSynchronize(c->stream2);
         for (hipEvent_t e : c->pipe_ev) if (e) hipEventDestroy(e);
+        if (c->pipe_fork) hipEventDestroy(c->pipe_fork);
         hipStreamDestroy(c->stream2);
     }
     if (c->own_stream) hipStreamDestroy(c->stream);
@@ -345,6 +375,15 @@ extern "C" int mpcf_halo_export(mpcf_ctx* c, int face, void* dst)
     if (rc != MPCF_OK) return rc;
     c->L->halo(c->g, c->u[c->cur], dst, face, 0, c->stream);
     LAUNCH_CHECK("halo export");
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_ghosts_read(mpcf_ctx* c, int face, void* dst)
+{
+    const int rc = halo_args(c, face, dst, "mpcf_ghosts_read");
+    if (rc != MPCF_OK) return rc;
+    c->L->halo(c->g, c->u[c->cur], dst, face, 2, c->stream);
+    LAUNCH_CHECK("ghosts read");
     return MPCF_OK;
 }
 
@@ -466,6 +505,7 @@ extern "C" int mpcf_max_sos(mpcf_ctx* c, float* sos)
     HIP_TRY(hipMemcpyAsync(c->st_host, c->st, sizeof(StepState), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
     *sos = c->st_host->sos;
+    if (c->st_host->bad) return fail(MPCF_ENUMERIC, std::string("mpcf_max_sos: ") + kBadCell);
     return MPCF_OK;
 }
 
@@ -497,28 +537,60 @@ extern "C" int mpcf_process_all(mpcf_ctx* c, float a, float b, float dtinvh)
     return MPCF_OK;
 }
 
-extern "C" int mpcf_step(mpcf_ctx* c, double cfl, double dt_max, double* dt)
+namespace {
+
+int step_args(mpcf_ctx* c, const double cfl, const double dt_max, const char* who)
 {
-    if (!c) return fail(MPCF_EINVAL, "mpcf_step: NULL context");
-    if (!(cfl > 0) || !std::isfinite(cfl)) return fail(MPCF_EINVAL, "mpcf_step: cfl must be positive and finite");
-    if (!(dt_max > 0)) return fail(MPCF_EINVAL, "mpcf_step: dt_max must be positive");
+    if (!c) return fail(MPCF_EINVAL, std::string(who) + ": NULL context");
+    if (!(cfl > 0) || !std::isfinite(cfl)) return fail(MPCF_EINVAL, std::string(who) + ": cfl must be positive and finite");
+    if (!(dt_max > 0)) return fail(MPCF_EINVAL, std::string(who) + ": dt_max must be positive");
     if (mpcf::detail::has_halo(c->bc))
-        return fail(MPCF_EINVAL, "mpcf_step: halo faces need an exchange before every stage; "
+        return fail(MPCF_EINVAL, std::string(who) + ": halo faces need an exchange before every stage; "
                                  "drive the stages with mpcf_max_sos and mpcf_process_all");
+    return MPCF_OK;
+}
+
+// one LSRK3 step, queued.  keep_open: a pipelined FAST step may leave its
+// last z sweeps unjoined (c->pipe_open) so that the next step's first x+y
+// sweeps overlap them; the caller joins before it returns.
+int step_queue(mpcf_ctx* c, const double cfl, const double dt_max, const bool keep_open)
+{
+    // the sweeps apply the boundary conditions as they load: no ghost fill
+    const int slabs = pipeline_slabs(c);
+    // pipelined FAST steps run dt on the second stream, ahead of the first
+    // stage's z sweeps, while the first stage's x+y sweeps (which need no dt)
+    // start on the context's stream as soon as their slab of the previous
+    // step is done
+    const bool defer = slabs && dt_deferred(c);
+    const bool prev_open = defer && c->pipe_open == slabs;
+    if (!prev_open) {
+        const int rc = join_pipeline(c);
+        if (rc != MPCF_OK) return rc;
+    }
+    hipStream_t sdt = c->stream;
+    if (defer) {
+        if (!c->stream2) HIP_TRY(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
+        if (!c->pipe_fork) HIP_TRY(hipEventCreateWithFlags(&c->pipe_fork, hipEventDisableTiming));
+        // an open previous step orders the second stream already; otherwise
+        // it follows what the context's stream queued (an upload, say)
+        if (!prev_open) {
+            HIP_TRY(hipEventRecord(c->pipe_fork, c->stream));
+            HIP_TRY(hipStreamWaitEvent(c->stream2, c->pipe_fork, 0));
+        }
+        sdt = c->stream2;
+    }
     {
-        Timed t(c, K_SOS);
+        Timed t(c, K_SOS, sdt);
         // the maximum of this state was folded into the previous step's last
         // z sweep unless something else wrote u since: then only dt remains
-        if (c->sos_folded) c->L->sos_folded(c->st, cfl, c->h, dt_max, c->stream);
-        else c->L->sos(c->g, c->u[c->cur], c->st, (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0, 1, cfl, c->h, dt_max,
-                       c->stream);
+        if (c->sos_folded) c->L->sos_folded(c->st, cfl, c->h, dt_max, sdt);
+        else c->L->sos(c->g, c->u[c->cur], c->st, (c->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0, 1, cfl, c->h, dt_max, sdt);
     }
     LAUNCH_CHECK("sos/dt");
     c->sos_folded = false;
-    // the sweeps apply the boundary conditions as they load: no ghost fill
-    const int slabs = pipeline_slabs(c);
+    c->pipe_open = 0;
     if (slabs) {
-        const int rc = run_stages_pipelined(c, slabs);
+        const int rc = run_stages_pipelined(c, slabs, prev_open, defer && keep_open);
         if (rc != MPCF_OK) return rc;
     } else {
         for (int s = 0; s < 3; ++s) {
@@ -528,16 +600,42 @@ extern "C" int mpcf_step(mpcf_ctx* c, double cfl, double dt_max, double* dt)
         }
     }
     c->sos_folded = true;
+    return MPCF_OK;
+}
+
+}  // namespace
+
+extern "C" int mpcf_step(mpcf_ctx* c, double cfl, double dt_max, double* dt)
+{
+    int rc = step_args(c, cfl, dt_max, "mpcf_step");
+    if (rc != MPCF_OK) return rc;
+    rc = step_queue(c, cfl, dt_max, false);
+    if (rc != MPCF_OK) return rc;
     if (dt) {
-        double d;
-        float sos;
-        const int rc = mpcf_last_dt(c, &d, &sos);
-        if (rc != MPCF_OK) return rc;
+        double d = 0;
+        float sos = 0;
+        rc = mpcf_last_dt(c, &d, &sos);
         *dt = d;
+        if (rc != MPCF_OK) return rc;
         if (!(d > 0) || !std::isfinite(d) || !(sos > 0) || !std::isfinite(sos))
             return fail(MPCF_ENUMERIC, "mpcf_step: non-finite or non-positive dt / speed of sound");
     }
     return MPCF_OK;
+}
+
+extern "C" int mpcf_steps(mpcf_ctx* c, int nsteps, double cfl, double dt_max)
+{
+    int rc = step_args(c, cfl, dt_max, "mpcf_steps");
+    if (rc != MPCF_OK) return rc;
+    if (nsteps < 0) return fail(MPCF_EINVAL, "mpcf_steps: nsteps must be >= 0");
+    for (int k = 0; k < nsteps; ++k) {
+        rc = step_queue(c, cfl, dt_max, k + 1 < nsteps);
+        if (rc != MPCF_OK) {
+            join_pipeline(c);
+            return rc;
+        }
+    }
+    return join_pipeline(c);
 }
 
 extern "C" int mpcf_last_dt(mpcf_ctx* c, double* dt, float* sos)
@@ -545,8 +643,14 @@ extern "C" int mpcf_last_dt(mpcf_ctx* c, double* dt, float* sos)
     if (!c) return fail(MPCF_EINVAL, "mpcf_last_dt: NULL context");
     HIP_TRY(hipMemcpyAsync(c->st_host, c->st, sizeof(StepState), hipMemcpyDeviceToHost, c->stream));
     HIP_TRY(hipStreamSynchronize(c->stream));
-    if (dt) *dt = c->st_host->dt;
-    if (sos) *sos = c->st_host->sos;
+    const StepState& h = *c->st_host;
+    if (dt) *dt = h.dt;
+    if (sos) *sos = h.sos;
+    // the state dt was taken from, and (after a step, which folds it) the state
+    // the step wrote
+    if (h.bad) return fail(MPCF_ENUMERIC, std::string("mpcf_last_dt: the state the time step was taken from ") + kBadCell);
+    if ((c->sos_folded || h.folded) && h.next_bad)
+        return fail(MPCF_ENUMERIC, std::string("mpcf_last_dt: the state the last step wrote ") + kBadCell);
     return MPCF_OK;
 }
 

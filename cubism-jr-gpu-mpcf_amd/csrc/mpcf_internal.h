@@ -60,7 +60,8 @@ struct Launchers
 };
 
 #define MPCF_LAUNCHERS(NS)                                                                           \
-    {mpcf::NS::launch_ghosts, mpcf::NS::launch_halo, mpcf::NS::launch_sos, mpcf::NS::launch_sos_folded, mpcf::NS::launch_sweep, mpcf::NS::launch_update, \
+    {mpcf::NS::launch_ghosts, mpcf::NS::launch_halo, mpcf::NS::launch_sos, mpcf::NS::launch_sos_folded,         \
+     mpcf::NS::launch_sweep, mpcf::NS::launch_update,                                                            \
      mpcf::NS::launch_pack, mpcf::NS::launch_stream, mpcf::NS::launch_test_weno,                      \
      mpcf::NS::launch_test_face,                                                                      \
      mpcf::NS::batch_sos, mpcf::NS::batch_xy, mpcf::NS::batch_z}
@@ -168,6 +169,12 @@ struct mpcf_ctx
     hipStream_t stream2;
     std::vector<hipEvent_t> pipe_ev;
     int pipe_req;   // -1 automatic, 0 whole-block stages, k >= 2: k slabs (mpcf_set_pipeline)
+    // slabs of the last pipelined step whose z sweeps the context's stream has
+    // not joined yet (0: none): within one mpcf_steps call, a FAST step leaves
+    // them open so that the next step's first x+y sweeps overlap them; the
+    // call joins them before it returns
+    int pipe_open;
+    hipEvent_t pipe_fork;   // orders the second stream after the context's stream
     // optional per-kernel event timing (mpcf_timing_enable)
     bool timing;
     std::vector<hipEvent_t> ev_pool;

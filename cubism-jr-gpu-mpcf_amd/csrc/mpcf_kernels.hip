@@ -59,15 +59,32 @@ constexpr bool kSosD = MPCF_P64 != 0 || MPCF_F64 != 0;
 typedef std::conditional<kSosD, double, float>::type SosT;
 
 // c + max(|rho u|, |rho v|, |rho w|) / rho of one cell (GPUkernels.cu:1206-1212,
-// or MaxSpeedOfSound.cpp:43-47 with 0.5/rho in binary64 for CPU_FORM)
+// or MaxSpeedOfSound.cpp:43-47 with 0.5/rho in binary64 for CPU_FORM).  bad:
+// the cell fails one of the reference's assertions (rho > 0, E > 0, G > 0,
+// P >= 0, c > 0, no NaN: GPUkernels.cu:1171-1174, MaxSpeedOfSound.cpp:31-48)
+// or its speed is infinite.  Every comparison is false for a NaN, and a NaN
+// in any field reaches p and c, so a NaN cell is always flagged -- the maximum
+// itself drops NaN operands (fmax) and would not show it.
 __device__ __forceinline__ SosT sos_cell(const bool cpu_form, const SosT r, const SosT mu, const SosT mv,
-                                         const SosT mw, const SosT E, const SosT G, const SosT P)
+                                         const SosT mw, const SosT E, const SosT G, const SosT P, bool& bad)
 {
     SosT p;
     if (cpu_form) p = (SosT)((E - (mu*mu + mv*mv + mw*mw)*(0.5/r) - P)/G);
     else          p = (E - (mu*mu + mv*mv + mw*mw)*(SosT(0.5f)/r) - P) / G;
     const SosT c = msqrt(((p + P) / G + p) / r);
-    return c + mmax(mmax(fabs(mu), fabs(mv)), fabs(mw)) / r;
+    const SosT s = c + mmax(mmax(fabs(mu), fabs(mv)), fabs(mw)) / r;
+    const SosT huge = kSosD ? SosT(__DBL_MAX__) : SosT(__FLT_MAX__);
+    bad = !(r > SosT(0) && E > SosT(0) && G > SosT(0) && P >= SosT(0) && c > SosT(0) && s <= huge);
+    return s;
+}
+
+// moves the maximum (and flag) folded into the previous step's last stage to
+// the reduction targets, where sos_finalize reads them
+__device__ __forceinline__ void take_folded(StepState* st)
+{
+    if (kSosD) atomicExch(&st->sos_bits_d, atomicOr(&st->next_sos_bits_d, 0ull));
+    else       atomicExch(&st->sos_bits, atomicOr(&st->next_sos_bits, 0u));
+    atomicExch(&st->bad_bits, atomicOr(&st->next_bad, 0u));
 }
 
 typedef FaceFluxT<Real> FaceFlux;
@@ -83,6 +100,14 @@ constexpr bool kFast = MPCF_FAST != 0;
 // bandwidth to spare, and drops 7 accumulator fields from the z sweep's
 // reads.  The bit-exact builds keep the reference's single update.
 constexpr bool kFold = kFast;
+// In a stage with a = 0 (the first stage of a step) the folded x+y part is
+// stored unscaled, R' = a R - (x+y flux differences), and the z sweep scales
+// it: R = dtinvh * R' - dtinvh (z part).  dtinvh * (-t) rounds to exactly
+// -(dtinvh * t), so the results are bit for bit those of scaling in the x+y
+// sweep -- but the x+y sweep of a step's first stage no longer needs dt, and
+// can run while the previous step's last z sweeps (which fold the maximum
+// speed of sound dt is taken from) are still in flight.
+__device__ __forceinline__ bool unscaled_stage(const Real a) { return kFold && a == Real(0.0f); }
 // accelerator complex dies of an MI355X; workgroups are dispatched to them in
 // turn by linear workgroup id
 constexpr int kXCDs = 8;
@@ -185,7 +210,7 @@ __device__ __forceinline__ void emit_load(const StageArgs& A, const int e, EmitI
 template <int MODE>
 __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, const int e, const EmitIn<MODE>& in,
                                      const FaceFlux& F, const FaceFlux& Fl, const Real cG, const Real cP,
-                                     SosT& smax)
+                                     SosT& smax, bool& bad)
 {
     Real* __restrict__ acc = A.acc;
     const int64_t fs = A.fs;
@@ -201,7 +226,7 @@ __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, cons
 #pragma unroll
         for (int q = 0; q < NVAR; ++q) {
             const Real t = (in.acc[q] + F.f[q]) - Fl.f[q];
-            if (kFold) A.R[q*fs + e] = A.a*in.R[q] - dtinvh*t;
+            if (kFold) A.R[q*fs + e] = unscaled_stage(A.a) ? A.a*in.R[q] - t : A.a*in.R[q] - dtinvh*t;
             else       acc[q*fs + e] = t;
         }
         acc[A_SUMG*fs + e] = in.acc[A_SUMG] + cG;
@@ -222,7 +247,11 @@ __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, cons
             if (q == FG)      rhs = dtinvh*(t - dU * sG * factor6);
             else if (q == FP) rhs = dtinvh*(t - dU * sP * factor6);
             else              rhs = dtinvh*t;
-            const Real Rn = (kFold ? in.R[q] : A.a*in.R[q]) - rhs;
+            // folded: R holds a R - dtinvh (x+y part), or a R - (x+y part)
+            // when unscaled (an explicit product: no contraction into the
+            // subtraction, so it rounds as the x+y sweep's product would)
+            const Real Rx = !kFold ? A.a*in.R[q] : unscaled_stage(A.a) ? __fmul_rn(dtinvh, in.R[q]) : in.R[q];
+            const Real Rn = Rx - rhs;
             A.R[q*fs + e] = Rn;
             if (MODE == MODE_STAGE) {
                 un[q] = A.b*Rn + in.u[q];
@@ -230,8 +259,11 @@ __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, cons
             }
         }
         // the next step's speed of sound, from the values just stored
-        if (MODE == MODE_STAGE && A.sos_next)
-            smax = mmax(smax, sos_cell(A.sos_cpu != 0, un[0], un[1], un[2], un[3], un[4], un[5], un[6]));
+        if (MODE == MODE_STAGE && A.sos_next) {
+            bool b;
+            smax = mmax(smax, sos_cell(A.sos_cpu != 0, un[0], un[1], un[2], un[3], un[4], un[5], un[6], b));
+            bad = bad || b;
+        }
     }
 }
 
@@ -244,6 +276,7 @@ struct March
     Real pGp, pPp;             // plus-state G,P (face k-3/2) of cell k-1
     FaceFlux Fl;               // flux at face k-3/2
     SosT smax = 0;             // running max speed of sound of the emitted cells (folded step)
+    bool bad = false;          // an emitted cell is non-finite or non-physical (folded step)
 
     __device__ __forceinline__ void push(const Prim& q) { w0 = w1; w1 = w2; w2 = w3; w3 = w4; w4 = q; }
 
@@ -257,7 +290,7 @@ struct March
         const CellRec r = reconstruct<Real, kFast>(w0, w1, w2, w3, w4);
         if (flux) {
             const FaceFlux F = riemann<D, Real, kFast>(pm, r.p);
-            if (em) emit<MODE>(A, dtinvh, e_km1, in, F, Fl, pGp + pm.G, pPp + pm.P, smax);
+            if (em) emit<MODE>(A, dtinvh, e_km1, in, F, Fl, pGp + pm.G, pPp + pm.P, smax, bad);
             Fl = F;
         }
         pm = r.m; pGp = r.p.G; pPp = r.p.P;
@@ -280,10 +313,11 @@ __device__ __forceinline__ Real stage_dtinvh(const StageArgs& A)
 // blockIdx.y = chunk of the marched axis.
 // ---------------------------------------------------------------------------
 // one column of a y / z sweep; returns the running max speed of sound of the
-// cells it wrote (zero unless A.sos_next)
+// cells it wrote and sets bad if one of them is non-finite or non-physical
+// (zero / false unless A.sos_next)
 template <int D, int MODE>
 __device__ __forceinline__ SosT sweep_yz_column(const Geom& g, const StageArgs& A, const int chunk, const int col,
-                                                const int bz)
+                                                const int bz, bool& bad)
 {
     const int nd = D == 1 ? g.ny : g.nz;
     const int x = col % g.nx;
@@ -295,7 +329,8 @@ __device__ __forceinline__ SosT sweep_yz_column(const Geom& g, const StageArgs& 
     const int c1 = min(c0 + chunk, D == 2 ? A.z1 : nd);
     const int s = g.stride(D);
     const int e0 = D == 1 ? g.idx(x, 0, t) : g.idx(x, t, 0);
-    const Real dtinvh = stage_dtinvh(A);
+    // the y sweep of a first stage stores its part unscaled and does not read dt
+    const Real dtinvh = (MODE == MODE_ACC && unscaled_stage(A.a)) ? Real(0.0f) : stage_dtinvh(A);
     const int64_t fs = A.fs;
     constexpr bool kRaw = MODE == MODE_STAGE;   // the update needs the raw u of the emitted cell
 
@@ -356,6 +391,7 @@ __device__ __forceinline__ SosT sweep_yz_column(const Geom& g, const StageArgs& 
     } else {
         for (int k = c0 - 1; k <= c1; ++k) step(k >= c0, k >= c0 + 1, k < c1, k);
     }
+    bad = m.bad;
     return m.smax;
 }
 
@@ -368,14 +404,20 @@ __device__ __forceinline__ void sweep_yz_body(const Geom& g, const StageArgs& A,
     const int nt = D == 1 ? A.z1 - A.z0 : g.ny;
     const int col = bcol * 256 + threadIdx.x;
     SosT smax = 0;
-    if (col < g.nx * nt) smax = sweep_yz_column<D, MODE>(g, A, chunk, col, bz);   // no barriers in the body
+    bool bad = false;
+    if (col < g.nx * nt) smax = sweep_yz_column<D, MODE>(g, A, chunk, col, bz, bad);   // no barriers in the body
     if (MODE == MODE_STAGE && A.sos_next) {
-        // one atomic max per wave; every lane of the wave takes part
+        // one atomic max (and, for a bad cell, one atomic or) per wave; every
+        // lane of the wave takes part
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) smax = mmax(smax, __shfl_xor(smax, off, 64));
-        if ((threadIdx.x & 63) == 0 && smax > SosT(0)) {
-            if (kSosD) atomicMax(&A.sos_next->next_sos_bits_d, (unsigned long long)__double_as_longlong((double)smax));
-            else       atomicMax(&A.sos_next->next_sos_bits, __float_as_uint((float)smax));
+        const bool wbad = __ballot(bad) != 0;
+        if ((threadIdx.x & 63) == 0) {
+            if (smax > SosT(0)) {
+                if (kSosD) atomicMax(&A.sos_next->next_sos_bits_d, (unsigned long long)__double_as_longlong((double)smax));
+                else       atomicMax(&A.sos_next->next_sos_bits, __float_as_uint((float)smax));
+            }
+            if (wbad) atomicOr(&A.sos_next->next_bad, 1u);
         }
     }
 }
@@ -630,7 +672,9 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
     fix(w, load_row(A, e0, c0, g.ny, s, 2, w));     m.w4 = to_prim<Real, kFast, kP64>(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
     Real nxt[NVAR];
     Real nsg = load_row(A, e0, c0 + 1, g.ny, s, 2, nxt);
-    const Real dtinvh = stage_dtinvh(A);
+    // a first stage stores its x+y part unscaled and does not read dt
+    const bool unscaled = unscaled_stage(A.a);
+    const Real dtinvh = unscaled ? Real(0.0f) : stage_dtinvh(A);
     // one march step at row k.  FLUX: the face below row k is solved
     // (k >= c0); EMIT: row k-1 is finished and written (k >= c0 + 1); PF: row
     // k+3 is prefetched (k < c1)
@@ -664,7 +708,7 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
 #pragma unroll
                     for (int f = 0; f < NVAR; ++f) {
                         const Real t = ((Fxr.f[f] - Fxl.f[f]) + Fy.f[f]) - m.Fl.f[f];
-                        if (kFold) A.R[f*fs + e] = A.a*Ro[f] - dtinvh*t;
+                        if (kFold) A.R[f*fs + e] = unscaled ? A.a*Ro[f] - t : A.a*Ro[f] - dtinvh*t;
                         else       acc[f*fs + e] = t;
                     }
                     acc[A_SUMG*fs + e] = (rx.p.G + rx.m.G) + cGy;
@@ -760,15 +804,19 @@ void k_ghosts(const Geom g, Real* __restrict__ u, const int64_t fs, const BCSet 
 // GPUlab.cpp:636-641).  The buffer holds the 7 fields one after the other,
 // each in the reference's ghost map order (Types.h:31-69): x faces
 // [NZ][NY][3], y faces [NZ][3][NX], z faces [3][NY][NX], layer l = the l-th
-// of the three cells in increasing index.  EXPORT reads the three boundary
-// cells (0..2 on the low face, n-3..n-1 on the high face), IMPORT writes the
-// ghost band (-3..-1 low, n..n+2 high): a high face's export is the high
-// neighbour's low-face import and vice versa.
+// of the three cells in increasing index.  MODE 0 (export) reads the three
+// boundary cells (0..2 on the low face, n-3..n-1 on the high face), MODE 1
+// (import) writes the ghost band (-3..-1 low, n..n+2 high): a high face's
+// export is the high neighbour's low-face import and vice versa.  MODE 2
+// reads the ghost band back in the same order (the halo buffers GPUlab's
+// load_ghosts leaves, GPUlab.cpp:622-644).
 // ---------------------------------------------------------------------------
-template <int IMPORT>
+template <int MODE>
 __global__ __launch_bounds__(256)
 void k_halo(const Geom g, Real* __restrict__ u, const int64_t fs, const int face, Real* __restrict__ buf)
 {
+    constexpr bool IMPORT = MODE == 1;
+    constexpr bool BAND = MODE != 0;   // the ghost band, not the boundary cells
     const int dir = face >> 1, side = face & 1;
     const int area = dir == 0 ? g.ny * g.nz : dir == 1 ? g.nx * g.nz : g.nx * g.ny;
     const int count = 3 * area;
@@ -779,7 +827,7 @@ void k_halo(const Geom g, Real* __restrict__ u, const int64_t fs, const int face
     else if (dir == 1) { x = t % g.nx;                  layer = (t / g.nx) % 3; z = t / (3 * g.nx); y = 0; }
     else               { x = t % g.nx;                  y = (t / g.nx) % g.ny;  layer = t / area;   z = 0; }
     const int n = g.n(dir);
-    const int cell = IMPORT ? (side == 0 ? layer - 3 : n + layer) : (side == 0 ? layer : n - 3 + layer);
+    const int cell = BAND ? (side == 0 ? layer - 3 : n + layer) : (side == 0 ? layer : n - 3 + layer);
     if (dir == 0) x = cell; else if (dir == 1) y = cell; else z = cell;
     const int e = g.idx(x, y, z);
 #pragma unroll
@@ -820,9 +868,11 @@ __device__ __forceinline__ void sos_finalize(StepState* st, const DtArgs& d)
         atomicExch(&st->sos_bits, 0u);
     }
     st->sos_is_d = kSosD ? 1 : 0;
-    // the folded maximum of the next step starts from zero
+    st->bad = atomicExch(&st->bad_bits, 0u);
+    // the folded maximum (and flag) of the next step starts from zero
     atomicExch(&st->next_sos_bits, 0u);
     atomicExch(&st->next_sos_bits_d, 0ull);
+    atomicExch(&st->next_bad, 0u);
     if (d.with_dt) {
         // binary64 dt from the binary32 (or binary64) maximum
         double dt = d.cfl*d.h/sos;
@@ -837,26 +887,31 @@ __device__ __forceinline__ void sos_finalize(StepState* st, const DtArgs& d)
 
 template <int CPU_FORM>
 __device__ __forceinline__ void sos_body(const Geom& g, const Real* __restrict__ u, const int64_t fs, StepState* st,
-                                         const int b, const int nb, SosT* wmax, const DtArgs& d)
+                                         const int b, const int nb, SosT* wmax, int* wbad, const DtArgs& d)
 {
     const int rows = g.ny * g.nz;
     SosT sos = 0.0f;
+    bool bad = false;
     for (int row = b * 4 + threadIdx.y; row < rows; row += nb * 4) {
         const int e0 = g.idx(0, row % g.ny, row / g.ny);
         for (int x = threadIdx.x; x < g.nx; x += 64) {
             const int e = e0 + x;
+            bool cb;
             sos = mmax(sos, sos_cell(CPU_FORM != 0, u[e], u[fs + e], u[2*fs + e], u[3*fs + e], u[4*fs + e],
-                                     u[5*fs + e], u[6*fs + e]));
+                                     u[5*fs + e], u[6*fs + e], cb));
+            bad = bad || cb;
         }
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) sos = mmax(sos, __shfl_xor(sos, off, 64));
-    if (threadIdx.x == 0) wmax[threadIdx.y] = sos;
+    const int anybad = __ballot(bad) != 0;
+    if (threadIdx.x == 0) { wmax[threadIdx.y] = sos; wbad[threadIdx.y] = anybad; }
     __syncthreads();
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         const SosT m = mmax(mmax(wmax[0], wmax[1]), mmax(wmax[2], wmax[3]));
         if (kSosD) atomicMax(&st->sos_bits_d, (unsigned long long)__double_as_longlong((double)m));
         else       atomicMax(&st->sos_bits, __float_as_uint((float)m));
+        if (wbad[0] | wbad[1] | wbad[2] | wbad[3]) atomicOr(&st->bad_bits, 1u);
         __threadfence();
         if (atomicAdd(&st->sos_count, 1u) == (unsigned)nb - 1u) {
             __threadfence();
@@ -870,8 +925,7 @@ __global__ __launch_bounds__(64)
 void k_sos_folded(StepState* st, const DtArgs d)
 {
     if (threadIdx.x != 0) return;
-    if (kSosD) atomicExch(&st->sos_bits_d, atomicOr(&st->next_sos_bits_d, 0ull));
-    else       atomicExch(&st->sos_bits, atomicOr(&st->next_sos_bits, 0u));
+    take_folded(st);
     __threadfence();
     sos_finalize(st, d);
 }
@@ -881,7 +935,8 @@ __global__ __launch_bounds__(256)
 void k_sos(const Geom g, const Real* __restrict__ u, const int64_t fs, StepState* st, const DtArgs d)
 {
     __shared__ SosT wmax[4];
-    sos_body<CPU_FORM>(g, u, fs, st, blockIdx.x, gridDim.x, wmax, d);
+    __shared__ int wbad[4];
+    sos_body<CPU_FORM>(g, u, fs, st, blockIdx.x, gridDim.x, wmax, wbad, d);
 }
 
 // ---------------------------------------------------------------------------
@@ -1035,6 +1090,7 @@ __global__ __launch_bounds__(256)
 void kb_sos(const BatchDev B, const double cfl, const double dt_max)
 {
     __shared__ SosT wmax[4];
+    __shared__ int wbad[4];
     const int c = find_case(B.off_sos, B.n, blockIdx.x);
     const CaseDesc& C = B.cases[c];
     const DtArgs d{1, cfl, C.h, dt_max};
@@ -1043,14 +1099,13 @@ void kb_sos(const BatchDev B, const double cfl, const double dt_max)
         // the previous step's last z sweep folded this state's maximum: only
         // dt remains, done by the case's first block
         if (local == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
-            if (kSosD) atomicExch(&C.st->sos_bits_d, atomicOr(&C.st->next_sos_bits_d, 0ull));
-            else       atomicExch(&C.st->sos_bits, atomicOr(&C.st->next_sos_bits, 0u));
+            take_folded(C.st);
             __threadfence();
             sos_finalize(C.st, d);
         }
         return;
     }
-    sos_body<CPU_FORM>(C.g, (const Real*)C.u[B.cur[c]], C.g.fstride, C.st, local, C.nb_sos, wmax, d);
+    sos_body<CPU_FORM>(C.g, (const Real*)C.u[B.cur[c]], C.g.fstride, C.st, local, C.nb_sos, wmax, wbad, d);
 }
 
 __global__ __launch_bounds__(256) MPCF_XY_WAVES
@@ -1126,13 +1181,16 @@ void launch_ghosts(const Geom& g, void* uv, const int bc[6], hipStream_t s)
     hipLaunchKernelGGL(k_ghosts, dim3((tmax + 255) / 256, 6), dim3(256), 0, s, g, (Real*)uv, g.fstride, b);
 }
 
-void launch_halo(const Geom& g, void* uv, void* buf, const int face, const int import, hipStream_t s)
+// mode 0: export the boundary cells, 1: import into the ghost band, 2: read the ghost band
+void launch_halo(const Geom& g, void* uv, void* buf, const int face, const int mode, hipStream_t s)
 {
     const int dir = face >> 1;
     const int count = 3 * (dir == 0 ? g.ny * g.nz : dir == 1 ? g.nx * g.nz : g.nx * g.ny);
     const dim3 grid((count + 255) / 256);
-    if (import)
+    if (mode == 1)
         hipLaunchKernelGGL(k_halo<1>, grid, dim3(256), 0, s, g, (Real*)uv, g.fstride, face, (Real*)buf);
+    else if (mode == 2)
+        hipLaunchKernelGGL(k_halo<2>, grid, dim3(256), 0, s, g, (Real*)uv, g.fstride, face, (Real*)buf);
     else
         hipLaunchKernelGGL(k_halo<0>, grid, dim3(256), 0, s, g, (Real*)uv, g.fstride, face, (Real*)buf);
 }

@@ -62,7 +62,7 @@ struct BatchDev
 #define MPCF_DECLARE_LAUNCHERS(NS)                                                                         \
 namespace mpcf { namespace NS {                                                                            \
 void launch_ghosts(const Geom& g, void* u, const int bc[6], hipStream_t s);                                \
-void launch_halo(const Geom& g, void* u, void* buf, int face, int import, hipStream_t s);                  \
+void launch_halo(const Geom& g, void* u, void* buf, int face, int mode, hipStream_t s);                    \
 void launch_sos_folded(StepState* st, double cfl, double h, double dt_max, hipStream_t s);                 \
 void launch_sos(const Geom& g, const void* u, StepState* st, int cpu_form, int with_dt, double cfl,          \
                 double h, double dt_max, hipStream_t s);                                                   \

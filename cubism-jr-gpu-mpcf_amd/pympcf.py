@@ -45,7 +45,7 @@ EXPORTS = [
     "mpcf_read_save_info", "mpcf_dump", "mpcf_save", "mpcf_restart", "mpcf_post_shock_ratio",
     "mpcf_sod_defaults", "mpcf_ic_sod", "mpcf_sbi_defaults", "mpcf_sbi_config", "mpcf_read_bubbles",
     "mpcf_ic_sbi", "mpcf_ic_static", "mpcf_ic_steady", "mpcf_halo_count", "mpcf_halo_export",
-    "mpcf_halo_import", "mpcf_wavelet_dump", "mpcf_wavelet_read", "mpcf_wavelet_encode", "mpcf_wavelet_decode",
+    "mpcf_halo_import", "mpcf_ghosts_read", "mpcf_steps", "mpcf_wavelet_dump", "mpcf_wavelet_read", "mpcf_wavelet_encode", "mpcf_wavelet_decode",
 ]
 
 # streamer kinds of the dumps (mpcf_stream_out)
@@ -180,6 +180,8 @@ def lib():
         L.mpcf_halo_count.argtypes = [P, I, P]
         L.mpcf_halo_export.argtypes = [P, I, P]
         L.mpcf_halo_import.argtypes = [P, I, P]
+        L.mpcf_ghosts_read.argtypes = [P, I, P]
+        L.mpcf_steps.argtypes = [P, I, D, D]
         L.mpcf_wavelet_dump.argtypes = [P, I, C.c_char_p, F, I, P]
         L.mpcf_wavelet_read.argtypes = [C.c_char_p, P, LL, P]
         L.mpcf_wavelet_encode.argtypes = [P, I, F, I, P, LL, P]
@@ -206,9 +208,14 @@ def lib():
     return _LIB
 
 
-def _check(rc):
+def _check(rc, **values):
+    """raise MpcfError for a failed call; `values` (what the call still wrote,
+    e.g. dt with MPCF_ENUMERIC) become attributes of the exception"""
     if rc != OK:
-        raise MpcfError(rc, lib().mpcf_last_error().decode())
+        e = MpcfError(rc, lib().mpcf_last_error().decode())
+        for k, v in values.items():
+            setattr(e, k, v)
+        raise e
 
 
 def _ptr(a):
@@ -382,14 +389,21 @@ class Block:
     def step(self, cfl, dt_max=1e30, sync=True):
         if sync:
             dt = C.c_double()
-            _check(lib().mpcf_step(self._h, float(cfl), float(dt_max), C.cast(C.byref(dt), C.c_void_p)))
+            rc = lib().mpcf_step(self._h, float(cfl), float(dt_max), C.cast(C.byref(dt), C.c_void_p))
+            _check(rc, dt=dt.value)
             return dt.value
         _check(lib().mpcf_step(self._h, float(cfl), float(dt_max), None))
         return None
 
+    def steps(self, nsteps, cfl, dt_max=1e30):
+        """nsteps LSRK3 steps queued at once (asynchronous; consecutive FAST
+        steps overlap); dt of the last one: last_dt()"""
+        _check(lib().mpcf_steps(self._h, int(nsteps), float(cfl), float(dt_max)))
+
     def last_dt(self):
         d, s = C.c_double(), C.c_float()
-        _check(lib().mpcf_last_dt(self._h, C.byref(d), C.byref(s)))
+        rc = lib().mpcf_last_dt(self._h, C.byref(d), C.byref(s))
+        _check(rc, dt=d.value, sos=s.value)
         return d.value, s.value
 
     def synchronize(self):
@@ -405,6 +419,11 @@ class Block:
         """The three boundary cells of `face` into the device buffer at dev_ptr
         (asynchronous on this block's stream)."""
         _check(lib().mpcf_halo_export(self._h, int(face), C.c_void_p(int(dev_ptr))))
+
+    def ghosts_read(self, face, dev_ptr):
+        """The ghost band of `face` into the device buffer at dev_ptr, in the
+        halo buffer order (asynchronous on this block's stream)."""
+        _check(lib().mpcf_ghosts_read(self._h, int(face), C.c_void_p(int(dev_ptr))))
 
     def halo_import(self, face, dev_ptr):
         """The device buffer at dev_ptr into the ghost band of `face`

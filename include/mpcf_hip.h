@@ -31,7 +31,11 @@ enum {
     MPCF_EINVAL = 1,   /* bad argument (null pointer, size, enum value)        */
     MPCF_EHIP = 2,     /* a HIP runtime call failed                             */
     MPCF_ENOMEM = 3,   /* device allocation failed                              */
-    MPCF_ENUMERIC = 4, /* non-finite or non-positive dt / speed of sound        */
+    MPCF_ENUMERIC = 4, /* non-finite or non-positive dt / speed of sound, or a
+                          cell that fails the reference's state assertions:
+                          NaN / inf, rho <= 0, E <= 0, G <= 0, P < 0, c <= 0
+                          (GPUkernels.cu:1171-1174,1218, MaxSpeedOfSound.cpp:31-48);
+                          the kernels flag such cells and finish normally      */
     MPCF_EBUSY = 5     /* a wait timed out with work still outstanding          */
 };
 
@@ -142,9 +146,16 @@ int mpcf_halo_count(mpcf_ctx* ctx, int face, long long* count);
 int mpcf_halo_export(mpcf_ctx* ctx, int face, void* dst);
 int mpcf_halo_import(mpcf_ctx* ctx, int face, const void* src);
 
+/* The ghost band of `face` (the three ghost layers mpcf_load_ghosts or
+ * mpcf_halo_import wrote) into the device buffer dst, in the halo buffer order
+ * above: the values GPUlab::load_ghosts leaves in its halo buffers
+ * (GPUlab.cpp:622-644, Types.h:31-69).  Asynchronous on the context's stream. */
+int mpcf_ghosts_read(mpcf_ctx* ctx, int face, void* dst);
+
 /* Max over the block of c + max(|rho u|,|rho v|,|rho w|)/rho
  * (GPUlab::max_sos / MaxSpeedOfSound_CUDA::compute, GPUlab.cpp:647-700,
- * MaxSpeedOfSound_CUDA.cpp:13-18).  Synchronous. */
+ * MaxSpeedOfSound_CUDA.cpp:13-18).  Synchronous.  MPCF_ENUMERIC (sos still
+ * written) when a cell is non-finite or non-physical. */
 int mpcf_max_sos(mpcf_ctx* ctx, float* sos);
 
 /* R <- a R - dtinvh (div F + quasi-conservative G/P terms), from the current u
@@ -168,7 +179,22 @@ int mpcf_process_all(mpcf_ctx* ctx, float a, float b, float dtinvh);
  * stages.  Writes dt (synchronizes) unless dt is NULL (asynchronous). */
 int mpcf_step(mpcf_ctx* ctx, double cfl, double dt_max, double* dt);
 
-/* dt and max speed of sound of the last mpcf_step (synchronizes). */
+/* nsteps LSRK3 steps, queued asynchronously: the time loop of the reference's
+ * drivers (Sim_*::run calling LSRK3_IntegratorMPI::operator() once per step,
+ * Sim_SteadyStateMPI.cpp:182-222) without a host round trip per step.
+ * Results equal nsteps calls of mpcf_step bit for bit.  In the FAST builds
+ * consecutive pipelined steps overlap: the first stage's x+y sweeps of a step
+ * need no dt, so they start on each z-slab as soon as the previous step's last
+ * z sweep of that slab is done, while that step's remaining slabs and the dt
+ * reduction finish.  When the call returns, the context's stream is ordered
+ * after every step (as after mpcf_step).  dt / sos of the last step:
+ * mpcf_last_dt. */
+int mpcf_steps(mpcf_ctx* ctx, int nsteps, double cfl, double dt_max);
+
+/* dt and max speed of sound of the last mpcf_step (synchronizes).
+ * MPCF_ENUMERIC (dt and sos still written) when the state the step started
+ * from, or the state it wrote, has a non-finite or non-physical cell; with a
+ * dt pointer, mpcf_step returns the same. */
 int mpcf_last_dt(mpcf_ctx* ctx, double* dt, float* sos);
 
 /* Max speed of sound of the last mpcf_step before rounding to binary32: the

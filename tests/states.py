@@ -114,3 +114,24 @@ def random_stencils(n, seed=0, two_phase=True):
     w = rng.uniform(-1, 1, (n, 1)) + 0.3 * pick(1, 1, 1)
     st = conserved(rho, u, v, w, p, G, P)          # (7, n, 6)
     return np.ascontiguousarray(np.transpose(st, (1, 0, 2)))
+
+
+def shock_bubble(nx, ny, nz):
+    """Shock-bubble without noise (the config-2 state): water (rho 1000,
+    p 1), an air bubble of radius 0.1 at the centre (rho 1, p 1), post-shock
+    water (rho 1000, p 1e3, u 10) at x < 0.3, tanh over 1.5 cells.  The
+    squared radius adds the y and z terms first, so the field is exactly
+    symmetric under y <-> z (and under y -> 1 - y, z -> 1 - z)."""
+    x, y, z, h = centres(nx, ny, nz)
+    r = np.sqrt((x - 0.5) ** 2 + ((y - 0.5) ** 2 + (z - 0.5) ** 2))
+    bubble = 0.5 * (1 - np.tanh((r - 0.1) / (1.5 * h)))
+    shock = 0.5 * (1 - np.tanh((x - 0.3) / (1.5 * h)))
+    Ga, Pa = gp(*AIR)
+    Gw, Pw = gp(*WATER)
+    rho = shock * 1000.0 + (1 - shock) * (1.0 * bubble + 1000.0 * (1 - bubble))
+    u = shock * 10.0
+    p = shock * 1000.0 + (1 - shock) * 1.0
+    G = Gw * (1 - bubble) + Ga * bubble
+    P = Pw * (1 - bubble) + Pa * bubble
+    zero = np.zeros_like(rho)
+    return conserved(rho, u, zero, zero, p, G, P)

@@ -1,0 +1,144 @@
+"""The benchmark configurations at (or near) their real sizes, through the C-ABI.
+
+* Config 2 (shock-bubble), bit-exact build against the CPU oracle: 96^3 for
+  the config's full 100 LSRK3 steps, u, R and dt compared with np.array_equal
+  after every step (the stage semantics of GPUlab.cpp:703-793; 96^3 keeps
+  the oracle, a scalar CPU restatement, at about a minute).
+* Config 2 at its full 256^3 x 100 steps in the FAST build, against what the
+  physics fixes for the noise-free state:
+    - the post-shock inflow at x = 0 is supersonic (u = 10 > c = 5.2), so the
+      inflow face keeps its state and the mass, x-momentum and energy change
+      by exactly (inflow flux - outflow flux) x elapsed time (the LSRK3 step
+      integrates a constant boundary flux exactly); y- and z-momentum stay 0;
+    - rho > 0 and p + pc > 0 everywhere (no MPCF_ENUMERIC on any step);
+    - the state stays symmetric under y <-> z (with v <-> w) to round-off.
+* Config 3 (bubble cloud, 50 bubbles) at 128^3 x 50 steps: the three binary32
+  builds against the binary64 build (itself bit-exact against the oracle's
+  binary64 instantiation) with the binary64 run's dt sequence; per-field
+  relative L1 / Linf bounds.
+* Config 4: all 64 ensemble cases (edges 32..128) for 2 steps in one batch,
+  graph-captured, equal to each case run alone bit for bit.
+"""
+import os
+import sys
+import time
+
+import numpy as np
+import pytest
+
+import states
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print("[gpu-configs]", *a, flush=True)
+    sys.stdout.flush()
+
+
+def test_config2_bitexact_every_step(mpcf, oracle):
+    n, steps = 96, 100
+    h = 1.0 / n
+    s = mpcf.ic_shock_bubble(n, n, n, 3)
+    ob = oracle.Block(n, n, n)
+    ob.data = s
+    say(f"config 2 bit-exact {n}^3 x {steps} steps, oracle threads {oracle.threads()}")
+    t0 = time.time()
+    with mpcf.Block(n, n, n, h) as b:
+        b.data = s
+        for k in range(steps):
+            dref, _ = ob.step(h, 0.3, 1e30, [0] * 6)
+            dt = b.step(0.3)
+            assert dt == dref, (k, dt, dref)
+            assert np.array_equal(b.data, ob.data), k
+            assert np.array_equal(b.tmp, ob.tmp), k
+            if k % 10 == 9:
+                say(f"  step {k + 1}: dt {dt:.6e} bit-identical, u and R bit-exact ({time.time() - t0:.0f} s)")
+
+
+def test_config2_full_size_fast_properties(mpcf):
+    n, steps = 256, 100
+    h = 1.0 / n
+    s = states.shock_bubble(n, n, n)
+    # the inflow cell state (uniform over the x = 0 face) and the outflow
+    # state at x = 1 (water at rest)
+    rho_in, mu_in, E_in = (float(s[q, 0, 0, 0]) for q in (0, 1, 4))
+    G_in, P_in = float(s[5, 0, 0, 0]), float(s[6, 0, 0, 0])
+    u_in = mu_in / rho_in
+    p_in = (E_in - 0.5 * mu_in * u_in - P_in) / G_in
+    p_out = (float(s[4, 0, 0, -1]) - float(s[6, 0, 0, -1])) / float(s[5, 0, 0, -1])
+    c_in = np.sqrt((p_in + P_in) / G_in + p_in) / np.sqrt(rho_in)
+    assert u_in > c_in                              # supersonic inflow
+    flux = {0: mu_in, 1: mu_in * u_in + p_in - p_out, 4: u_in * (E_in + p_in)}   # per unit area, x = 0 minus x = 1
+    say(f"config 2 FAST {n}^3 x {steps} steps (noise-free)")
+    with mpcf.Block(n, n, n, h, fast=True) as b:
+        b.data = s
+        I0 = np.array(b.integrals())
+        t = 0.0
+        for k in range(steps):
+            t += b.step(0.3)        # raises MPCF_ENUMERIC on a non-physical cell
+            if k % 20 == 19 or k == steps - 1:
+                I = np.array(b.integrals())
+                for q, f in flux.items():
+                    want = f * t                    # face area 1
+                    assert abs((I[q] - I0[q]) - want) <= 1e-6 * abs(I0[q]) + 1e-5 * abs(want), (k, q, I[q] - I0[q], want)
+                for q in (2, 3):
+                    assert abs(I[q] - I0[q]) <= 1e-6 * flux[1] * t, (k, q, I[q] - I0[q])
+                say(f"  step {k + 1}: t {t:.4e}, mass {I[0] - I0[0]:.6e} (inflow {flux[0] * t:.6e}), "
+                    f"energy {I[4] - I0[4]:.6e} (inflow {flux[4] * t:.6e})")
+        u = b.data
+    r = u[0].astype(np.float64)
+    vel2 = (u[1].astype(np.float64) ** 2 + u[2].astype(np.float64) ** 2 + u[3].astype(np.float64) ** 2) / r ** 2
+    p = (u[4] - 0.5 * r * vel2 - u[6]) / u[5]
+    pc = u[6] / (1.0 + u[5])                        # P = gamma pc / (gamma - 1) = (G + 1) pc
+    assert np.all(r > 0) and np.all(p + pc > 0)
+    # y <-> z mirror: swap the y and z axes and the v and w momenta
+    mirrored = np.swapaxes(u, 1, 2)[[0, 1, 3, 2, 4, 5, 6]]
+    scale = [np.abs(u[0]).max(), np.abs(u[1]).max(), np.abs(u[1]).max(), np.abs(u[1]).max(), np.abs(u[4]).max(),
+             np.abs(u[5]).max(), np.abs(u[6]).max()]
+    worst = max(float(np.abs(mirrored[q].astype(np.float64) - u[q]).max() / scale[q]) for q in range(7))
+    say(f"  y<->z mirror: worst |difference| / field max {worst:.3g}")
+    assert worst < 1e-5, worst
+
+
+def test_config3_precision_128(mpcf):
+    import precision
+    say("config 3 precision 128^3 x 50 steps, 50 bubbles")
+    res = precision.run(n=128, steps=50, nbubbles=50, seed=0, variants=("parity", "fast", "p64"),
+                        log=lambda m: say("  " + m))
+    for name, v in res["variants"].items():
+        l1, linf = v["rel_l1"], v["rel_linf"]
+        for f in ("rho", "E", "G", "P"):
+            assert l1[f] < 2e-6, (name, f, l1[f])
+            assert linf[f] < 1e-3, (name, f, linf[f])
+        for f in ("rho_u", "rho_v", "rho_w"):
+            assert l1[f] < 1e-3, (name, f, l1[f])
+        assert v["dt_own_rel_err_max"] < 1e-5, name
+    # FAST is no less accurate than the reference arithmetic on the
+    # conserved scalars
+    for f in ("rho", "E"):
+        assert res["variants"]["fast"]["rel_l1"][f] <= 1.5 * res["variants"]["parity"]["rel_l1"][f] + 1e-9, f
+
+
+def test_config4_all_64_cases_batch_equals_alone(mpcf):
+    import ensemble
+    cases = ensemble.ensemble_cases(64, steps=2, seed=0)
+    steps = 2
+    say(f"config 4: {len(cases)} cases, edges {sorted(set(c.n for c in cases))}, {steps} steps")
+    fields = [c.fields() for c in cases]
+    with mpcf.Batch(fast=True, graph=True) as B:
+        ids = [B.add(f) for f in fields]
+        dts = []
+        for _ in range(steps):
+            B.step(0.3)
+            dts.append([B.last_dt(i)[0] for i in ids])
+        got = [(B.download(i), B.download(i, mpcf.TMP)) for i in ids]
+    for j, f in enumerate(fields):
+        nz, ny, nx = f.shape[1:]
+        with mpcf.Block(nx, ny, nz, 1.0 / nx, fast=True, split_xy=False) as b:
+            b.data = f
+            d = [b.step(0.3) for _ in range(steps)]
+            assert d == [dts[k][j] for k in range(steps)], j
+            assert np.array_equal(b.data, got[j][0]), j
+            assert np.array_equal(b.tmp, got[j][1]), j
+    say("  all 64 cases bit-identical alone and batched")

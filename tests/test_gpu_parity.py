@@ -320,6 +320,41 @@ def test_pipelined_step_equals_whole_block(mpcf, flags, slabs):
         assert np.array_equal(a.data, b.data) and np.array_equal(a.tmp, b.tmp)
 
 
+@pytest.mark.parametrize("flags,slabs", [(dict(fast=True), 4), (dict(fast=True), 8), (dict(p64=True), 3),
+                                         (dict(), 4), (dict(fast=True), False), (dict(fp64=True), 2)])
+def test_steps_equal_single_steps(mpcf, flags, slabs):
+    """mpcf_steps (consecutive pipelined FAST steps overlapped across the step
+    boundary, dt on the second stream) equals the same number of mpcf_step
+    calls of a whole-block context bit for bit, dt included, also when
+    single steps and multi-step calls alternate"""
+    n, nz = 48, 192
+    s = states.random_smooth(n, n, nz, seed=54)
+    say("mpcf_steps vs mpcf_step", flags, slabs)
+    with mpcf.Block(n, n, nz, 1.0 / nz, pipeline=slabs, **flags) as a, \
+            mpcf.Block(n, n, nz, 1.0 / nz, pipeline=False, **flags) as b:
+        a.data = s
+        b.data = s
+        for chunk in (5, 1, 3):
+            if chunk == 1:
+                da = a.step(0.3)
+            else:
+                a.steps(chunk, 0.3)
+                da = a.last_dt()[0]
+            for _ in range(chunk):
+                db = b.step(0.3)
+            assert da == db, chunk
+            assert np.array_equal(a.data, b.data) and np.array_equal(a.tmp, b.tmp), chunk
+
+
+def test_steps_arguments(mpcf):
+    with mpcf.Block(16, 16, 16, 1.0 / 16, fast=True) as b:
+        b.data = states.uniform(16, 16, 16)
+        b.steps(0, 0.3)
+        for bad in (dict(nsteps=-1, cfl=0.3), dict(nsteps=2, cfl=0.0), dict(nsteps=2, cfl=float("nan"))):
+            with pytest.raises(mpcf.MpcfError):
+                b.steps(**bad)
+
+
 def sweep_launches(mpcf, b, steps=2):
     b.timing(True)
     b.timing_read()
