@@ -211,7 +211,7 @@ int join_pipeline(mpcf_ctx* c)
 
 // message of MPCF_ENUMERIC for a flagged cell (the reference's assertions,
 // GPUkernels.cu:1171-1174,1218, MaxSpeedOfSound.cpp:31-48)
-const char* const kBadCell = "has a non-finite or non-physical cell (NaN / inf, rho <= 0, E <= 0, G <= 0, P < 0 or c <= 0)";
+const char* const kBadCell = "has a non-finite or non-physical cell (NaN / inf, rho <= 0, E <= 0, G <= 0 or c <= 0)";
 
 extern "C" const char* mpcf_last_error(void) { return mpcf::detail::last_error(); }
 

@@ -90,9 +90,9 @@ struct StepState
     // next_sos_bits holds the maximum of the state it wrote (every batched
     // step folds; a case's state is zeroed when it joins a batch)
     int folded;
-    // non-finite or non-physical cells -- rho <= 0, E <= 0, G <= 0, P < 0,
+    // non-finite or non-physical cells -- rho <= 0, E <= 0, G <= 0,
     // c^2 <= 0, or a NaN / inf in any field (the assertions of
-    // GPUkernels.cu:1171-1174,1218 and MaxSpeedOfSound.cpp:31-48): bad_bits
+    // GPUkernels.cu:1171-1173,1218 and MaxSpeedOfSound.cpp:31-48): bad_bits
     // collects the running reduction, bad is the finished reduction's result
     // (the state dt was taken from), next_bad that of the state the last stage
     // of a step wrote (folded with next_sos_bits)

@@ -61,8 +61,11 @@ typedef std::conditional<kSosD, double, float>::type SosT;
 // c + max(|rho u|, |rho v|, |rho w|) / rho of one cell (GPUkernels.cu:1206-1212,
 // or MaxSpeedOfSound.cpp:43-47 with 0.5/rho in binary64 for CPU_FORM).  bad:
 // the cell fails one of the reference's assertions (rho > 0, E > 0, G > 0,
-// P >= 0, c > 0, no NaN: GPUkernels.cu:1171-1174, MaxSpeedOfSound.cpp:31-48)
-// or its speed is infinite.  Every comparison is false for a NaN, and a NaN
+// c > 0, no NaN: GPUkernels.cu:1171-1173, MaxSpeedOfSound.cpp:31-48) or its
+// speed is infinite.  The update's P >= 0 (GPUkernels.cu:1174) is left out:
+// rounding leaves P a few ulp of the neighbouring water's P below zero in air
+// cells next to a sharp interface in one arithmetic and at +0 in another, and
+// a P negative enough to matter drives c^2 <= 0, which is flagged.  Every comparison is false for a NaN, and a NaN
 // in any field reaches p and c, so a NaN cell is always flagged -- the maximum
 // itself drops NaN operands (fmax) and would not show it.
 __device__ __forceinline__ SosT sos_cell(const bool cpu_form, const SosT r, const SosT mu, const SosT mv,
@@ -74,7 +77,7 @@ __device__ __forceinline__ SosT sos_cell(const bool cpu_form, const SosT r, cons
     const SosT c = msqrt(((p + P) / G + p) / r);
     const SosT s = c + mmax(mmax(fabs(mu), fabs(mv)), fabs(mw)) / r;
     const SosT huge = kSosD ? SosT(__DBL_MAX__) : SosT(__FLT_MAX__);
-    bad = !(r > SosT(0) && E > SosT(0) && G > SosT(0) && P >= SosT(0) && c > SosT(0) && s <= huge);
+    bad = !(r > SosT(0) && E > SosT(0) && G > SosT(0) && c > SosT(0) && s <= huge);
     return s;
 }
 

@@ -33,8 +33,8 @@ enum {
     MPCF_ENOMEM = 3,   /* device allocation failed                              */
     MPCF_ENUMERIC = 4, /* non-finite or non-positive dt / speed of sound, or a
                           cell that fails the reference's state assertions:
-                          NaN / inf, rho <= 0, E <= 0, G <= 0, P < 0, c <= 0
-                          (GPUkernels.cu:1171-1174,1218, MaxSpeedOfSound.cpp:31-48);
+                          NaN / inf, rho <= 0, E <= 0, G <= 0, c <= 0
+                          (GPUkernels.cu:1171-1173,1218, MaxSpeedOfSound.cpp:31-48);
                           the kernels flag such cells and finish normally      */
     MPCF_EBUSY = 5     /* a wait timed out with work still outstanding          */
 };

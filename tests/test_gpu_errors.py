@@ -1,7 +1,7 @@
 """Error paths and the ghost band, through the C-ABI on the device.
 
 * A cell that fails the reference's state assertions -- NaN, rho <= 0,
-  E <= 0, G <= 0, P < 0, c <= 0 (GPUkernels.cu:1171-1174,1218,
+  E <= 0, G <= 0, c <= 0 (GPUkernels.cu:1171-1173,1218,
   MaxSpeedOfSound.cpp:31-48) -- is flagged by the speed-of-sound reduction
   and by the reduction folded into the last stage of a step, and surfaces as
   MPCF_ENUMERIC from mpcf_max_sos, mpcf_step and mpcf_last_dt.  The kernels
@@ -45,7 +45,7 @@ def bad_cells(s):
         p = (E - (mu * mu + mv * mv + mw * mw) * (np.float32(0.5) / r) - P) / G
         c = np.sqrt(((p + P) / G + p) / r)
         sp = c + np.maximum(np.maximum(np.abs(mu), np.abs(mv)), np.abs(mw)) / r
-        ok = (r > 0) & (E > 0) & (G > 0) & (P >= 0) & (c > 0) & (sp <= np.finfo(np.float32).max)
+        ok = (r > 0) & (E > 0) & (G > 0) & (c > 0) & (sp <= np.finfo(np.float32).max)
     return ~ok
 
 
@@ -63,7 +63,11 @@ def poison(s, kind):
     elif kind == "zero_G":
         out[5, z, y, x] = 0.0
     elif kind == "negative_P":
-        out[6, z, y, x] = -1.0
+        # P below -G p (G + 1) / ... : negative enough that c^2 = ((p + P)/G + p)/rho < 0
+        r, mu, mv, mw, E, G, P = (float(out[q, z, y, x]) for q in range(7))
+        p = (E - 0.5 * (mu * mu + mv * mv + mw * mw) / r - P) / G
+        out[6, z, y, x] = -2.0 * p * (G + 1.0) - 1.0
+        out[4, z, y, x] = G * p + out[6, z, y, x] + 0.5 * (mu * mu + mv * mv + mw * mw) / r
     elif kind == "negative_c2":
         # p far below -pc: ((p + P)/G + p) < 0
         out[4, z, y, x] = out[6, z, y, x] - 50.0 * out[5, z, y, x] * 5000.0
