@@ -12,6 +12,7 @@ Objects:
   f64.o     mpcf_kernels.hip, MPCF_F64=1, -ffp-contract=off (the scheme in binary64)
   reduce.o  mpcf_reduce.hip  (binary64 block comparisons and integrals)
   wavelet.o mpcf_wavelet.hip (wavelet-compressed dumps; linked with zlib)
+  noalloc.o mpcf_noalloc.hip (the allocation-free speed-of-sound / dt entry point)
   capi.o    the C-ABI (include/mpcf_hip.h)
   ic.o      initial conditions;  sic.o  the shock-induced cloud input and IC
   sim_ic.o  the Sod, 2D shock-bubble and static-index drivers' ICs (-ffp-contract=off)
@@ -72,6 +73,7 @@ def build(verbose: bool = False) -> str:
         ("f64.o", "mpcf_kernels.hip", ["-DMPCF_FAST=0", "-DMPCF_F64=1", "-ffp-contract=off"]),
         ("reduce.o", "mpcf_reduce.hip", ["-ffp-contract=off"]),
         ("wavelet.o", "mpcf_wavelet.hip", ["-ffp-contract=off"]),
+        ("noalloc.o", "mpcf_noalloc.hip", []),
         ("capi.o", "mpcf_capi.cpp", ["-x", "hip"]),
         ("serve.o", "mpcf_serve.cpp", ["-x", "hip"]),
         ("ic.o", "ic.cpp", ["-x", "hip"]),

@@ -48,6 +48,7 @@ struct Launchers
     decltype(&mpcf::parity::launch_halo) halo;
     decltype(&mpcf::parity::launch_sos) sos;
     decltype(&mpcf::parity::launch_sos_folded) sos_folded;
+    decltype(&mpcf::parity::launch_sos_dense) sos_dense;
     decltype(&mpcf::parity::launch_sweep) sweep;
     decltype(&mpcf::parity::launch_update) update;
     decltype(&mpcf::parity::launch_pack) pack;
@@ -61,6 +62,7 @@ struct Launchers
 
 #define MPCF_LAUNCHERS(NS)                                                                           \
     {mpcf::NS::launch_ghosts, mpcf::NS::launch_halo, mpcf::NS::launch_sos, mpcf::NS::launch_sos_folded,         \
+     mpcf::NS::launch_sos_dense,                                                                                \
      mpcf::NS::launch_sweep, mpcf::NS::launch_update,                                                            \
      mpcf::NS::launch_pack, mpcf::NS::launch_stream, mpcf::NS::launch_test_weno,                      \
      mpcf::NS::launch_test_face,                                                                      \

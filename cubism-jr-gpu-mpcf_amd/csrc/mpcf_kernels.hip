@@ -888,15 +888,19 @@ __device__ __forceinline__ void sos_finalize(StepState* st, const DtArgs& d)
     atomicExch(&st->sos_count, 0u);
 }
 
-template <int CPU_FORM>
-__device__ __forceinline__ void sos_body(const Geom& g, const Real* __restrict__ u, const int64_t fs, StepState* st,
+// the reduction over the rows of a block, blocks b = 0..nb-1 of it in this
+// launch; returns true in the one thread that finalized (thread (0, 0) of the
+// last block to finish).  DENSE: the fields are in the host layout, cell
+// (x, y, z) at x + nx*(y + ny*z), instead of the padded one
+template <int CPU_FORM, bool DENSE = false>
+__device__ __forceinline__ bool sos_body(const Geom& g, const Real* __restrict__ u, const int64_t fs, StepState* st,
                                          const int b, const int nb, SosT* wmax, int* wbad, const DtArgs& d)
 {
     const int rows = g.ny * g.nz;
     SosT sos = 0.0f;
     bool bad = false;
     for (int row = b * 4 + threadIdx.y; row < rows; row += nb * 4) {
-        const int e0 = g.idx(0, row % g.ny, row / g.ny);
+        const int e0 = DENSE ? row * g.nx : g.idx(0, row % g.ny, row / g.ny);
         for (int x = threadIdx.x; x < g.nx; x += 64) {
             const int e = e0 + x;
             bool cb;
@@ -919,8 +923,10 @@ __device__ __forceinline__ void sos_body(const Geom& g, const Real* __restrict__
         if (atomicAdd(&st->sos_count, 1u) == (unsigned)nb - 1u) {
             __threadfence();
             sos_finalize(st, d);
+            return true;
         }
     }
+    return false;
 }
 
 // dt from the maximum folded into the previous step's last z sweep
@@ -940,6 +946,25 @@ void k_sos(const Geom g, const Real* __restrict__ u, const int64_t fs, StepState
     __shared__ SosT wmax[4];
     __shared__ int wbad[4];
     sos_body<CPU_FORM>(g, u, fs, st, blockIdx.x, gridDim.x, wmax, wbad, d);
+}
+
+// the same reduction over fields in the host layout (cell x + nx*(y + ny*z),
+// field q at u + q*fs), with st a caller-provided reduction state that must
+// hold zeros in sos_bits[_d], bad_bits and sos_count; the finalizing thread
+// copies the maximum (binary64), dt and the flag to the outputs (any may be
+// null)
+template <int CPU_FORM>
+__global__ __launch_bounds__(256)
+void k_sos_dense(const Geom g, const Real* __restrict__ u, const int64_t fs, StepState* st, const DtArgs d,
+                 double* sos_out, double* dt_out, int* bad_out)
+{
+    __shared__ SosT wmax[4];
+    __shared__ int wbad[4];
+    if (sos_body<CPU_FORM, true>(g, u, fs, st, blockIdx.x, gridDim.x, wmax, wbad, d)) {
+        if (sos_out) *sos_out = st->sos_d;
+        if (dt_out) *dt_out = st->dt;
+        if (bad_out) *bad_out = (int)st->bad;
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -1206,6 +1231,20 @@ void launch_sos(const Geom& g, const void* uv, StepState* st, const int cpu_form
     const DtArgs d{with_dt, cfl, h, dt_max};
     if (cpu_form) hipLaunchKernelGGL(k_sos<1>, dim3(nb), dim3(64, 4), 0, s, g, u, g.fstride, st, d);
     else          hipLaunchKernelGGL(k_sos<0>, dim3(nb), dim3(64, 4), 0, s, g, u, g.fstride, st, d);
+}
+
+void launch_sos_dense(const Geom& g, const void* uv, const int64_t fs, StepState* st, const int cpu_form,
+                      const double cfl, const double h, const double dt_max, double* sos_out, double* dt_out,
+                      int* bad_out, hipStream_t s)
+{
+    const Real* u = (const Real*)uv;
+    const int rows = (g.ny * g.nz + 3) / 4;
+    const int nb = rows < 2048 ? rows : 2048;
+    const DtArgs d{1, cfl, h, dt_max};
+    if (cpu_form)
+        hipLaunchKernelGGL(k_sos_dense<1>, dim3(nb), dim3(64, 4), 0, s, g, u, fs, st, d, sos_out, dt_out, bad_out);
+    else
+        hipLaunchKernelGGL(k_sos_dense<0>, dim3(nb), dim3(64, 4), 0, s, g, u, fs, st, d, sos_out, dt_out, bad_out);
 }
 
 void launch_sos_folded(StepState* st, const double cfl, const double h, const double dt_max, hipStream_t s)

@@ -64,6 +64,8 @@ namespace mpcf { namespace NS {                                                 
 void launch_ghosts(const Geom& g, void* u, const int bc[6], hipStream_t s);                                \
 void launch_halo(const Geom& g, void* u, void* buf, int face, int mode, hipStream_t s);                    \
 void launch_sos_folded(StepState* st, double cfl, double h, double dt_max, hipStream_t s);                 \
+void launch_sos_dense(const Geom& g, const void* u, int64_t fs, StepState* st, int cpu_form, double cfl,   \
+                      double h, double dt_max, double* sos_out, double* dt_out, int* bad_out, hipStream_t s); \
 void launch_sos(const Geom& g, const void* u, StepState* st, int cpu_form, int with_dt, double cfl,          \
                 double h, double dt_max, hipStream_t s);                                                   \
 void launch_sweep(const Geom& g, int dir, const SweepParams& p, int stage, hipStream_t s);                \

@@ -153,6 +153,7 @@ def lib():
         L.mpcf_upload_f64.argtypes = [P, I, P]
         L.mpcf_download_f64.argtypes = [P, I, P]
         L.mpcf_last_sos64.argtypes = [P, C.POINTER(D)]
+        L.mpcf_max_sos_noalloc.argtypes = [P, C.c_longlong, I, I, I, D, D, D, C.c_uint, P, P, P, P]
         L.mpcf_compare.argtypes = [P, P, I, P]
         L.mpcf_integrals.argtypes = [P, I, P]
         LL = C.c_longlong

@@ -202,6 +202,31 @@ int mpcf_last_dt(mpcf_ctx* ctx, double* dt, float* sos);
  * value widened otherwise).  Synchronizes. */
 int mpcf_last_sos64(mpcf_ctx* ctx, double* sos);
 
+/* Allocation-free speed of sound and dt.  No counterpart in the reference:
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The reduction of mpcf_max_sos, and the dt that mpcf_step takes from it
+ * (dt = min(cfl h / sos, dt_max) in binary64), over a state the caller keeps
+ * in its own device memory in the host layout: field q of cell (ix,iy,iz) at
+ * element q*field_stride + ix + nx*(iy + ny*iz) of u (binary32, or binary64
+ * with MPCF_FLAG_FP64).  flags: MPCF_FLAG_FAST, MPCF_FLAG_SOS_CPU,
+ * MPCF_FLAG_P64, MPCF_FLAG_FP64, with the meaning they have for a context;
+ * the results equal mpcf_max_sos / mpcf_last_sos64 and mpcf_step's dt for
+ * the same state and flags, bit for bit.  Outputs are caller device memory
+ * (any may be NULL): sos_out[0] the maximum (binary64; the binary32 maximum
+ * widened in the binary32 builds), dt_out[0], bad_out[0] nonzero when a cell
+ * fails the state assertions of MPCF_ENUMERIC.  Asynchronous on hip_stream
+ * (NULL: the null stream) of the current device.
+ * It calls no hipMalloc or hipFree: the reduction's working state (256 bytes)
+ * is a static device array of the library, one per GPU, located once per
+ * device on the first call there.  Each call clears it with a kernel (no
+ * memset node), so the call can be graph-captured.  Calls that overlap on one
+ * GPU race on that array: order them.  Limits: nx*ny*nz <= 2^31 - 1 cells,
+ * field_stride >= nx*ny*nz; larger states are refused with MPCF_EINVAL. */
+int mpcf_max_sos_noalloc(const void* u, long long field_stride, int nx, int ny, int nz, double h, double cfl,
+                         double dt_max, unsigned flags, double* sos_out, double* dt_out, int* bad_out,
+                         void* hip_stream);
+
 /* Field-wise difference of two blocks of equal extent on one device, in
  * binary64 (any storage precision on either side): out[4q + k] for field q
  * is k=0: sum |a-b|, 1: max |a-b|, 2: sum |b|, 3: max |b| over the interior

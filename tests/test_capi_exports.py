@@ -44,3 +44,20 @@ def test_error_path_without_device_work(mpcf):
     assert lib.mpcf_create(ctypes.byref(h), 8, 8, 8, 0.1, 64) == mpcf.EINVAL  # unknown flag
     assert lib.mpcf_set_bc(None, None) == mpcf.EINVAL
     assert lib.mpcf_step(None, 0.3, 1.0, None) == mpcf.EINVAL
+
+
+def test_noalloc_refuses_bad_arguments_before_device_work(mpcf):
+    """mpcf_max_sos_noalloc checks its arguments before it touches a device:
+    a null state, a state beyond 2^31 - 1 cells, a field stride below the
+    cell count and unknown flags are refused with MPCF_EINVAL (the pointer
+    below is never dereferenced: every call fails its checks first)."""
+    lib = mpcf.lib()
+    fake = 4096
+    assert lib.mpcf_max_sos_noalloc(None, 8, 2, 2, 2, 0.1, 0.3, 1.0, 0, None, None, None, None) == mpcf.EINVAL
+    assert b"NULL" in lib.mpcf_last_error()
+    rc = lib.mpcf_max_sos_noalloc(fake, 1 << 33, 2048, 2048, 1024, 0.1, 0.3, 1.0, 0, None, None, None, None)
+    assert rc == mpcf.EINVAL
+    assert b"2^31" in lib.mpcf_last_error()
+    assert lib.mpcf_max_sos_noalloc(fake, 7, 2, 2, 2, 0.1, 0.3, 1.0, 0, None, None, None, None) == mpcf.EINVAL
+    assert lib.mpcf_max_sos_noalloc(fake, 8, 2, 2, 2, 0.1, 0.3, 1.0, 4, None, None, None, None) == mpcf.EINVAL
+    assert lib.mpcf_max_sos_noalloc(fake, 8, 2, 2, 2, 0.0, 0.3, 1.0, 0, None, None, None, None) == mpcf.EINVAL
