@@ -11,6 +11,7 @@
 // PCIe until mpcf_download, and dt is produced and consumed on the device.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <functional>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -794,6 +795,7 @@ extern "C" int mpcf_test_face(int n, int dir, const float* st, float* out, unsig
 // device tables that the batch kernels read.
 // ===========================================================================
 #include <algorithm>
+#include <functional>
 #include <map>
 
 struct mpcf_batch
@@ -827,8 +829,21 @@ struct mpcf_batch
     hipGraphExec_t exec = nullptr;
     long long captures = 0;
     long long steps = 0;
-    // autotuned march length by case extent
-    std::map<int, int> tuned_len;
+    // autotuned march lengths of the x+y sweep by NY and of the z sweep by NZ
+    std::map<int, int> tuned_xy, tuned_z;
+    // how a step is launched: MPCF_BATCH_MODE_PERSISTENT -- per stage, two
+    // persistent launches (x+y items, then z items) whose workgroups pull
+    // (case, tile) items from a device queue, all on the batch's stream;
+    // MPCF_BATCH_MODE_GROUPS -- one flat launch per kernel kind and case group,
+    // the groups on their own streams
+    int mode = MPCF_BATCH_MODE_GROUPS;
+    int p_threads = 256;                     // workgroup size of the persistent kernel
+    int p_blocks[2] = {0, 0};                // grids of the x+y and the z launch
+    mpcf::PSeg* d_seg = nullptr;             // queue segments (2 per case + 1)
+    unsigned* d_done = nullptr;              // per case: finished z items of the running stage
+    mpcf::PCtl* d_ctl = nullptr;             // queue head, workgroups done
+    int p_cap = 0;
+    mpcf::PersistDev pdev{};
 };
 
 namespace {
@@ -859,10 +874,64 @@ int batch_pull_cur(mpcf_batch* b)
 int march_len(const mpcf_batch* b, int i, int nd, int which, long long columns)
 {
     const int forced = which == 0 ? b->chunk_xy[i] : b->chunk_z[i];
-    auto it = b->tuned_len.find(nd);
-    const int tuned = it != b->tuned_len.end() ? it->second : 0;
+    const std::map<int, int>& tuned_map = which == 0 ? b->tuned_xy : b->tuned_z;
+    auto it = tuned_map.find(nd);
+    const int tuned = it != tuned_map.end() ? it->second : 0;
     if (forced > 0 || tuned > 0) return mpcf::detail::batch_march(nd, forced, tuned);
     return mpcf::detail::adaptive_march(nd, columns);
+}
+
+// the queues of the persistent stage launches: the x+y items of every case,
+// largest case first, then their z items in the same order
+int persist_tables(mpcf_batch* b, std::vector<mpcf::CaseDesc>& desc)
+{
+    const int n = (int)desc.size();
+    if (n > b->p_cap) {
+        if (b->d_seg) hipFree(b->d_seg);
+        if (b->d_done) hipFree(b->d_done);
+        if (b->d_ctl) hipFree(b->d_ctl);
+        b->d_seg = nullptr; b->d_done = nullptr; b->d_ctl = nullptr;
+        const int cap = std::max(16, 2 * n);
+        hipError_t e = hipMalloc((void**)&b->d_seg, (2 * cap + 1) * sizeof(mpcf::PSeg));
+        if (e == hipSuccess) e = hipMalloc((void**)&b->d_done, cap * sizeof(unsigned));
+        if (e == hipSuccess) e = hipMalloc((void**)&b->d_ctl, sizeof(mpcf::PCtl));
+        if (e != hipSuccess) return fail(MPCF_ENOMEM, std::string("batch queue tables: ") + hipGetErrorString(e));
+        b->p_cap = cap;
+    }
+    const int W = b->p_threads / 64;
+    for (mpcf::CaseDesc& d : desc) mpcf::detail::case_items(d, W);
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    auto cells = [&](int i) { return (long long)desc[i].g.nx * desc[i].g.ny * desc[i].g.nz; };
+    std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return cells(x) > cells(y); });
+    std::vector<mpcf::PSeg> seg;
+    unsigned total = 0, items[2] = {0, 0};
+    for (int kind = 0; kind < 2; ++kind) {
+        const unsigned start = total;
+        for (int k = 0; k < n; ++k) {
+            const int c = order[k];
+            seg.push_back({c, kind, total});
+            total += kind == 0 ? desc[c].items_xy : desc[c].items_z;
+        }
+        items[kind] = total - start;
+    }
+    const int nseg = (int)seg.size();
+    seg.push_back({-1, 0, total});
+    HIP_TRY(hipMemcpyAsync(b->d_seg, seg.data(), seg.size() * sizeof(mpcf::PSeg), hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemsetAsync(b->d_done, 0, n * sizeof(unsigned), b->stream));
+    HIP_TRY(hipMemsetAsync(b->d_ctl, 0, sizeof(mpcf::PCtl), b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->cases[0]->device));
+    for (int ph = 0; ph < 2; ++ph) {
+        const int occ = std::max(1, b->L->persist_occ(b->p_threads, ph));
+        b->p_blocks[ph] = (int)std::max(1LL, std::min((long long)occ * cus, (long long)items[ph]));
+    }
+    mpcf::PersistDev& P = b->pdev;
+    P.cases = b->d_cases; P.cur = b->d_cur; P.seg = b->d_seg; P.nseg = nseg; P.nxy = n;
+    P.done = b->d_done; P.ctl = b->d_ctl;
+    P.nblocks = 0;
+    return MPCF_OK;
 }
 
 int batch_rebuild(mpcf_batch* b)
@@ -912,6 +981,10 @@ int batch_rebuild(mpcf_batch* b)
         os[j + 1] = os[j] + blocks[2];
         cur[i] = c->cur;
     }
+    if (b->mode == MPCF_BATCH_MODE_PERSISTENT && n > 0) {
+        const int prc = persist_tables(b, desc);   // fills the descriptors' item counts
+        if (prc != MPCF_OK) return prc;
+    }
     if (n > 0) {
         HIP_TRY(hipMemcpyAsync(b->d_cases, desc.data(), n * sizeof(mpcf::CaseDesc), hipMemcpyHostToDevice, b->stream));
         HIP_TRY(hipMemcpyAsync(b->d_cur, cur.data(), n * sizeof(int), hipMemcpyHostToDevice, b->stream));
@@ -933,9 +1006,18 @@ int batch_rebuild(mpcf_batch* b)
 
 int batch_enqueue_step(mpcf_batch* b, double cfl, double dt_max)
 {
+    const int cpu_form = (b->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0;
+    if (b->mode == MPCF_BATCH_MODE_PERSISTENT) {
+        // dt of every case (from the maximum the previous step folded, or a
+        // full reduction), then one persistent launch per stage
+        for (int grp = 0; grp < kBatchGroups; ++grp)
+            if (b->dev[grp].n > 0) b->L->b_sos(b->dev[grp], b->tot_sos[grp], cpu_form, cfl, dt_max, b->stream);
+        for (int s = 0; s < 3; ++s)
+            b->L->persist(b->pdev, b->p_blocks, b->p_threads, kA[s], kB[s], s == 2, cpu_form, b->stream);
+        return MPCF_OK;
+    }
     // the sweeps apply each case's boundary conditions as they load: a step is
     // seven launches per group; groups 1.. run on side streams
-    const int cpu_form = (b->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0;
     bool forked = false;
     for (int grp = kBatchGroups - 1; grp >= 0; --grp) {
         const mpcf::BatchDev& d = b->dev[grp];
@@ -968,6 +1050,7 @@ int find_case_index(const mpcf_batch* b, int id)
         if (b->ids[i] == id) return (int)i;
     return -1;
 }
+
 
 } // namespace
 
@@ -1006,6 +1089,9 @@ extern "C" int mpcf_batch_destroy(mpcf_batch* b)
     if (b->d_cases) hipFree(b->d_cases);
     if (b->d_cur) hipFree(b->d_cur);
     if (b->d_off) hipFree(b->d_off);
+    if (b->d_seg) hipFree(b->d_seg);
+    if (b->d_done) hipFree(b->d_done);
+    if (b->d_ctl) hipFree(b->d_ctl);
     for (int grp = 1; grp < kBatchGroups; ++grp)
     {
         if (b->side[grp]) {
@@ -1160,15 +1246,121 @@ extern "C" int mpcf_batch_query(mpcf_batch* b, long long out[6])
     out[2] = bytes;
     out[3] = b->captures;
     out[4] = b->steps;
-    out[5] = b->tuned_len.size();
+    std::map<int, int> ext(b->tuned_xy);
+    ext.insert(b->tuned_z.begin(), b->tuned_z.end());
+    out[5] = (long long)ext.size();
     return MPCF_OK;
 }
 
-// Autotuner of the march length (how many cells one thread marches in the
-// x+y and z sweeps before handing over to the next chunk): for each candidate
-// length, time `steps` batch steps of the live cases from a saved state and
-// keep the fastest; the state is restored afterwards, so tuning does not
-// advance the cases.  Candidates: 16, 32, 64, 128 and whole columns.
+extern "C" int mpcf_batch_set_mode(mpcf_batch* b, int mode, int threads)
+{
+    if (!b) return fail(MPCF_EINVAL, "mpcf_batch_set_mode: NULL batch");
+    if (mode != MPCF_BATCH_MODE_GROUPS && mode != MPCF_BATCH_MODE_PERSISTENT)
+        return fail(MPCF_EINVAL, "mpcf_batch_set_mode: mode must be MPCF_BATCH_MODE_GROUPS or MPCF_BATCH_MODE_PERSISTENT");
+    if (threads == 0) threads = 256;
+    if (threads != 64 && threads != 128 && threads != 256)
+        return fail(MPCF_EINVAL, "mpcf_batch_set_mode: threads must be 64, 128 or 256");
+    if (b->mode != mode || b->p_threads != threads) {
+        const int rc = batch_pull_cur(b);
+        if (rc != MPCF_OK) return rc;
+        b->mode = mode;
+        b->p_threads = threads;
+        b->dirty = true;
+    }
+    return MPCF_OK;
+}
+
+extern "C" int mpcf_batch_tuning(mpcf_batch* b, int extent, int out[4])
+{
+    if (!b || !out) return fail(MPCF_EINVAL, "mpcf_batch_tuning: NULL argument");
+    out[0] = b->mode;
+    out[1] = b->p_threads;
+    auto ixy = b->tuned_xy.find(extent);
+    auto iz = b->tuned_z.find(extent);
+    out[2] = ixy != b->tuned_xy.end() ? ixy->second : 0;
+    out[3] = iz != b->tuned_z.end() ? iz->second : 0;
+    return MPCF_OK;
+}
+
+// Autotuner: the launch mode and the persistent workgroup size, then per case
+// extent the two march lengths (how many cells one wave marches in the x+y
+// and z sweeps before the next chunk takes over), by coordinate descent.
+// Every candidate is timed over `steps` batch steps after a warm-up step,
+// from a saved state that is restored afterwards, so tuning does not advance
+// the cases.
+namespace {
+
+struct TuneState
+{
+    mpcf_batch* b;
+    double cfl;
+    int steps;
+    std::vector<void*> save;
+    std::vector<int> cur0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+};
+
+int tune_save(TuneState& T)
+{
+    mpcf_batch* b = T.b;
+    for (mpcf_ctx* c : b->cases) {
+        void* p = nullptr;
+        const size_t reg = 7 * (size_t)c->g.fstride * c->esize;   // one 7-field register
+        const hipError_t e = hipMalloc(&p, 2 * reg);
+        if (e != hipSuccess) return fail(MPCF_ENOMEM, "mpcf_batch_autotune: no room to save the state");
+        T.save.push_back(p);
+        HIP_TRY(hipMemcpyAsync(p, c->u[c->cur], reg, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync((char*)p + reg, c->R, reg, hipMemcpyDeviceToDevice, b->stream));
+        T.cur0.push_back(c->cur);
+    }
+    HIP_TRY(hipEventCreate(&T.e0));
+    HIP_TRY(hipEventCreate(&T.e1));
+    return MPCF_OK;
+}
+
+int tune_restore(TuneState& T)
+{
+    mpcf_batch* b = T.b;
+    int rc = batch_pull_cur(b);
+    for (size_t i = 0; i < b->cases.size(); ++i) {
+        mpcf_ctx* c = b->cases[i];
+        c->cur = T.cur0[i];
+        const size_t reg = 7 * (size_t)c->g.fstride * c->esize;
+        HIP_TRY(hipMemcpyAsync(c->u[c->cur], T.save[i], reg, hipMemcpyDeviceToDevice, b->stream));
+        HIP_TRY(hipMemcpyAsync(c->R, (char*)T.save[i] + reg, reg, hipMemcpyDeviceToDevice, b->stream));
+        // the folded speed of sound belongs to the trial state: reduce again
+        HIP_TRY(hipMemsetAsync(&c->st->folded, 0, sizeof(int), b->stream));
+    }
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    return rc;
+}
+
+void tune_free(TuneState& T)
+{
+    for (void* q : T.save) hipFree(q);
+    if (T.e0) hipEventDestroy(T.e0);
+    if (T.e1) hipEventDestroy(T.e1);
+}
+
+// milliseconds of `steps` steps in the batch's current configuration
+int tune_trial(TuneState& T, float* ms)
+{
+    mpcf_batch* b = T.b;
+    b->dirty = true;
+    int rc = mpcf_batch_step(b, T.cfl, 1e30);        // warm-up in this configuration
+    if (rc == MPCF_OK) {
+        HIP_TRY(hipEventRecord(T.e0, b->stream));
+        for (int k = 0; k < T.steps && rc == MPCF_OK; ++k) rc = mpcf_batch_step(b, T.cfl, 1e30);
+        HIP_TRY(hipEventRecord(T.e1, b->stream));
+        HIP_TRY(hipEventSynchronize(T.e1));
+        HIP_TRY(hipEventElapsedTime(ms, T.e0, T.e1));
+    }
+    const int rc2 = tune_restore(T);
+    return rc != MPCF_OK ? rc : rc2;
+}
+
+}  // namespace
+
 extern "C" int mpcf_batch_autotune(mpcf_batch* b, int steps, double cfl, int* best_len)
 {
     if (!b || steps < 1 || !(cfl > 0)) return fail(MPCF_EINVAL, "mpcf_batch_autotune: bad argument");
@@ -1176,66 +1368,63 @@ extern "C" int mpcf_batch_autotune(mpcf_batch* b, int steps, double cfl, int* be
     int rc = batch_pull_cur(b);
     if (rc != MPCF_OK) return rc;
     HIP_TRY(hipStreamSynchronize(b->stream));
-    // save u (current buffer) and R of every case
-    std::vector<void*> save;
-    for (mpcf_ctx* c : b->cases) {
-        void* p = nullptr;
-        const size_t reg = 7 * (size_t)c->g.fstride * c->esize;   // one 7-field register
-        hipError_t e = hipMalloc(&p, 2 * reg);
-        if (e != hipSuccess) {
-            for (void* q : save) hipFree(q);
-            return fail(MPCF_ENOMEM, "mpcf_batch_autotune: no room to save the state");
-        }
-        HIP_TRY(hipMemcpyAsync(p, c->u[c->cur], reg, hipMemcpyDeviceToDevice, b->stream));
-        HIP_TRY(hipMemcpyAsync((char*)p + reg, c->R, reg, hipMemcpyDeviceToDevice, b->stream));
-        save.push_back(p);
-    }
-    std::vector<int> cur0(b->cases.size());
-    for (size_t i = 0; i < b->cases.size(); ++i) cur0[i] = b->cases[i]->cur;
-    const int cand[] = {16, 32, 64, 128, 1 << 20};
+    TuneState T{b, cfl, steps};
+    rc = tune_save(T);
     const bool graph = b->use_graph;
     b->use_graph = false;
+    batch_drop_graph(b);
+    // 1. launch mode and workgroup size, with the lengths tuned so far
+    const int modes[][2] = {{MPCF_BATCH_MODE_PERSISTENT, 256}, {MPCF_BATCH_MODE_PERSISTENT, 128},
+                            {MPCF_BATCH_MODE_PERSISTENT, 64}, {MPCF_BATCH_MODE_GROUPS, 256}};
     float best = 1e30f;
-    int bestl = 64;
-    hipEvent_t e0, e1;
-    HIP_TRY(hipEventCreate(&e0));
-    HIP_TRY(hipEventCreate(&e1));
-    for (int l : cand) {
-        b->tuned_len.clear();
-        for (mpcf_ctx* c : b->cases) { b->tuned_len[c->g.ny] = l; b->tuned_len[c->g.nz] = l; }
-        b->dirty = true;
-        rc = mpcf_batch_step(b, cfl, 1e30);        // warm-up with this shape
-        if (rc == MPCF_OK) {
-            hipEventRecord(e0, b->stream);
-            for (int k = 0; k < steps && rc == MPCF_OK; ++k) rc = mpcf_batch_step(b, cfl, 1e30);
-            hipEventRecord(e1, b->stream);
-            hipEventSynchronize(e1);
-            float ms = 0;
-            hipEventElapsedTime(&ms, e0, e1);
-            if (ms < best) { best = ms; bestl = l; }
-        }
-        // restore
-        rc = batch_pull_cur(b);
-        for (size_t i = 0; i < b->cases.size(); ++i) {
-            mpcf_ctx* c = b->cases[i];
-            c->cur = cur0[i];
-            const size_t reg = 7 * (size_t)c->g.fstride * c->esize;
-            hipMemcpyAsync(c->u[c->cur], save[i], reg, hipMemcpyDeviceToDevice, b->stream);
-            hipMemcpyAsync(c->R, (char*)save[i] + reg, reg, hipMemcpyDeviceToDevice, b->stream);
-            // the folded speed of sound belongs to the trial state: reduce again
-            hipMemsetAsync(&c->st->folded, 0, sizeof(int), b->stream);
-        }
-        hipStreamSynchronize(b->stream);
+    int bm = b->mode, bt = b->p_threads;
+    for (const auto& m : modes) {
         if (rc != MPCF_OK) break;
+        b->mode = m[0];
+        b->p_threads = m[1];
+        float ms = 0;
+        rc = tune_trial(T, &ms);
+        if (rc == MPCF_OK && ms < best) { best = ms; bm = m[0]; bt = m[1]; }
     }
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
-    for (void* q : save) hipFree(q);
-    b->tuned_len.clear();
-    for (mpcf_ctx* c : b->cases) { b->tuned_len[c->g.ny] = bestl; b->tuned_len[c->g.nz] = bestl; }
+    b->mode = bm;
+    b->p_threads = bt;
+    // 2. per extent, largest first: the x+y march length by NY, then the z
+    // march length by NZ, the other extents' lengths held at their best
+    std::vector<int> ext;
+    for (mpcf_ctx* c : b->cases) { ext.push_back(c->g.ny); ext.push_back(c->g.nz); }
+    std::sort(ext.begin(), ext.end(), std::greater<int>());
+    ext.erase(std::unique(ext.begin(), ext.end()), ext.end());
+    for (int which = 0; which < 2 && rc == MPCF_OK; ++which) {
+        std::map<int, int>& tuned = which == 0 ? b->tuned_xy : b->tuned_z;
+        for (int E : ext) {
+            if (rc != MPCF_OK) break;
+            bool used = false;
+            for (mpcf_ctx* c : b->cases) used |= (which == 0 ? c->g.ny : c->g.nz) == E;
+            if (!used) continue;
+            const int keep = tuned.count(E) ? tuned[E] : 0;
+            int bl = keep;
+            std::vector<int> cand = {8, 16, 32, 64, E};
+            std::sort(cand.begin(), cand.end());
+            cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+            for (int l : cand) {
+                if (l > E || rc != MPCF_OK) continue;
+                tuned[E] = l;
+                float ms = 0;
+                rc = tune_trial(T, &ms);
+                if (rc == MPCF_OK && ms < best) { best = ms; bl = l; }
+            }
+            if (bl > 0) tuned[E] = bl; else tuned.erase(E);
+        }
+    }
+    tune_free(T);
     b->dirty = true;
     b->use_graph = graph;
     b->steps = 0;
-    if (best_len) *best_len = bestl;
+    if (best_len && !ext.empty()) {
+        int E = 0;
+        for (mpcf_ctx* c : b->cases) E = std::max(E, c->g.ny);
+        auto it = b->tuned_xy.find(E);
+        *best_len = it != b->tuned_xy.end() ? it->second : 0;
+    }
     return rc;
 }

@@ -58,6 +58,8 @@ struct Launchers
     decltype(&mpcf::parity::batch_sos) b_sos;
     decltype(&mpcf::parity::batch_xy) b_xy;
     decltype(&mpcf::parity::batch_z) b_z;
+    decltype(&mpcf::parity::persist_stage) persist;
+    decltype(&mpcf::parity::persist_occupancy) persist_occ;
 };
 
 #define MPCF_LAUNCHERS(NS)                                                                           \
@@ -66,7 +68,8 @@ struct Launchers
      mpcf::NS::launch_sweep, mpcf::NS::launch_update,                                                            \
      mpcf::NS::launch_pack, mpcf::NS::launch_stream, mpcf::NS::launch_test_weno,                      \
      mpcf::NS::launch_test_face,                                                                      \
-     mpcf::NS::batch_sos, mpcf::NS::batch_xy, mpcf::NS::batch_z}
+     mpcf::NS::batch_sos, mpcf::NS::batch_xy, mpcf::NS::batch_z, mpcf::NS::persist_stage,        \
+     mpcf::NS::persist_occupancy}
 
 inline const Launchers kParity = MPCF_LAUNCHERS(parity);
 inline const Launchers kFast = MPCF_LAUNCHERS(fast);
@@ -217,6 +220,16 @@ inline void case_desc(const mpcf_ctx* c, const int cur, const int len_xy, const 
     d.nb_sos = rows4 < 2048 ? rows4 : 2048;
     blocks[2] = d.nb_sos;
     (void)cur;
+}
+
+// the persistent stage kernel's items of a case with workgroups of W waves
+inline void case_items(CaseDesc& d, const int W)
+{
+    const int tiles = ((d.g.nx + 6) * d.g.nz + kXYTile - 1) / kXYTile;
+    d.wxy = (tiles + W - 1) / W;
+    d.items_xy = (unsigned)d.wxy * (unsigned)((d.g.ny + d.chunk_xy - 1) / d.chunk_xy);
+    d.wz = (d.g.nx * d.g.ny + 64 * W - 1) / (64 * W);
+    d.items_z = (unsigned)d.wz * (unsigned)((d.g.nz + d.chunk_z - 1) / d.chunk_z);
 }
 
 }  // namespace detail

@@ -398,14 +398,14 @@ __device__ __forceinline__ SosT sweep_yz_column(const Geom& g, const StageArgs& 
     return m.smax;
 }
 
+// column `col` of the columns (x, t) flattened x-fastest: consecutive lanes
+// are consecutive x (coalesced) and no lane idles when nx is not a multiple
+// of 64.  Every lane of the wave takes part (the fold reduction is per wave)
 template <int D, int MODE>
 __device__ __forceinline__ void sweep_yz_body(const Geom& g, const StageArgs& A, const int chunk,
-                                              const int bcol, const int bz)
+                                              const int col, const int bz)
 {
-    // columns (x, t) flattened x-fastest, 256 per block: consecutive lanes are
-    // consecutive x (coalesced) and no lane idles when nx is not a multiple of 64
     const int nt = D == 1 ? A.z1 - A.z0 : g.ny;
-    const int col = bcol * 256 + threadIdx.x;
     SosT smax = 0;
     bool bad = false;
     if (col < g.nx * nt) smax = sweep_yz_column<D, MODE>(g, A, chunk, col, bz, bad);   // no barriers in the body
@@ -429,7 +429,7 @@ template <int D, int MODE>
 __global__ __launch_bounds__(256)
 void k_sweep_yz(const Geom g, const StageArgs A, const int chunk)
 {
-    sweep_yz_body<D, MODE>(g, A, chunk, blockIdx.x, blockIdx.y);
+    sweep_yz_body<D, MODE>(g, A, chunk, blockIdx.x * 256 + threadIdx.x, blockIdx.y);
 }
 
 // ---------------------------------------------------------------------------
@@ -555,7 +555,7 @@ void k_sweep_x(const Geom g, const StageArgs A, const int chunk)
 // already summed in the x-then-y order of the reference (GPUkernels.cu:1111),
 // so the x sweep's accumulator round trip disappears.
 // ---------------------------------------------------------------------------
-enum { XY_TILE = 58 };
+enum { XY_TILE = kXYTile };
 
 // value of lane l-1 / lane l+1.  The wave's edge lanes (no source lane) read
 // zero: they are halo positions whose results are never stored, and with no
@@ -634,21 +634,22 @@ __device__ __forceinline__ CellRec reconstruct_lanes(const Prim& c)
     return o;
 }
 
+// wave `wv` of the x+y sweep over rows [bz * chunk, bz * chunk + chunk)
 __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A, const int chunk,
-                                              const int bw, const int bz)
+                                              const int wv, const int bz)
 {
     // positions q along the padded rows (x = -3 .. nx+2) of all z-planes,
     // flattened; wave w covers q = 58 w .. 58 w + 63 and stores lanes 3..60,
     // so consecutive waves overlap by six positions and ghost positions are
     // computed but never stored
-    const int lane = threadIdx.x;
+    const int lane = threadIdx.x & 63;
     const int rowlen = g.nx + 6;
     const int nzr = A.z1 - A.z0;                  // planes of this launch
-    const int q = (bw * 4 + threadIdx.y) * XY_TILE + lane;
+    const int q = wv * XY_TILE + lane;
     int z = q / rowlen;
     const int x = q - z * rowlen - 3;
     const bool store = lane >= 3 && lane <= 60 && x >= 0 && x < g.nx && z < nzr;
-    if ((bw * 4 + threadIdx.y) * XY_TILE >= rowlen * nzr) return;   // whole wave past the end
+    if (wv * XY_TILE >= rowlen * nzr) return;     // whole wave past the end
     z = A.z0 + min(z, nzr - 1);                   // lanes past the last plane read its last row
     // lanes outside [0, nx) read the cell their x-ghost copies
     Real sgx;
@@ -757,7 +758,7 @@ void k_sweep_xy(const Geom g, const StageArgs A, const int chunk)
 {
     const int gx = gridDim.x;
     const int t = xcd_order(blockIdx.x + blockIdx.y * gx, gx * gridDim.y);
-    sweep_xy_body(g, A, chunk, t % gx, t / gx);
+    sweep_xy_body(g, A, chunk, (t % gx) * 4 + threadIdx.y, t / gx);
 }
 
 // ---------------------------------------------------------------------------
@@ -1146,7 +1147,7 @@ void kb_sweep_xy(const BatchDev B, const Real a, const Real b)
     const CaseDesc& C = B.cases[c];
     const int local = blockIdx.x - B.off_xy[c];
     const StageArgs A = case_args(C, B.cur[c], a, b);
-    sweep_xy_body(C.g, A, C.chunk_xy, local % C.nbx_xy, local / C.nbx_xy);
+    sweep_xy_body(C.g, A, C.chunk_xy, (local % C.nbx_xy) * 4 + threadIdx.y, local / C.nbx_xy);
 }
 
 // the z sweep finishes the stage; the last block of each case to finish
@@ -1161,7 +1162,7 @@ void kb_sweep_z(const BatchDev B, const Real a, const Real b, const int fold, co
     const int local = blockIdx.x - B.off_z[c];
     StageArgs A = case_args(C, B.cur[c], a, b);
     if (fold) { A.sos_next = C.st; A.sos_cpu = cpu_form; }
-    sweep_yz_body<2, MODE_STAGE>(C.g, A, C.chunk_z, local % C.nbx_z, local / C.nbx_z);
+    sweep_yz_body<2, MODE_STAGE>(C.g, A, C.chunk_z, (local % C.nbx_z) * 256 + threadIdx.x, local / C.nbx_z);
     __syncthreads();
     if (threadIdx.x == 0) {
         __threadfence();
@@ -1172,6 +1173,105 @@ void kb_sweep_z(const BatchDev B, const Real a, const Real b, const int fold, co
             atomicExch(&C.st->z_count, 0u);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// persistent batched stage: two launches per RK stage for every case, one for
+// the x+y items of all cases, then one for their z items (a z item reads the
+// R and accumulators the x+y items of its case wrote; the launch boundary
+// orders them -- inside one launch that order would take a device-scope
+// release and acquire per item, an L2 write-back and invalidation of the XCD,
+// which measured 1.7x slower than flat launches).  Each workgroup pulls items
+// from a device queue (an atomic counter) until it is empty; an item is one
+// workgroup's share of one case's x+y sweep (W = blockDim.x / 64 consecutive
+// 58-position tiles over one y-chunk) or z sweep (blockDim.x columns over one
+// z-chunk), running the single-case bodies, so every case gets exactly the
+// arithmetic it gets alone.  The queue holds the items of the larger cases
+// first, so the small ones fill the launch's tail.
+// ---------------------------------------------------------------------------
+// Thread 0 alone takes an item or counts a finished one; those steps are
+// calls of their own, bracketed by barriers.  Written inline, the compiler
+// may restructure the item loop around such a branch so that thread 0 leaves
+// the loop body on its own while the other threads of its wave go round
+// again, which breaks the barriers and lets those threads run an item twice;
+// a call is convergent, so the loop body the compiler sees has no branch that
+// splits the workgroup.
+__device__ __attribute__((noinline)) unsigned take_item(PCtl* ctl)
+{
+    __shared__ unsigned s_item;
+    if (threadIdx.x == 0) s_item = atomicAdd(&ctl->head, 1u);
+    __syncthreads();
+    const unsigned i = s_item;
+    __syncthreads();                              // read everywhere before the next take
+    return i;
+}
+
+// a z item is finished; the case's last one flips its current-buffer index
+// (every z item of the case has read it by then: each reads it before it
+// counts itself) and clears the count for the next stage
+__device__ __attribute__((noinline)) void count_z(unsigned* done, const unsigned items_z, int* cur_c,
+                                                  const int cur, int* folded, const int fold)
+{
+    __syncthreads();
+    if (threadIdx.x == 0 && atomicAdd(done, 1u) == items_z - 1u) {
+        atomicExch(cur_c, 1 - cur);
+        if (fold) atomicExch(folded, 1);
+        atomicExch(done, 0u);
+    }
+}
+
+// PHASE 0: the x+y items (queue items [0, seg[nxy].first)); PHASE 1: the z
+// items (the rest).  Two kernels, so that each holds only its own sweep's
+// registers
+template <int PHASE>
+__device__ __forceinline__ void kp_stage_body(const PersistDev& P, const Real a, const Real b, const int fold,
+                                              const int cpu_form)
+{
+    constexpr int phase = PHASE;
+    const int W = (int)(blockDim.x >> 6);        // waves per item
+    const int wib = (int)(threadIdx.x >> 6);     // this wave's place in the item
+    const int s0 = phase == 0 ? 0 : P.nxy, s1 = phase == 0 ? P.nxy : P.nseg;
+    const unsigned first = P.seg[s0].first, end = P.seg[s1].first;
+    for (;;) {
+        // the same value in every thread: readfirstlane only makes it scalar
+        const unsigned i = first + (unsigned)__builtin_amdgcn_readfirstlane((int)take_item(P.ctl));
+        if (i >= end) break;
+        int lo = s0, hi = s1 - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (P.seg[mid].first <= i) lo = mid; else hi = mid - 1;
+        }
+        const PSeg S = P.seg[lo];
+        const CaseDesc& C = P.cases[S.c];
+        const int j = (int)(i - S.first);
+        const int cur = P.cur[S.c];
+        StageArgs A = case_args(C, cur, a, b);
+        if constexpr (phase == 0) {
+            (void)fold; (void)cpu_form;
+            sweep_xy_body(C.g, A, C.chunk_xy, (j % C.wxy) * W + wib, j / C.wxy);
+        } else {
+            if (fold) { A.sos_next = C.st; A.sos_cpu = cpu_form; }
+            sweep_yz_body<2, MODE_STAGE>(C.g, A, C.chunk_z, (j % C.wz) * (int)blockDim.x + (int)threadIdx.x, j / C.wz);
+            count_z(P.done + S.c, C.items_z, &P.cur[S.c], cur, &C.st->folded, fold);
+        }
+    }
+    // the last workgroup out puts the queue back for the next launch
+    if (threadIdx.x == 0 && atomicAdd(&P.ctl->exited, 1u) == P.nblocks - 1u) {
+        atomicExch(&P.ctl->head, 0u);
+        atomicExch(&P.ctl->exited, 0u);
+    }
+}
+
+__global__ __launch_bounds__(256) MPCF_XY_WAVES
+void kp_stage_xy(const PersistDev P, const Real a, const Real b)
+{
+    kp_stage_body<0>(P, a, b, 0, 0);
+}
+
+__global__ __launch_bounds__(256)
+void kp_stage_z(const PersistDev P, const Real a, const Real b, const int fold, const int cpu_form)
+{
+    kp_stage_body<1>(P, a, b, fold, cpu_form);
 }
 
 // ---------------------------------------------------------------------------
@@ -1368,6 +1468,26 @@ void batch_z(const BatchDev& B, const int blocks, const double a, const double b
              hipStream_t s)
 {
     hipLaunchKernelGGL(kb_sweep_z, dim3(blocks), dim3(256), 0, s, B, (Real)a, (Real)b, fold, cpu_form);
+}
+// the two launches of one stage; blocks[0] workgroups for the x+y items,
+// blocks[1] for the z items
+void persist_stage(const PersistDev& P, const int blocks[2], const int threads, const double a, const double b,
+                   const int fold, const int cpu_form, hipStream_t s)
+{
+    PersistDev Q = P;
+    Q.nblocks = (unsigned)blocks[0];
+    hipLaunchKernelGGL(kp_stage_xy, dim3(blocks[0]), dim3(threads), 0, s, Q, (Real)a, (Real)b);
+    Q.nblocks = (unsigned)blocks[1];
+    hipLaunchKernelGGL(kp_stage_z, dim3(blocks[1]), dim3(threads), 0, s, Q, (Real)a, (Real)b, fold, cpu_form);
+}
+// resident workgroups of `threads` threads per CU of the x+y (phase 0) or z
+// (phase 1) launch
+int persist_occupancy(const int threads, const int phase)
+{
+    int n = 0;
+    const hipError_t e = phase == 0 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kp_stage_xy, threads, 0)
+                                    : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, kp_stage_z, threads, 0);
+    return e == hipSuccess ? n : 0;
 }
 
 } // namespace MPCF_NS

@@ -9,6 +9,10 @@
 #include "mpcf_geom.h"
 
 namespace mpcf {
+// positions of a wave of the fused x+y sweep that it stores (64 lanes less
+// the three-cell x-halo on each side)
+constexpr int kXYTile = 58;
+
 // arguments of one direction sweep of a stage
 struct SweepParams
 {
@@ -43,6 +47,11 @@ struct CaseDesc
     int nbx_xy, nby_xy;    // x+y sweep grid: x tiles, z groups (y chunks = third dim)
     int nbx_z, nby_z;      // z sweep grid: x blocks, y groups (z chunks = third dim)
     int nb_sos;            // speed-of-sound blocks
+    // items of the persistent stage kernel (W = workgroup size / 64 waves
+    // each): x+y items per y-chunk (W tiles each) and in all; z items per
+    // z-chunk (64 W columns each) and in all
+    int wxy, wz;
+    unsigned items_xy, items_z;
 };
 
 // device tables of a batch: per-case descriptors, the current-buffer index of
@@ -56,6 +65,32 @@ struct BatchDev
     const int* off_xy;
     const int* off_z;
     const int* off_sos;
+};
+
+// the work queue of a persistent batched stage: a run of the x+y (kind 0) or
+// the z (kind 1) items of one case, starting at item `first` of the queue
+// (the x+y items of all cases come first)
+struct PSeg
+{
+    int c;
+    int kind;
+    unsigned first;
+};
+// queue head and the workgroups that found it empty
+struct PCtl
+{
+    unsigned head, exited;
+};
+struct PersistDev
+{
+    const CaseDesc* cases;
+    int* cur;
+    const PSeg* seg;       // nseg + 1 entries: the x+y segments [0, nxy), then the z
+                           // segments; seg[nseg].first = items in all
+    int nseg, nxy;
+    unsigned* done;        // per case: finished z items of the running stage
+    PCtl* ctl;
+    unsigned nblocks;      // workgroups of the launch (set per launch)
 };
 }
 
@@ -77,6 +112,9 @@ void launch_test_face(int n, int dir, const void* st, void* out, hipStream_t s);
 void batch_sos(const BatchDev& B, int blocks, int cpu_form, double cfl, double dt_max, hipStream_t s);    \
 void batch_xy(const BatchDev& B, int blocks, double a, double b, hipStream_t s);                           \
 void batch_z(const BatchDev& B, int blocks, double a, double b, int fold, int cpu_form, hipStream_t s);    \
+void persist_stage(const PersistDev& P, const int blocks[2], int threads, double a, double b, int fold,     \
+                   int cpu_form, hipStream_t s);                                                           \
+int persist_occupancy(int threads, int phase);                                                             \
 void xy_grid(const Geom& g, int chunk, int dims[3]);                                                       \
 void z_grid(const Geom& g, int chunk, int dims[3]);                                                        \
 int default_chunk(int nd, long long columns);                                                              \

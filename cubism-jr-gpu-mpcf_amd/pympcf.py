@@ -21,6 +21,7 @@ BC_ABSORBING, BC_REFLECTING, BC_HALO = 0, 1, 2
 FLAG_FAST, FLAG_SOS_CPU, FLAG_SPLIT_XY, FLAG_FUSED_XY, FLAG_P64, FLAG_FP64 = 1, 2, 4, 8, 16, 32
 FLAG_NO_PIPELINE = 128
 DATA, TMP = 0, 1
+BATCH_GROUPS, BATCH_PERSISTENT = 0, 1
 NVAR = 7
 
 # LSRK3 coefficients of LSRK3_IntegratorMPI.cpp:46,51,56
@@ -37,7 +38,7 @@ EXPORTS = [
     "mpcf_host_face", "mpcf_case_bytes", "mpcf_batch_create", "mpcf_batch_destroy", "mpcf_batch_add",
     "mpcf_batch_remove", "mpcf_batch_download", "mpcf_batch_step", "mpcf_batch_use_graph",
     "mpcf_batch_last_dt", "mpcf_batch_synchronize", "mpcf_batch_get_stream", "mpcf_batch_set_march",
-    "mpcf_batch_autotune", "mpcf_batch_query", "mpcf_upload_f64", "mpcf_download_f64",
+    "mpcf_batch_autotune", "mpcf_batch_query", "mpcf_batch_set_mode", "mpcf_batch_tuning", "mpcf_upload_f64", "mpcf_download_f64",
     "mpcf_last_sos64", "mpcf_compare", "mpcf_integrals", "mpcf_server_create", "mpcf_server_submit",
     "mpcf_server_poll", "mpcf_server_wait", "mpcf_server_stats", "mpcf_server_clock", "mpcf_server_destroy",
     "mpcf_sic_defaults", "mpcf_sic_post_shock", "mpcf_read_cloud_config", "mpcf_read_cloud", "mpcf_ic_sic_cloud",
@@ -149,6 +150,8 @@ def lib():
         L.mpcf_batch_get_stream.restype = P
         L.mpcf_batch_set_march.argtypes = [P, I, I, I]
         L.mpcf_batch_autotune.argtypes = [P, I, D, P]
+        L.mpcf_batch_set_mode.argtypes = [P, I, I]
+        L.mpcf_batch_tuning.argtypes = [P, I, P]
         L.mpcf_batch_query.argtypes = [P, P]
         L.mpcf_upload_f64.argtypes = [P, I, P]
         L.mpcf_download_f64.argtypes = [P, I, P]
@@ -747,14 +750,29 @@ def case_bytes(nx, ny, nz):
 class Batch:
     """A batched ensemble (mpcf_batch): cases of uneven sizes advanced together."""
 
-    def __init__(self, fast=False, sos_cpu=False, graph=False, p64=False):
+    def __init__(self, fast=False, sos_cpu=False, graph=False, p64=False, mode=None, threads=0):
+        """mode: None = the library's default (BATCH_GROUPS), or
+        BATCH_GROUPS / BATCH_PERSISTENT; threads: the persistent launches'
+        workgroup size (64, 128, 256; 0 = 256)."""
         h_ = C.c_void_p()
         flags = (FLAG_FAST if fast else 0) | (FLAG_SOS_CPU if sos_cpu else 0) | (FLAG_P64 if p64 else 0)
         _check(lib().mpcf_batch_create(C.byref(h_), flags))
         self._h = h_
         self.shapes = {}
+        if mode is not None:
+            self.set_mode(mode, threads)
         if graph:
             self.use_graph(True)
+
+    def set_mode(self, mode, threads=0):
+        _check(lib().mpcf_batch_set_mode(self._h, int(mode), int(threads)))
+
+    def tuning(self, extent):
+        """(mode, persistent workgroup size, x+y march length, z march length)
+        in force for cases of this extent (0: no tuned length)"""
+        out = (C.c_int * 4)()
+        _check(lib().mpcf_batch_tuning(self._h, int(extent), C.cast(out, C.c_void_p)))
+        return tuple(out)
 
     def close(self):
         if getattr(self, "_h", None):

@@ -314,9 +314,37 @@ int mpcf_batch_synchronize(mpcf_batch* batch);
 void* mpcf_batch_get_stream(mpcf_batch* batch);
 /* Force the march lengths of one case (0 = tuned / default). */
 int mpcf_batch_set_march(mpcf_batch* batch, int case_id, int len_xy, int len_z);
-/* Autotune the march length on the live cases: times `steps` steps per
- * candidate from a saved state and restores it (the cases do not advance). */
+
+/* How mpcf_batch_step launches a step (same results either way):
+ *   MPCF_BATCH_MODE_PERSISTENT: the speed of sound / dt of every case, then
+ *     per RK stage two persistent launches, one over the x+y sweep items of
+ *     all cases and one over their z sweep items; each workgroup pulls
+ *     (case, tile) items from a device queue -- one workgroup's share of a
+ *     case's x+y sweep (threads/64 consecutive 58-cell x-tiles over one
+ *     y-chunk) or z sweep (`threads` columns over one z-chunk) -- largest case
+ *     first, until the queue is empty.  threads: workgroup size, 64, 128 or
+ *     256 (0: 256).
+ *   MPCF_BATCH_MODE_GROUPS (default): the cases in four groups of about equal
+ *     cells, each on its own stream, with one flat launch per kernel kind per
+ *     stage. */
+enum { MPCF_BATCH_MODE_GROUPS = 0, MPCF_BATCH_MODE_PERSISTENT = 1 };
+int mpcf_batch_set_mode(mpcf_batch* batch, int mode, int threads);
+
+/* Autotune the batch on its live cases: the launch mode and the persistent
+ * launches' workgroup size, then for each case extent (largest first) the
+ * x+y sweep's march length (by NY) and the z sweep's (by NZ) among 8, 16,
+ * 32, 64 and the whole extent, each by coordinate descent.  Every candidate
+ * is timed over `steps` steps after a warm-up step, from a saved state that
+ * is restored afterwards (the cases do not advance).  The choice is kept per
+ * extent for the cases added later.  best_len (may be NULL): the x+y march
+ * length chosen for the largest extent. */
 int mpcf_batch_autotune(mpcf_batch* batch, int steps, double cfl, int* best_len);
+
+/* The tuning in force: out[0] mode, [1] persistent workgroup size, [2] x+y
+ * and [3] z march length of cases of extent `extent` (0: no tuned value, the
+ * size-based default). */
+int mpcf_batch_tuning(mpcf_batch* batch, int extent, int out[4]);
+
 /* out[0] live cases, [1] interior cells, [2] device bytes, [3] graph captures,
  * [4] steps since the last autotune, [5] tuned extents. */
 int mpcf_batch_query(mpcf_batch* batch, long long out[6]);

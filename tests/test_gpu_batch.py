@@ -2,7 +2,11 @@
 context) and in a batch -- bit for bit, since the batch kernels run the same
 per-block bodies -- with eager launches, with graph replay (checked after every
 replay, with eager work and synchronisations in between), across additions
-and removals between steps (continuous batching), and after autotuning."""
+and removals between steps (continuous batching), and after autotuning; in
+both launch modes (per stage, two persistent launches pulling (case, tile)
+items from a device queue, in three workgroup sizes and with march lengths
+short enough that every case has many items; or flat launches per case
+group)."""
 import sys
 
 import numpy as np
@@ -38,14 +42,22 @@ def alone(mpcf, s, bc, steps, fast=False):
         return b.data, b.tmp, dts
 
 
+MODES = {"persistent": (1, 256, 0), "persistent64": (1, 64, 8), "persistent128": (1, 128, 16), "groups": (0, 0, 0)}
+
+
+@pytest.mark.parametrize("mode", list(MODES))
 @pytest.mark.parametrize("fast", [False, True], ids=["bitexact", "fast"])
 @pytest.mark.parametrize("graph", [False, True], ids=["eager", "graph"])
-def test_batch_equals_alone(mpcf, fast, graph):
+def test_batch_equals_alone(mpcf, fast, graph, mode):
     cs = cases(mpcf)
     steps = 3
-    say("batch", len(cs), "cases fast", fast, "graph", graph)
-    with mpcf.Batch(fast=fast, graph=graph) as B:
+    m, threads, march = MODES[mode]
+    say("batch", len(cs), "cases fast", fast, "graph", graph, "mode", mode)
+    with mpcf.Batch(fast=fast, graph=graph, mode=m, threads=threads) as B:
         ids = [B.add(s, bc=bc) for s, bc in cs]
+        if march:
+            for i in ids:
+                B.set_march(i, march, march)
         per_step = []
         for k in range(steps):
             B.step(0.3)
@@ -130,7 +142,12 @@ def test_autotune_keeps_state_and_results(mpcf):
     with mpcf.Batch(fast=True) as B:
         ids = [B.add(s, bc=bc) for s, bc in cs]
         best = B.autotune(steps=1)
-        assert best in (16, 32, 64, 128, 1 << 20)
+        ny_max = max(s.shape[2] for s, _ in cs)
+        assert best in (0, 8, 16, 32, 64, ny_max), best
+        mode, threads, lxy, lz = B.tuning(ny_max)
+        say("  tuned: mode", mode, "threads", threads, "x+y march", lxy, "z march", lz)
+        assert mode in (mpcf.BATCH_GROUPS, mpcf.BATCH_PERSISTENT) and threads in (64, 128, 256)
+        assert lxy == best
         for i, (s, _) in zip(ids, cs):
             assert np.array_equal(B.download(i), s)     # tuning did not advance the cases
         B.step(0.3)
