@@ -96,7 +96,8 @@ bool split_xy(const mpcf_ctx* c)
                                                    : !(c->flags & MPCF_FLAG_FAST);
 }
 
-// R <- a R - dtinvh L(u) (stage == 0), or that and u_out <- u + b R (stage == 1)
+// R <- a R - dtinvh L(u) (stage == 0), or that and u_out <- u + b R (stage == 1;
+// stage == 2: u_out only, R not stored)
 void run_rhs(mpcf_ctx* c, const void* u, void* uo, double a, double b, double dtinvh, const StepState* st, int stage,
              StepState* sos_next = nullptr)
 {
@@ -153,7 +154,8 @@ bool dt_deferred(const mpcf_ctx* c)
 // the first stage's x+y sweep of slab j waits for that step's last z sweep of
 // slab j (which wrote the slab's u) instead of the whole step.  open: leave
 // this step open the same way (the caller's stream has not joined it).
-int run_stages_pipelined(mpcf_ctx* c, const int S, const bool prev_open, const bool open)
+// skipR: the last stage's z sweeps do not store R.
+int run_stages_pipelined(mpcf_ctx* c, const int S, const bool prev_open, const bool open, const bool skipR)
 {
     if (!c->stream2) HIP_TRY(hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking));
     while (c->pipe_ev.size() < (size_t)(2 * 3 * S)) {
@@ -184,7 +186,7 @@ int run_stages_pipelined(mpcf_ctx* c, const int S, const bool prev_open, const b
             }
             HIP_TRY(hipEventRecord(exy(s, j), c->stream));
             HIP_TRY(hipStreamWaitEvent(c->stream2, exy(s, j), 0));
-            { Timed t(c, K_SWEEP_Z, c->stream2); c->L->sweep(c->g, 2, p, 1, c->stream2); }
+            { Timed t(c, K_SWEEP_Z, c->stream2); c->L->sweep(c->g, 2, p, (s == 2 && skipR) ? 2 : 1, c->stream2); }
             HIP_TRY(hipEventRecord(ez(s, j), c->stream2));
         }
         LAUNCH_CHECK("stage");
@@ -590,12 +592,16 @@ int step_queue(mpcf_ctx* c, const double cfl, const double dt_max, const bool ke
     LAUNCH_CHECK("sos/dt");
     c->sos_folded = false;
     c->pipe_open = 0;
+    // a FAST step that another step of the same call follows does not store
+    // R in its last stage: the next step's first stage multiplies it by a = 0
+    const bool skipR = keep_open && dt_deferred(c);
     if (slabs) {
-        const int rc = run_stages_pipelined(c, slabs, prev_open, defer && keep_open);
+        const int rc = run_stages_pipelined(c, slabs, prev_open, defer && keep_open, skipR);
         if (rc != MPCF_OK) return rc;
     } else {
         for (int s = 0; s < 3; ++s) {
-            run_rhs(c, c->u[c->cur], c->u[1 - c->cur], kA[s], kB[s], 0.0, c->st, 1, s == 2 ? c->st : nullptr);
+            run_rhs(c, c->u[c->cur], c->u[1 - c->cur], kA[s], kB[s], 0.0, c->st, (s == 2 && skipR) ? 2 : 1,
+                    s == 2 ? c->st : nullptr);
             LAUNCH_CHECK("stage");
             c->cur = 1 - c->cur;
         }

@@ -120,7 +120,11 @@ constexpr bool kUnrollXY = kFast && !MPCF_F64;
 // likewise the y / z marches (a separate switch: their register budget differs)
 constexpr bool kUnrollMarch = kFast && !MPCF_F64;
 
-enum { MODE_FIRST = 0, MODE_ACC = 1, MODE_CONV = 2, MODE_STAGE = 3 };
+// MODE_STAGE_NOR: a stage whose R is not stored -- the last stage of a step
+// that another step of the same mpcf_steps call follows (that step's first
+// stage multiplies R by a = 0), 28 B/cell less for the bandwidth-bound z sweep
+enum { MODE_FIRST = 0, MODE_ACC = 1, MODE_CONV = 2, MODE_STAGE = 3, MODE_STAGE_NOR = 4 };
+constexpr bool is_stage(const int mode) { return mode == MODE_STAGE || mode == MODE_STAGE_NOR; }
 
 struct StageArgs
 {
@@ -202,7 +206,7 @@ __device__ __forceinline__ void emit_load(const StageArgs& A, const int e, EmitI
         in.acc[A_SUMP] = A.acc[A_SUMP*fs + e];
         in.acc[A_DIVU] = A.acc[A_DIVU*fs + e];
     }
-    if (MODE == MODE_CONV || MODE == MODE_STAGE || (kFold && MODE == MODE_ACC)) {
+    if (MODE == MODE_CONV || is_stage(MODE) || (kFold && MODE == MODE_ACC)) {
 #pragma unroll
         for (int q = 0; q < NVAR; ++q) in.R[q] = A.R[q*fs + e];
     }
@@ -255,14 +259,14 @@ __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, cons
             // subtraction, so it rounds as the x+y sweep's product would)
             const Real Rx = !kFold ? A.a*in.R[q] : unscaled_stage(A.a) ? __fmul_rn(dtinvh, in.R[q]) : in.R[q];
             const Real Rn = Rx - rhs;
-            A.R[q*fs + e] = Rn;
-            if (MODE == MODE_STAGE) {
+            if (MODE != MODE_STAGE_NOR) A.R[q*fs + e] = Rn;
+            if (is_stage(MODE)) {
                 un[q] = A.b*Rn + in.u[q];
                 A.uo[q*fs + e] = un[q];
             }
         }
         // the next step's speed of sound, from the values just stored
-        if (MODE == MODE_STAGE && A.sos_next) {
+        if (is_stage(MODE) && A.sos_next) {
             bool b;
             smax = mmax(smax, sos_cell(A.sos_cpu != 0, un[0], un[1], un[2], un[3], un[4], un[5], un[6], b));
             bad = bad || b;
@@ -335,7 +339,7 @@ __device__ __forceinline__ SosT sweep_yz_column(const Geom& g, const StageArgs& 
     // the y sweep of a first stage stores its part unscaled and does not read dt
     const Real dtinvh = (MODE == MODE_ACC && unscaled_stage(A.a)) ? Real(0.0f) : stage_dtinvh(A);
     const int64_t fs = A.fs;
-    constexpr bool kRaw = MODE == MODE_STAGE;   // the update needs the raw u of the emitted cell
+    constexpr bool kRaw = is_stage(MODE);   // the update needs the raw u of the emitted cell
 
     constexpr int NEGQ = D == 1 ? FV : FW;   // normal momentum of the marched axis
     constexpr int FLO = 2 * D;               // its low face
@@ -409,7 +413,7 @@ __device__ __forceinline__ void sweep_yz_body(const Geom& g, const StageArgs& A,
     SosT smax = 0;
     bool bad = false;
     if (col < g.nx * nt) smax = sweep_yz_column<D, MODE>(g, A, chunk, col, bz, bad);   // no barriers in the body
-    if (MODE == MODE_STAGE && A.sos_next) {
+    if (is_stage(MODE) && A.sos_next) {
         // one atomic max (and, for a bad cell, one atomic or) per wave; every
         // lane of the wave takes part
 #pragma unroll
@@ -1356,7 +1360,8 @@ void launch_sos_folded(StepState* st, const double cfl, const double h, const do
 // one direction sweep.  dir 0 (x) assigns the accumulators, dir 1 (y)
 // accumulates, dir 3 does x and y fused (assigns), dir 2 (z) finishes the
 // register update (stage == 0: R only, as Convection; stage == 1: R and the
-// new solution, as process_all).
+// new solution, as process_all; stage == 2: the new solution only, R not
+// stored).
 void launch_sweep(const Geom& g, const int dir, const SweepParams& p, const int stage, hipStream_t s)
 {
     StageArgs A;
@@ -1384,8 +1389,9 @@ void launch_sweep(const Geom& g, const int dir, const SweepParams& p, const int 
     } else {
         const int chunk = pick_chunk(nzr, (int64_t)g.nx * g.ny, alone);
         const dim3 grid((g.nx * g.ny + 255) / 256, (nzr + chunk - 1) / chunk);
-        if (stage) hipLaunchKernelGGL((k_sweep_yz<2, MODE_STAGE>), grid, dim3(256), 0, s, g, A, chunk);
-        else       hipLaunchKernelGGL((k_sweep_yz<2, MODE_CONV>), grid, dim3(256), 0, s, g, A, chunk);
+        if (stage == 2)      hipLaunchKernelGGL((k_sweep_yz<2, MODE_STAGE_NOR>), grid, dim3(256), 0, s, g, A, chunk);
+        else if (stage == 1) hipLaunchKernelGGL((k_sweep_yz<2, MODE_STAGE>), grid, dim3(256), 0, s, g, A, chunk);
+        else                 hipLaunchKernelGGL((k_sweep_yz<2, MODE_CONV>), grid, dim3(256), 0, s, g, A, chunk);
     }
 }
 
