@@ -9,6 +9,7 @@ Objects:
             results do not depend on which unrolled or peeled step, chunk or
             batch computes a cell)
   fastp.o   mpcf_kernels.hip, fast + MPCF_P64=1 (binary64 pressure recovery and reductions)
+  fastpe.o  mpcf_kernels.hip, fastp + MPCF_EPAIR=1 (E stored as a binary32 hi/lo pair)
   f64.o     mpcf_kernels.hip, MPCF_F64=1, -ffp-contract=off (the scheme in binary64)
   reduce.o  mpcf_reduce.hip  (binary64 block comparisons and integrals)
   wavelet.o mpcf_wavelet.hip (wavelet-compressed dumps; linked with zlib)
@@ -70,6 +71,7 @@ def build(verbose: bool = False) -> str:
         ("parity.o", "mpcf_kernels.hip", ["-DMPCF_FAST=0", "-ffp-contract=off"]),
         ("fast.o", "mpcf_kernels.hip", ["-DMPCF_FAST=1", "-ffp-contract=on"]),
         ("fastp.o", "mpcf_kernels.hip", ["-DMPCF_FAST=1", "-DMPCF_P64=1", "-ffp-contract=on"]),
+        ("fastpe.o", "mpcf_kernels.hip", ["-DMPCF_FAST=1", "-DMPCF_P64=1", "-DMPCF_EPAIR=1", "-ffp-contract=on"]),
         ("f64.o", "mpcf_kernels.hip", ["-DMPCF_FAST=0", "-DMPCF_F64=1", "-ffp-contract=off"]),
         ("reduce.o", "mpcf_reduce.hip", ["-ffp-contract=off"]),
         ("wavelet.o", "mpcf_wavelet.hip", ["-ffp-contract=off"]),

@@ -229,8 +229,11 @@ extern "C" int mpcf_create(mpcf_ctx** out, int nx, int ny, int nz, double h, uns
     if (flags & ~kAllFlags) return fail(MPCF_EINVAL, "mpcf_create: unknown flag");
     if ((flags & MPCF_FLAG_SPLIT_XY) && (flags & MPCF_FLAG_FUSED_XY))
         return fail(MPCF_EINVAL, "mpcf_create: SPLIT_XY and FUSED_XY are exclusive");
-    if ((flags & MPCF_FLAG_FP64) && (flags & (MPCF_FLAG_FAST | MPCF_FLAG_P64)))
-        return fail(MPCF_EINVAL, "mpcf_create: FP64 excludes FAST and P64");
+    if ((flags & MPCF_FLAG_FP64) && (flags & (MPCF_FLAG_FAST | MPCF_FLAG_P64 | MPCF_FLAG_E_PAIR)))
+        return fail(MPCF_EINVAL, "mpcf_create: FP64 excludes FAST, P64 and E_PAIR");
+    if ((flags & MPCF_FLAG_E_PAIR) && (flags & MPCF_FLAG_SPLIT_XY))
+        return fail(MPCF_EINVAL, "mpcf_create: E_PAIR runs the fused x+y sweep only");
+    if (flags & MPCF_FLAG_E_PAIR) flags |= MPCF_FLAG_P64;  // the pair is the P64 arithmetic plus E's low part
     if (flags & MPCF_FLAG_P64) flags |= MPCF_FLAG_FAST;   // P64 is the FAST arithmetic plus binary64 steps
     const Geom g = mpcf::make_geom(nx, ny, nz);
     if ((int64_t)g.px * g.py * g.pz >= (int64_t(1) << 31))
@@ -245,7 +248,9 @@ extern "C" int mpcf_create(mpcf_ctx** out, int nx, int ny, int nz, double h, uns
     hipError_t e = hipGetDevice(&c->device);
     if (e != hipSuccess) { delete c; return fail(MPCF_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e)); }
 
-    const size_t nfields = 7 + 7 + 7 + mpcf::NACC;
+    // a solution buffer holds 7 fields, or 8 with E's low part (E_PAIR)
+    const int nu = (flags & MPCF_FLAG_E_PAIR) ? 8 : 7;
+    const size_t nfields = nu + nu + 7 + mpcf::NACC;
     c->bytes = nfields * (size_t)g.fstride * c->esize + 256;
     e = hipMalloc((void**)&c->base, c->bytes);
     if (e != hipSuccess) {
@@ -270,9 +275,9 @@ extern "C" int mpcf_create(mpcf_ctx** out, int nx, int ny, int nz, double h, uns
     c->own_stream = true;
     const size_t fb = (size_t)g.fstride * c->esize;   // bytes per field (a multiple of 256)
     c->u[0] = c->base;
-    c->u[1] = c->base + 7 * fb;
-    c->R = c->base + 14 * fb;
-    c->acc = c->base + 21 * fb;
+    c->u[1] = c->base + nu * fb;
+    c->R = c->base + 2 * nu * fb;
+    c->acc = c->base + (2 * nu + 7) * fb;
     c->st = (StepState*)(c->base + nfields * fb);
     static_assert(sizeof(StepState) <= 256, "step state must fit the reserved tail");
     std::memset(c->st_host, 0, sizeof(StepState));
@@ -341,6 +346,8 @@ extern "C" int mpcf_set_bc(mpcf_ctx* c, const int bc[6])
     for (int i = 0; i < 6; ++i)
         if (bc[i] != MPCF_BC_ABSORBING && bc[i] != MPCF_BC_REFLECTING && bc[i] != MPCF_BC_HALO)
             return fail(MPCF_EINVAL, "mpcf_set_bc: unknown boundary kind");
+    if (mpcf::detail::has_halo(bc) && mpcf::detail::refuse_epair(c->flags, "mpcf_set_bc (halo faces)") != MPCF_OK)
+        return MPCF_EINVAL;
     for (int i = 0; i < 6; ++i) c->bc[i] = bc[i];
     return MPCF_OK;
 }
@@ -351,7 +358,7 @@ int halo_args(mpcf_ctx* c, const int face, const void* buf, const char* who)
 {
     if (!c || !buf) return fail(MPCF_EINVAL, std::string(who) + ": NULL argument");
     if (face < 0 || face > 5) return fail(MPCF_EINVAL, std::string(who) + ": face must be 0..5");
-    return MPCF_OK;
+    return mpcf::detail::refuse_epair(c->flags, who);
 }
 
 long long halo_count(const mpcf_ctx* c, const int face)
@@ -415,14 +422,18 @@ int transfer(mpcf_ctx* c, const int which, void* const* fields, const int host_f
     const size_t hb = host_f64 ? sizeof(double) : sizeof(float);
     const size_t fb = (size_t)c->g.fstride * c->esize;
     char* blk = (char*)(which == MPCF_DATA ? c->u[c->cur] : c->R);
+    // E of an E_PAIR solution: the pair (hi in field 4, lo in field 7)
+    const bool pair = which == MPCF_DATA && (c->flags & MPCF_FLAG_E_PAIR);
     for (int q = 0; q < 7; ++q) {
         void* dq = blk + q * fb;
         if (to_device) {
             HIP_TRY(hipMemcpyAsync(c->acc, fields[q], n * hb, hipMemcpyHostToDevice, c->stream));
-            c->L->pack(c->g, dq, c->acc, 1, 1, host_f64, c->stream);
+            if (pair && q == 4) c->L->pack_epair(c->g, dq, blk + 7 * fb, c->acc, 1, host_f64, c->stream);
+            else                c->L->pack(c->g, dq, c->acc, 1, 1, host_f64, c->stream);
             LAUNCH_CHECK("pack");
         } else {
-            c->L->pack(c->g, dq, c->acc, 0, 1, host_f64, c->stream);
+            if (pair && q == 4) c->L->pack_epair(c->g, dq, blk + 7 * fb, c->acc, 0, host_f64, c->stream);
+            else                c->L->pack(c->g, dq, c->acc, 0, 1, host_f64, c->stream);
             LAUNCH_CHECK("unpack");
             HIP_TRY(hipMemcpyAsync(fields[q], c->acc, n * hb, hipMemcpyDeviceToHost, c->stream));
         }
@@ -463,6 +474,7 @@ extern "C" int mpcf_stream_channels(int kind)
 extern "C" int mpcf_stream_out(mpcf_ctx* c, int kind, float* out)
 {
     if (!c || !out) return fail(MPCF_EINVAL, "mpcf_stream_out: NULL argument");
+    if (mpcf::detail::refuse_epair(c->flags, "mpcf_stream_out") != MPCF_OK) return MPCF_EINVAL;
     const int nc = mpcf_stream_channels(kind);
     if (!nc) return fail(MPCF_EINVAL, "mpcf_stream_out: unknown streamer kind");
     const size_t n = (size_t)c->g.nx * c->g.ny * c->g.nz;
@@ -478,6 +490,7 @@ extern "C" int mpcf_stream_out(mpcf_ctx* c, int kind, float* out)
 extern "C" int mpcf_stream_in(mpcf_ctx* c, const float* in)
 {
     if (!c || !in) return fail(MPCF_EINVAL, "mpcf_stream_in: NULL argument");
+    if (mpcf::detail::refuse_epair(c->flags, "mpcf_stream_in") != MPCF_OK) return MPCF_EINVAL;
     c->sos_folded = false;
     const size_t n = (size_t)c->g.nx * c->g.ny * c->g.nz;
     const Launchers* L = (c->flags & MPCF_FLAG_FP64) ? c->L : &kParity;
@@ -491,6 +504,7 @@ extern "C" int mpcf_stream_in(mpcf_ctx* c, const float* in)
 extern "C" int mpcf_load_ghosts(mpcf_ctx* c)
 {
     if (!c) return fail(MPCF_EINVAL, "mpcf_load_ghosts: NULL context");
+    if (mpcf::detail::refuse_epair(c->flags, "mpcf_load_ghosts") != MPCF_OK) return MPCF_EINVAL;
     { Timed t(c, K_GHOSTS); c->L->ghosts(c->g, c->u[c->cur], c->bc, c->stream); }
     LAUNCH_CHECK("ghosts");
     return MPCF_OK;
@@ -683,7 +697,10 @@ extern "C" int mpcf_compare(mpcf_ctx* a, mpcf_ctx* b, int which, double out[28])
     // the accumulator area of `a` holds the partials; the result follows them
     double* scratch = (double*)a->acc;
     double* res = scratch + mpcf::reduce::scratch_doubles(a->g);
-    mpcf::reduce::diff(a->g, ua, a->esize == 8, a->g.fstride, ub, b->esize == 8, b->g.fstride, scratch, res, a->stream);
+    const int ap = which == MPCF_DATA && (a->flags & MPCF_FLAG_E_PAIR) ? 1 : 0;
+    const int bp = which == MPCF_DATA && (b->flags & MPCF_FLAG_E_PAIR) ? 1 : 0;
+    mpcf::reduce::diff(a->g, ua, a->esize == 8, a->g.fstride, ap, ub, b->esize == 8, b->g.fstride, bp, scratch, res,
+                       a->stream);
     LAUNCH_CHECK("compare");
     HIP_TRY(hipMemcpyAsync(out, res, 28 * sizeof(double), hipMemcpyDeviceToHost, a->stream));
     HIP_TRY(hipStreamSynchronize(a->stream));
@@ -697,7 +714,8 @@ extern "C" int mpcf_integrals(mpcf_ctx* c, int which, double out[7])
     const void* u = which == MPCF_DATA ? c->u[c->cur] : c->R;
     double* scratch = (double*)c->acc;
     double* res = scratch + mpcf::reduce::scratch_doubles(c->g);
-    mpcf::reduce::sums(c->g, u, c->esize == 8, c->g.fstride, scratch, res, c->stream);
+    const int pair = which == MPCF_DATA && (c->flags & MPCF_FLAG_E_PAIR) ? 1 : 0;
+    mpcf::reduce::sums(c->g, u, c->esize == 8, c->g.fstride, pair, scratch, res, c->stream);
     LAUNCH_CHECK("integrals");
     double sums[7];
     HIP_TRY(hipMemcpyAsync(sums, res, sizeof(sums), hipMemcpyDeviceToHost, c->stream));

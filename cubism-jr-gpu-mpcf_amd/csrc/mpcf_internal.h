@@ -52,6 +52,7 @@ struct Launchers
     decltype(&mpcf::parity::launch_sweep) sweep;
     decltype(&mpcf::parity::launch_update) update;
     decltype(&mpcf::parity::launch_pack) pack;
+    decltype(&mpcf::parity::launch_pack_epair) pack_epair;
     decltype(&mpcf::parity::launch_stream) stream;
     decltype(&mpcf::parity::launch_test_weno) test_weno;
     decltype(&mpcf::parity::launch_test_face) test_face;
@@ -66,7 +67,8 @@ struct Launchers
     {mpcf::NS::launch_ghosts, mpcf::NS::launch_halo, mpcf::NS::launch_sos, mpcf::NS::launch_sos_folded,         \
      mpcf::NS::launch_sos_dense,                                                                                \
      mpcf::NS::launch_sweep, mpcf::NS::launch_update,                                                            \
-     mpcf::NS::launch_pack, mpcf::NS::launch_stream, mpcf::NS::launch_test_weno,                      \
+     mpcf::NS::launch_pack, mpcf::NS::launch_pack_epair, mpcf::NS::launch_stream,                      \
+     mpcf::NS::launch_test_weno,                                                                      \
      mpcf::NS::launch_test_face,                                                                      \
      mpcf::NS::batch_sos, mpcf::NS::batch_xy, mpcf::NS::batch_z, mpcf::NS::persist_stage,        \
      mpcf::NS::persist_occupancy}
@@ -74,6 +76,7 @@ struct Launchers
 inline const Launchers kParity = MPCF_LAUNCHERS(parity);
 inline const Launchers kFast = MPCF_LAUNCHERS(fast);
 inline const Launchers kFastP = MPCF_LAUNCHERS(fastp);
+inline const Launchers kFastPE = MPCF_LAUNCHERS(fastpe);
 inline const Launchers kF64 = MPCF_LAUNCHERS(f64);
 #undef MPCF_LAUNCHERS
 
@@ -110,18 +113,29 @@ inline void split_groups(const long long* cells, const int n, const int G, int* 
 }
 
 inline const unsigned kAllFlags = MPCF_FLAG_FAST | MPCF_FLAG_SOS_CPU | MPCF_FLAG_SPLIT_XY | MPCF_FLAG_FUSED_XY |
-                           MPCF_FLAG_P64 | MPCF_FLAG_FP64 | MPCF_FLAG_NO_PIPELINE;
+                           MPCF_FLAG_P64 | MPCF_FLAG_FP64 | MPCF_FLAG_NO_PIPELINE | MPCF_FLAG_E_PAIR;
 
 // arithmetic build selected by the flags
 inline const Launchers* pick_launchers(const unsigned flags)
 {
     if (flags & MPCF_FLAG_FP64) return &kF64;
+    if (flags & MPCF_FLAG_E_PAIR) return &kFastPE;
     if (flags & MPCF_FLAG_P64) return &kFastP;
     return (flags & MPCF_FLAG_FAST) ? &kFast : &kParity;
 }
 
 // storage bytes per element of a context with these flags
 inline size_t elem_bytes(const unsigned flags) { return (flags & MPCF_FLAG_FP64) ? sizeof(double) : sizeof(float); }
+
+// the entry points that read or write the solution field by field without
+// E's low part (ghost band, halos, streamers, dumps, wavelets) refuse an
+// E_PAIR context
+inline int refuse_epair(const unsigned flags, const char* who)
+{
+    if (flags & MPCF_FLAG_E_PAIR)
+        return fail(MPCF_EINVAL, std::string(who) + ": not available for an MPCF_FLAG_E_PAIR context");
+    return MPCF_OK;
+}
 
 // query helper used by mpcf_query: which x/y variant runs
 // LSRK3 coefficients (LSRK3_IntegratorMPI.cpp:46,51,56), rounded to the

@@ -32,9 +32,14 @@
 #ifndef MPCF_F64
 #define MPCF_F64 0
 #endif
+#ifndef MPCF_EPAIR
+#define MPCF_EPAIR 0
+#endif
 
 #if MPCF_F64
 #define MPCF_NS f64
+#elif MPCF_EPAIR
+#define MPCF_NS fastpe
 #elif MPCF_P64
 #define MPCF_NS fastp
 #elif MPCF_FAST
@@ -53,8 +58,14 @@ typedef float Real;
 #endif
 typedef PrimT<Real> Prim;
 constexpr bool kP64 = MPCF_P64 != 0;
+// E stored as a binary32 pair: the solution buffers hold an eighth field,
+// the low part of E (E = E[hi] + E[lo], exact in binary64), read wherever E
+// is read and written wherever E is written
+constexpr bool kEPair = MPCF_EPAIR != 0;
+constexpr int NU = 7 + (kEPair ? 1 : 0);   // fields of a solution buffer
+constexpr int FELO = 7;                   // the low part of E
 // the speed-of-sound reduction and dt/h run in binary64
-constexpr bool kSosD = MPCF_P64 != 0 || MPCF_F64 != 0;
+constexpr bool kSosD = MPCF_P64 != 0 || MPCF_F64 != 0 || MPCF_EPAIR != 0;
 // scalar type of the speed-of-sound reduction
 typedef std::conditional<kSosD, double, float>::type SosT;
 
@@ -179,8 +190,14 @@ __device__ __forceinline__ Real load_row(const StageArgs& A, const int e0, const
     Real sg;
     const int r = bc_index(row, n, A.bcmask, face_lo, sg);
 #pragma unroll
-    for (int q = 0; q < NVAR; ++q) out[q] = A.u[q*A.fs + e0 + r * s];
+    for (int q = 0; q < NU; ++q) out[q] = A.u[q*A.fs + e0 + r * s];
     return sg;
+}
+
+// primitives of a loaded row of NU conserved values
+__device__ __forceinline__ Prim prim_of(const Real* w)
+{
+    return to_prim<Real, kFast, kP64, kEPair>(w[0], w[1], w[2], w[3], w[4], w[5], w[6], kEPair ? w[FELO] : Real(0));
 }
 
 // operands of one cell's accumulator update, loaded one step ahead
@@ -189,7 +206,7 @@ struct EmitIn
 {
     Real acc[NACC];
     Real R[NVAR];
-    Real u[NVAR];
+    Real u[NU];
 };
 
 template <int MODE>
@@ -246,6 +263,7 @@ __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, cons
         const Real dU = in.acc[A_DIVU] + (F.vel - Fl.vel);
         const Real factor6 = Real(1.0f) / Real(6.0f);
         Real un[NVAR];
+        double Eun = 0.0;   // E + b R in binary64 (E pair builds)
 #pragma unroll
         for (int q = 0; q < NVAR; ++q) {
             // folded: R already holds a R - dtinvh (x+y part)
@@ -261,14 +279,26 @@ __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, cons
             const Real Rn = Rx - rhs;
             if (MODE != MODE_STAGE_NOR) A.R[q*fs + e] = Rn;
             if (is_stage(MODE)) {
-                un[q] = A.b*Rn + in.u[q];
-                A.uo[q*fs + e] = un[q];
+                if (kEPair && q == FE) {
+                    // E + b R in binary64, stored as the rounded sum and its
+                    // remainder (exact: |remainder| <= half an ulp of the sum)
+                    const double Ed = (double(in.u[FE]) + double(in.u[FELO])) + double(A.b) * double(Rn);
+                    const Real hi = Real(Ed);
+                    un[q] = hi;
+                    A.uo[q*fs + e] = hi;
+                    A.uo[FELO*fs + e] = Real(Ed - double(hi));
+                    Eun = Ed;
+                } else {
+                    un[q] = A.b*Rn + in.u[q];
+                    A.uo[q*fs + e] = un[q];
+                }
             }
         }
         // the next step's speed of sound, from the values just stored
         if (is_stage(MODE) && A.sos_next) {
             bool b;
-            smax = mmax(smax, sos_cell(A.sos_cpu != 0, un[0], un[1], un[2], un[3], un[4], un[5], un[6], b));
+            smax = mmax(smax, sos_cell(A.sos_cpu != 0, un[0], un[1], un[2], un[3], kEPair ? SosT(Eun) : SosT(un[4]),
+                                       un[5], un[6], b));
             bad = bad || b;
         }
     }
@@ -306,7 +336,8 @@ struct March
 
 __device__ __forceinline__ Prim load_prim(const Real* __restrict__ u, const int64_t fs, const int e)
 {
-    return to_prim<Real, kFast, kP64>(u[e], u[fs + e], u[2*fs + e], u[3*fs + e], u[4*fs + e], u[5*fs + e], u[6*fs + e]);
+    return to_prim<Real, kFast, kP64, kEPair>(u[e], u[fs + e], u[2*fs + e], u[3*fs + e], u[4*fs + e], u[5*fs + e],
+                                              u[6*fs + e], kEPair ? u[FELO*fs + e] : Real(0));
 }
 
 __device__ __forceinline__ Real stage_dtinvh(const StageArgs& A)
@@ -346,18 +377,18 @@ __device__ __forceinline__ SosT sweep_yz_column(const Geom& g, const StageArgs& 
     March<D, MODE> m;
     // raw conserved values of cells k-1 .. k+2 (kept for the register update;
     // the emitted cell k-1 is always interior, so its raw values are exact)
-    Real r0[NVAR], r1[NVAR], r2[NVAR], r3[NVAR], w[NVAR];
+    Real r0[NU], r1[NU], r2[NU], r3[NU], w[NU];
     r1[NEGQ] *= load_row(A, e0, c0 - 2, nd, s, FLO, r1);
     r2[NEGQ] *= load_row(A, e0, c0 - 1, nd, s, FLO, r2);
     r3[NEGQ] *= load_row(A, e0, c0, nd, s, FLO, r3);
     w[NEGQ] *= load_row(A, e0, c0 - 3, nd, s, FLO, w);
-    m.w1 = to_prim<Real, kFast, kP64>(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
-    m.w2 = to_prim<Real, kFast, kP64>(r1[0], r1[1], r1[2], r1[3], r1[4], r1[5], r1[6]);
-    m.w3 = to_prim<Real, kFast, kP64>(r2[0], r2[1], r2[2], r2[3], r2[4], r2[5], r2[6]);
-    m.w4 = to_prim<Real, kFast, kP64>(r3[0], r3[1], r3[2], r3[3], r3[4], r3[5], r3[6]);
+    m.w1 = prim_of(w);
+    m.w2 = prim_of(r1);
+    m.w3 = prim_of(r2);
+    m.w4 = prim_of(r3);
     // software pipeline: raw values of the next incoming cell and the emit
     // operands of the next step are loaded one step ahead
-    Real nxt[NVAR];
+    Real nxt[NU];
     Real nsg = load_row(A, e0, c0 + 1, nd, s, FLO, nxt);
     EmitIn<MODE> ein, ein_next;
     // one march step at row k.  FLUX: face k-1/2 is solved (k >= c0); EMIT:
@@ -367,13 +398,13 @@ __device__ __forceinline__ SosT sweep_yz_column(const Geom& g, const StageArgs& 
         nxt[NEGQ] *= nsg;   // boundary sign of the prefetched row
         // rotate the raw ring: r0 = cell k-1, r3 = cell k+2
 #pragma unroll
-        for (int q = 0; q < NVAR; ++q) { r0[q] = r1[q]; r1[q] = r2[q]; r2[q] = r3[q]; r3[q] = nxt[q]; }
-        const Prim q = to_prim<Real, kFast, kP64>(nxt[0], nxt[1], nxt[2], nxt[3], nxt[4], nxt[5], nxt[6]);  // cell k+2
+        for (int q = 0; q < NU; ++q) { r0[q] = r1[q]; r1[q] = r2[q]; r2[q] = r3[q]; r3[q] = nxt[q]; }
+        const Prim q = prim_of(nxt);  // cell k+2
         if (PF) nsg = load_row(A, e0, k + 3, nd, s, FLO, nxt);
         if (PF && FLUX) emit_load<MODE>(A, e0 + k * s, ein_next);   // emitted at step k+1
         if (kRaw) {
 #pragma unroll
-            for (int f = 0; f < NVAR; ++f) ein.u[f] = r0[f];                 // cell k-1
+            for (int f = 0; f < NU; ++f) ein.u[f] = r0[f];                   // cell k-1
         }
         m.push(q);
         m.step(A, dtinvh, FLUX, EMIT, e0 + (k - 1) * s, ein);
@@ -673,12 +704,12 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
     };
 
     March<1, MODE_ACC> m;
-    Real w[NVAR];
-    fix(w, load_row(A, e0, c0 - 3, g.ny, s, 2, w)); m.w1 = to_prim<Real, kFast, kP64>(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
-    fix(w, load_row(A, e0, c0 - 2, g.ny, s, 2, w)); m.w2 = to_prim<Real, kFast, kP64>(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
-    fix(w, load_row(A, e0, c0 - 1, g.ny, s, 2, w)); m.w3 = to_prim<Real, kFast, kP64>(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
-    fix(w, load_row(A, e0, c0, g.ny, s, 2, w));     m.w4 = to_prim<Real, kFast, kP64>(w[0], w[1], w[2], w[3], w[4], w[5], w[6]);
-    Real nxt[NVAR];
+    Real w[NU];
+    fix(w, load_row(A, e0, c0 - 3, g.ny, s, 2, w)); m.w1 = prim_of(w);
+    fix(w, load_row(A, e0, c0 - 2, g.ny, s, 2, w)); m.w2 = prim_of(w);
+    fix(w, load_row(A, e0, c0 - 1, g.ny, s, 2, w)); m.w3 = prim_of(w);
+    fix(w, load_row(A, e0, c0, g.ny, s, 2, w));     m.w4 = prim_of(w);
+    Real nxt[NU];
     Real nsg = load_row(A, e0, c0 + 1, g.ny, s, 2, nxt);
     // a first stage stores its x+y part unscaled and does not read dt
     const bool unscaled = unscaled_stage(A.a);
@@ -688,7 +719,7 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
     // k+3 is prefetched (k < c1)
     auto step = [&](auto FLUX, auto EMIT, auto PF, const int k) {
         fix(nxt, nsg);
-        const Prim q = to_prim<Real, kFast, kP64>(nxt[0], nxt[1], nxt[2], nxt[3], nxt[4], nxt[5], nxt[6]);  // row k+2
+        const Prim q = prim_of(nxt);  // row k+2
         if (PF) nsg = load_row(A, e0, k + 3, g.ny, s, 2, nxt);
         // folded builds: R of row k-1, emitted below (every lane's address is
         // an interior cell, stored or not)
@@ -909,7 +940,9 @@ __device__ __forceinline__ bool sos_body(const Geom& g, const Real* __restrict__
         for (int x = threadIdx.x; x < g.nx; x += 64) {
             const int e = e0 + x;
             bool cb;
-            sos = mmax(sos, sos_cell(CPU_FORM != 0, u[e], u[fs + e], u[2*fs + e], u[3*fs + e], u[4*fs + e],
+            const SosT Ec = (kEPair && !DENSE) ? SosT(double(u[4*fs + e]) + double(u[FELO*fs + e]))
+                                               : SosT(u[4*fs + e]);
+            sos = mmax(sos, sos_cell(CPU_FORM != 0, u[e], u[fs + e], u[2*fs + e], u[3*fs + e], Ec,
                                      u[5*fs + e], u[6*fs + e], cb));
             bad = bad || cb;
         }
@@ -984,7 +1017,40 @@ void k_update(const Geom g, Real* __restrict__ u, const Real* __restrict__ R, co
         for (int x = threadIdx.x; x < g.nx; x += 64) {
             const int e = e0 + x;
 #pragma unroll
-            for (int q = 0; q < NVAR; ++q) u[q*fs + e] = b*R[q*fs + e] + u[q*fs + e];
+            for (int q = 0; q < NVAR; ++q) {
+                if (kEPair && q == FE) {
+                    const double Ed = (double(u[FE*fs + e]) + double(u[FELO*fs + e])) + double(b) * double(R[FE*fs + e]);
+                    const Real hi = Real(Ed);
+                    u[FE*fs + e] = hi;
+                    u[FELO*fs + e] = Real(Ed - double(hi));
+                } else {
+                    u[q*fs + e] = b*R[q*fs + e] + u[q*fs + e];
+                }
+            }
+        }
+    }
+}
+
+// E of an E-pair solution <-> a dense host field: to the device, hi = the
+// value rounded to binary32 and lo = the remainder (zero for binary32 host
+// values); from the device, hi + lo (binary64), rounded once for binary32
+template <int TO_PADDED, typename Dense>
+__global__ __launch_bounds__(256)
+void k_pack_epair(const Geom g, Real* __restrict__ hi, Real* __restrict__ lo, Dense* __restrict__ dense)
+{
+    const int rows = g.ny * g.nz;
+    for (int row = blockIdx.x * 4 + threadIdx.y; row < rows; row += gridDim.x * 4) {
+        const int e0 = g.idx(0, row % g.ny, row / g.ny);
+        const int64_t d0 = (int64_t)g.nx * row;
+        for (int x = threadIdx.x; x < g.nx; x += 64) {
+            if (TO_PADDED) {
+                const double v = (double)dense[d0 + x];
+                const Real h = Real(v);
+                hi[e0 + x] = h;
+                lo[e0 + x] = Real(v - double(h));
+            } else {
+                dense[d0 + x] = (Dense)(double(hi[e0 + x]) + double(lo[e0 + x]));
+            }
         }
     }
 }
@@ -1414,6 +1480,21 @@ void launch_pack(const Geom& g, void* pad, void* dense, const int to_padded, con
     }
 }
 
+void launch_pack_epair(const Geom& g, void* hi, void* lo, void* dense, const int to_padded, const int dense_f64,
+                       hipStream_t s)
+{
+    Real* H = (Real*)hi;
+    Real* L = (Real*)lo;
+    const dim3 grid(rows_grid(g)), block(64, 4);
+    if (dense_f64) {
+        if (to_padded) hipLaunchKernelGGL((k_pack_epair<1, double>), grid, block, 0, s, g, H, L, (double*)dense);
+        else           hipLaunchKernelGGL((k_pack_epair<0, double>), grid, block, 0, s, g, H, L, (double*)dense);
+    } else {
+        if (to_padded) hipLaunchKernelGGL((k_pack_epair<1, float>), grid, block, 0, s, g, H, L, (float*)dense);
+        else           hipLaunchKernelGGL((k_pack_epair<0, float>), grid, block, 0, s, g, H, L, (float*)dense);
+    }
+}
+
 void launch_stream(const Geom& g, void* pad, float* dense, const int kind, const int out, hipStream_t s)
 {
     Real* P = (Real*)pad;
@@ -1443,6 +1524,8 @@ void launch_test_face(const int n, const int dir, const void* stv, void* outv, h
 }
 
 int default_chunk(const int nd, const long long columns) { return pick_chunk(nd, columns, 1); }
+// the march length of a whole-block launch that runs alone
+int whole_chunk(const int nd, const long long columns) { return pick_chunk(nd, columns, 2); }
 
 // grid of the fused x+y sweep: dims[0] blocks of four waves over the padded
 // rows of all planes, dims[2] y-chunks (dims[1] = 1)

@@ -1,7 +1,7 @@
-// mpcf_launch.h -- host launchers exported by the four builds of
+// mpcf_launch.h -- host launchers exported by the five builds of
 // mpcf_kernels.hip (namespace parity: bit-exact scheme; fast: FMA-contracted;
 // fastp: fast with binary64 pressure recovery and speed-of-sound reduction;
-// f64: the scheme in binary64).  Field pointers are untyped: they point to
+// fastpe: fastp with E stored as a binary32 pair; f64: the scheme in binary64).  Field pointers are untyped: they point to
 // the build's storage type (binary32, or binary64 for f64).
 #pragma once
 
@@ -106,6 +106,8 @@ void launch_sos(const Geom& g, const void* u, StepState* st, int cpu_form, int w
 void launch_sweep(const Geom& g, int dir, const SweepParams& p, int stage, hipStream_t s);                \
 void launch_update(const Geom& g, void* u, const void* R, double b, hipStream_t s);                        \
 void launch_pack(const Geom& g, void* pad, void* dense, int to_padded, int nq, int dense_f64, hipStream_t s); \
+void launch_pack_epair(const Geom& g, void* hi, void* lo, void* dense, int to_padded, int dense_f64,         \
+                       hipStream_t s);                                                                     \
 void launch_stream(const Geom& g, void* pad, float* dense, int kind, int out, hipStream_t s);             \
 void launch_test_weno(int n, const void* in5, void* m, void* p, hipStream_t s);                            \
 void launch_test_face(int n, int dir, const void* st, void* out, hipStream_t s);                           \
@@ -118,9 +120,11 @@ int persist_occupancy(int threads, int phase);                                  
 void xy_grid(const Geom& g, int chunk, int dims[3]);                                                       \
 void z_grid(const Geom& g, int chunk, int dims[3]);                                                        \
 int default_chunk(int nd, long long columns);                                                              \
+int whole_chunk(int nd, long long columns);                                                                \
 } }
 
 MPCF_DECLARE_LAUNCHERS(parity)
 MPCF_DECLARE_LAUNCHERS(fast)
 MPCF_DECLARE_LAUNCHERS(fastp)
+MPCF_DECLARE_LAUNCHERS(fastpe)
 MPCF_DECLARE_LAUNCHERS(f64)

@@ -39,11 +39,13 @@ __device__ __forceinline__ double block_max(double v, double* sh)
     return fmax(fmax(sh[0], sh[1]), fmax(sh[2], sh[3]));
 }
 
-// per field q: [0] sum |a-b|, [1] max |a-b|, [2] sum |b|, [3] max |b|
+// per field q: [0] sum |a-b|, [1] max |a-b|, [2] sum |b|, [3] max |b|.
+// apair / bpair: that block stores E as a binary32 pair, its low part in
+// field 7 (E = field 4 + field 7)
 template <typename TA, typename TB>
 __global__ __launch_bounds__(RB)
 void k_diff(const Geom g, const TA* __restrict__ a, const TB* __restrict__ b, const int64_t fsa, const int64_t fsb,
-            double* __restrict__ part)
+            const int apair, const int bpair, double* __restrict__ part)
 {
     __shared__ double sh[4];
     const int rows = g.ny * g.nz;
@@ -52,7 +54,9 @@ void k_diff(const Geom g, const TA* __restrict__ a, const TB* __restrict__ b, co
         for (int row = blockIdx.x; row < rows; row += gridDim.x) {
             const int e0 = g.idx(0, row % g.ny, row / g.ny);
             for (int x = threadIdx.x; x < g.nx; x += RB) {
-                const double va = (double)a[q*fsa + e0 + x], vb = (double)b[q*fsb + e0 + x];
+                double va = (double)a[q*fsa + e0 + x], vb = (double)b[q*fsb + e0 + x];
+                if (q == 4 && apair) va += (double)a[7*fsa + e0 + x];
+                if (q == 4 && bpair) vb += (double)b[7*fsb + e0 + x];
                 const double d = fabs(va - vb);
                 s1 += d; m1 = fmax(m1, d);
                 s2 += fabs(vb); m2 = fmax(m2, fabs(vb));
@@ -69,7 +73,7 @@ void k_diff(const Geom g, const TA* __restrict__ a, const TB* __restrict__ b, co
 
 template <typename T>
 __global__ __launch_bounds__(RB)
-void k_sums(const Geom g, const T* __restrict__ u, const int64_t fs, double* __restrict__ part)
+void k_sums(const Geom g, const T* __restrict__ u, const int64_t fs, const int pair, double* __restrict__ part)
 {
     __shared__ double sh[4];
     const int rows = g.ny * g.nz;
@@ -77,7 +81,10 @@ void k_sums(const Geom g, const T* __restrict__ u, const int64_t fs, double* __r
         double s = 0;
         for (int row = blockIdx.x; row < rows; row += gridDim.x) {
             const int e0 = g.idx(0, row % g.ny, row / g.ny);
-            for (int x = threadIdx.x; x < g.nx; x += RB) s += (double)u[q*fs + e0 + x];
+            for (int x = threadIdx.x; x < g.nx; x += RB) {
+                s += (double)u[q*fs + e0 + x];
+                if (q == 4 && pair) s += (double)u[7*fs + e0 + x];
+            }
         }
         s = block_sum(s, sh);
         if (threadIdx.x == 0) part[(int64_t)blockIdx.x * 7 + q] = s;
@@ -111,25 +118,25 @@ static int nblocks(const Geom& g)
 
 int scratch_doubles(const Geom& g) { return nblocks(g) * 28 + 64; }
 
-void diff(const Geom& g, const void* a, int a_f64, int64_t fsa, const void* b, int b_f64, int64_t fsb,
-          double* scratch, double* out28, hipStream_t s)
+void diff(const Geom& g, const void* a, int a_f64, int64_t fsa, int a_pair, const void* b, int b_f64, int64_t fsb,
+          int b_pair, double* scratch, double* out28, hipStream_t s)
 {
     const int nb = nblocks(g);
-    if (a_f64 && b_f64)  hipLaunchKernelGGL((k_diff<double, double>), dim3(nb), dim3(RB), 0, s, g, (const double*)a, (const double*)b, fsa, fsb, scratch);
-    else if (a_f64)      hipLaunchKernelGGL((k_diff<double, float>), dim3(nb), dim3(RB), 0, s, g, (const double*)a, (const float*)b, fsa, fsb, scratch);
-    else if (b_f64)      hipLaunchKernelGGL((k_diff<float, double>), dim3(nb), dim3(RB), 0, s, g, (const float*)a, (const double*)b, fsa, fsb, scratch);
-    else                 hipLaunchKernelGGL((k_diff<float, float>), dim3(nb), dim3(RB), 0, s, g, (const float*)a, (const float*)b, fsa, fsb, scratch);
+    if (a_f64 && b_f64)  hipLaunchKernelGGL((k_diff<double, double>), dim3(nb), dim3(RB), 0, s, g, (const double*)a, (const double*)b, fsa, fsb, a_pair, b_pair, scratch);
+    else if (a_f64)      hipLaunchKernelGGL((k_diff<double, float>), dim3(nb), dim3(RB), 0, s, g, (const double*)a, (const float*)b, fsa, fsb, a_pair, b_pair, scratch);
+    else if (b_f64)      hipLaunchKernelGGL((k_diff<float, double>), dim3(nb), dim3(RB), 0, s, g, (const float*)a, (const double*)b, fsa, fsb, a_pair, b_pair, scratch);
+    else                 hipLaunchKernelGGL((k_diff<float, float>), dim3(nb), dim3(RB), 0, s, g, (const float*)a, (const float*)b, fsa, fsb, a_pair, b_pair, scratch);
     unsigned long long mask = 0;
     for (int i = 0; i < 28; ++i)
         if (i % 4 == 1 || i % 4 == 3) mask |= 1ull << i;
     hipLaunchKernelGGL(k_fold, dim3(1), dim3(RB), 0, s, scratch, nb, 28, mask, out28);
 }
 
-void sums(const Geom& g, const void* u, int f64, int64_t fs, double* scratch, double* out7, hipStream_t s)
+void sums(const Geom& g, const void* u, int f64, int64_t fs, int pair, double* scratch, double* out7, hipStream_t s)
 {
     const int nb = nblocks(g);
-    if (f64) hipLaunchKernelGGL((k_sums<double>), dim3(nb), dim3(RB), 0, s, g, (const double*)u, fs, scratch);
-    else     hipLaunchKernelGGL((k_sums<float>), dim3(nb), dim3(RB), 0, s, g, (const float*)u, fs, scratch);
+    if (f64) hipLaunchKernelGGL((k_sums<double>), dim3(nb), dim3(RB), 0, s, g, (const double*)u, fs, pair, scratch);
+    else     hipLaunchKernelGGL((k_sums<float>), dim3(nb), dim3(RB), 0, s, g, (const float*)u, fs, pair, scratch);
     hipLaunchKernelGGL(k_fold, dim3(1), dim3(RB), 0, s, scratch, nb, 7, 0ull, out7);
 }
 

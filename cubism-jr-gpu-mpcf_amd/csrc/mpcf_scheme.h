@@ -91,12 +91,15 @@ typedef PrimT<Real> Prim;
 // G*p is a few units, so E - KE - P cancels all but the last few bits of a
 // binary32 evaluation; in binary64 that cancellation is exact and p carries
 // only the storage error of E.
-template <typename T, bool FAST = false, bool P64 = false>
-MPCF_HD PrimT<T> to_prim(T r, T ru, T rv, T rw, T E, T G, T P)
+//
+// EPAIR (E stored as a binary32 pair, E = hi + lo; implies P64): Elo is the
+// low part, and the recovery uses hi + lo, exact in binary64.
+template <typename T, bool FAST = false, bool P64 = false, bool EPAIR = false>
+MPCF_HD PrimT<T> to_prim(T r, T ru, T rv, T rw, T E, T G, T P, T Elo = T())
 {
     PrimT<T> q;
     q.r = r;
-    if constexpr (P64) {
+    if constexpr (P64 || EPAIR) {
         const double rd = r;
         const double ird = 1.0 / rd;
         const double ud = ru * ird, vd = rv * ird, wd = rw * ird;
@@ -104,7 +107,8 @@ MPCF_HD PrimT<T> to_prim(T r, T ru, T rv, T rw, T E, T G, T P)
         q.v = T(vd);
         q.w = T(wd);
         const double ke = 0.5 * (double(ru)*ud + double(rv)*vd + double(rw)*wd);
-        q.p = T(((double(E) - ke) - double(P)) / double(G));
+        const double Ed = EPAIR ? double(E) + double(Elo) : double(E);
+        q.p = T(((Ed - ke) - double(P)) / double(G));
     } else if (FAST) {
         const T ir = mrcp(r);
         q.u = ru * ir;

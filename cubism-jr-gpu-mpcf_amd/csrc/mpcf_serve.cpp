@@ -483,8 +483,14 @@ bool rebuild(mpcf_server* S)
         int* oz = oxy + (cap + 1);
         int* os = oxy + 2 * (cap + 1);
         int blocks[3];
-        mpcf::detail::case_desc(c, c->cur, mpcf::detail::adaptive_march(c->g.ny, cols_xy),
-                                mpcf::detail::adaptive_march(c->g.nz, cols_z), hc[i], blocks);
+        // a lone case marches as a whole-block single-case launch would
+        // (shorter chunks, more waves to fill the chip); a mix shares the
+        // chip and takes the longer, cheaper chunks of adaptive_march
+        const int lxy = n == 1 ? mpcf::parity::whole_chunk(c->g.ny, (long long)c->g.nx * c->g.nz)
+                               : mpcf::detail::adaptive_march(c->g.ny, cols_xy);
+        const int lz = n == 1 ? mpcf::parity::whole_chunk(c->g.nz, (long long)c->g.nx * c->g.ny)
+                              : mpcf::detail::adaptive_march(c->g.nz, cols_z);
+        mpcf::detail::case_desc(c, c->cur, lxy, lz, hc[i], blocks);
         oxy[j + 1] = oxy[j] + blocks[0];
         oz[j + 1] = oz[j] + blocks[1];
         os[j + 1] = os[j] + blocks[2];

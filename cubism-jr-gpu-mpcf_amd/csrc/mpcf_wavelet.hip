@@ -476,6 +476,7 @@ extern "C" int mpcf_wavelet_dump(mpcf_ctx* c, int channel, const char* path, flo
                                  double stats[3])
 {
     if (!c || !path) return fail(MPCF_EINVAL, "mpcf_wavelet_dump: NULL argument");
+    if (mpcf::detail::refuse_epair(c->flags, "mpcf_wavelet_dump") != MPCF_OK) return MPCF_EINVAL;
     if (channel < 0 || channel > 6) return fail(MPCF_EINVAL, "mpcf_wavelet_dump: channel must be 0..6");
     const int n = c->g.nx;
     if (c->g.ny != n || c->g.nz != n || !cube_size_ok(n))

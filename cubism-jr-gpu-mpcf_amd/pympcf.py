@@ -20,6 +20,7 @@ OK, EINVAL, EHIP, ENOMEM, ENUMERIC, EBUSY = 0, 1, 2, 3, 4, 5
 BC_ABSORBING, BC_REFLECTING, BC_HALO = 0, 1, 2
 FLAG_FAST, FLAG_SOS_CPU, FLAG_SPLIT_XY, FLAG_FUSED_XY, FLAG_P64, FLAG_FP64 = 1, 2, 4, 8, 16, 32
 FLAG_NO_PIPELINE = 128
+FLAG_E_PAIR = 256
 DATA, TMP = 0, 1
 BATCH_GROUPS, BATCH_PERSISTENT = 0, 1
 NVAR = 7
@@ -237,18 +238,21 @@ class Block:
     """One resident node block (an mpcf_ctx)."""
 
     def __init__(self, nx, ny, nz, h, fast=False, sos_cpu=False, bc=None, split_xy=None, p64=False, fp64=False,
-                 pipeline=True):
+                 pipeline=True, epair=False):
         """split_xy: None = the library's default for the build, True/False = force.
         p64: binary32 storage with binary64 pressure recovery and speed-of-sound
-        reduction (FAST arithmetic otherwise); fp64: the whole scheme in binary64.
+        reduction (FAST arithmetic otherwise); fp64: the whole scheme in binary64;
+        epair: p64 with E stored as a binary32 pair (hi + lo).
         pipeline: True = the library's choice of z-slabs for mpcf_step, False =
         whole-block sweeps in every stage (MPCF_FLAG_NO_PIPELINE), an int k >= 2
         = k slabs (mpcf_set_pipeline)."""
         self.shape = (nz, ny, nx)
         self.h = float(h)
+        p64 = p64 or epair
         fast = fast or p64
         flags = (FLAG_FAST if fast else 0) | (FLAG_SOS_CPU if sos_cpu else 0)
         flags |= (FLAG_P64 if p64 else 0) | (FLAG_FP64 if fp64 else 0) | (0 if pipeline is not False else FLAG_NO_PIPELINE)
+        flags |= FLAG_E_PAIR if epair else 0
         if split_xy is True:
             flags |= FLAG_SPLIT_XY
         elif split_xy is False:

@@ -61,6 +61,16 @@ enum {
                                 reduction in binary64 (implies MPCF_FLAG_FAST)                      */
     MPCF_FLAG_NO_PIPELINE = 128u, /* start with mpcf_set_pipeline(ctx, 0): every stage of mpcf_step as
                                 whole-block sweeps                                                */
+    MPCF_FLAG_E_PAIR = 256u, /* MPCF_FLAG_P64 with the total energy stored as a binary32 pair, E = hi +
+                                lo, exact in binary64 (the other fields stay binary32): the pressure
+                                recovery, the register update of E and the speed of sound use hi + lo,
+                                so p no longer carries the binary32 storage error of E (half an ulp of
+                                E ~ gamma pc/(gamma-1) in liquid cells, divided by G).  Implies P64;
+                                excludes FP64 and SPLIT_XY.  Transfers, steps, mpcf_compare and
+                                mpcf_integrals see E = hi + lo; the ghost band, halo, streamer, dump,
+                                checkpoint and wavelet entry points refuse such a context.
+                                No reference counterpart (the reference stores E in binary32,
+                                NodeBlock.h:47-48).                                                */
     MPCF_FLAG_FP64 = 32u     /* the whole scheme in binary64 (storage, arithmetic, reductions): the
                                 high-precision run the binary32 builds are measured against.  Host
                                 arrays stay binary32 for mpcf_upload/mpcf_download (rounded on the way

@@ -42,6 +42,8 @@ def test_error_path_without_device_work(mpcf):
     assert lib.mpcf_create(ctypes.byref(h), 4, 8, 8, 0.1, 0) == mpcf.EINVAL   # extents >= 5
     assert lib.mpcf_create(ctypes.byref(h), 8, 8, 8, -1.0, 0) == mpcf.EINVAL  # h > 0
     assert lib.mpcf_create(ctypes.byref(h), 8, 8, 8, 0.1, 64) == mpcf.EINVAL  # unknown flag
+    assert lib.mpcf_create(ctypes.byref(h), 8, 8, 8, 0.1, 256 | 32) == mpcf.EINVAL  # E_PAIR with FP64
+    assert lib.mpcf_create(ctypes.byref(h), 8, 8, 8, 0.1, 256 | 4) == mpcf.EINVAL   # E_PAIR with SPLIT_XY
     assert lib.mpcf_set_bc(None, None) == mpcf.EINVAL
     assert lib.mpcf_step(None, 0.3, 1.0, None) == mpcf.EINVAL
 

@@ -134,3 +134,72 @@ def test_precision_driver_small_cloud(mpcf):
             bound = 1e-2 if f.startswith("rho_") else 1e-4
             assert np.isfinite(v["rel_l1"][f]) and v["rel_l1"][f] < bound, (name, f, v["rel_l1"][f])
         assert v["dt_own_rel_err_max"] < 1e-5, (name, v["dt_own_rel_err_max"])
+
+
+def test_epair_transfers_keep_e_in_binary64(mpcf):
+    """MPCF_FLAG_E_PAIR stores E as hi + lo: a binary64 E survives the upload
+    / download round trip to ~2^-48 relative, the other fields are rounded
+    to binary32, and a binary32 upload gives lo = 0 (E comes back exactly)."""
+    s = states.random_smooth(13, 11, 9, seed=5).astype(np.float64)
+    rng = np.random.default_rng(9)
+    s[4] *= 1.0 + 1e-9 * rng.standard_normal(s[4].shape)       # bits below binary32
+    nz, ny, nx = s.shape[1:]
+    say("E pair transfers", (nx, ny, nz))
+    with mpcf.Block(nx, ny, nz, 1.0 / nx, epair=True) as b, mpcf.Block(nx, ny, nz, 1.0 / nx, p64=True) as p:
+        b.upload_f64(mpcf.DATA, s)
+        p.upload_f64(mpcf.DATA, s)
+        got, plain = b.download_f64(), p.download_f64()
+        assert np.abs(got[4] - s[4]).max() <= 2.0 ** -48 * np.abs(s[4]).max()
+        assert np.abs(plain[4] - s[4]).max() > 2.0 ** -30 * np.abs(s[4]).max()
+        for q in (0, 1, 2, 3, 5, 6):
+            assert np.array_equal(got[q], s[q].astype(np.float32).astype(np.float64)), q
+        s32 = s.astype(np.float32)
+        b.data = s32
+        assert np.array_equal(b.download_f64()[4], s32[4].astype(np.float64))
+        assert np.array_equal(b.data, s32)
+        # the field-by-field paths without E's low part refuse the context
+        with pytest.raises(mpcf.MpcfError) as e:
+            b.load_ghosts()
+        assert e.value.code == mpcf.EINVAL
+
+
+@pytest.mark.parametrize("slabs", [3, False])
+def test_epair_steps_independent_of_launch_structure(mpcf, slabs):
+    """E pair: mpcf_steps over z-slabs (or whole-block) equals single
+    whole-block mpcf_step calls bit for bit, E's low part included."""
+    n, nz = 40, 192
+    s = states.random_smooth(n, n, nz, seed=61)
+    say("E pair steps, slabs", slabs)
+    with mpcf.Block(n, n, nz, 1.0 / nz, epair=True, pipeline=slabs) as a, \
+            mpcf.Block(n, n, nz, 1.0 / nz, epair=True, pipeline=False) as b:
+        a.data = s
+        b.data = s
+        a.steps(4, 0.3)
+        da = a.last_dt()[0]
+        for _ in range(4):
+            db = b.step(0.3)
+        assert da == db
+        assert np.array_equal(a.download_f64(), b.download_f64())
+        assert np.array_equal(a.tmp, b.tmp)
+
+
+def test_epair_error_below_p64(mpcf):
+    """The bubble-cloud collapse at 64^3 with the binary64 run's dt sequence:
+    storing E as a pair lowers the error against the binary64 run of every
+    field and of the pressure recovered from E (the runs are deterministic:
+    measured E 0.81x and p 0.70x the P64 build's).  Neither build reaches the
+    binary32 storage floor of E (the binary64 final state rounded to
+    binary32): the binary32 WENO / HLLC arithmetic, not the storage of E,
+    sets most of the error."""
+    import precision
+    say("E pair vs p64, bubble cloud 64^3 x 40 steps")
+    res = precision.run(n=64, steps=40, nbubbles=20, seed=2, variants=("p64", "p64e"),
+                        log=lambda m: say("  " + m))
+    p64, pe = res["variants"]["p64"], res["variants"]["p64e"]
+    fl = res["storage_floor"]
+    say(f"  E rel L1: p64 {p64['rel_l1']['E']:.3e}, p64e {pe['rel_l1']['E']:.3e}, floor {fl['E'][0]:.3e}")
+    say(f"  p rel L1: p64 {p64['p_rel_l1']:.3e}, p64e {pe['p_rel_l1']:.3e}, floor {fl['p'][0]:.3e}")
+    assert pe["rel_l1"]["E"] < 0.9 * p64["rel_l1"]["E"]
+    assert pe["p_rel_l1"] < 0.8 * p64["p_rel_l1"]
+    for f in precision.FIELDS:
+        assert pe["rel_l1"][f] <= p64["rel_l1"][f], f

@@ -54,12 +54,16 @@ def test_precision_report_markdown():
     import precision
     F = precision.FIELDS
     v = {"rel_l1": {f: 1e-7 for f in F}, "rel_linf": {f: 2e-5 for f in F},
-         "integral_diff": {f: 3e-9 for f in F}, "dt_own_rel_err_max": 1e-6, "seconds": 1.5}
+         "integral_diff": {f: 3e-9 for f in F}, "dt_own_rel_err_max": 1e-6, "seconds": 1.5,
+         "p_rel_l1": 4e-6, "p_rel_linf": 5e-5}
     res = {"workload": "bubble-cloud collapse 32^3, 2 bubbles, 3 LSRK3 steps, CFL 0.3",
-           "fp64": {"ms_per_step": 2.0, "t_end": 1e-3}, "variants": {"parity": v, "fast": v}}
+           "fp64": {"ms_per_step": 2.0, "t_end": 1e-3}, "variants": {"parity": v, "fast": v},
+           "storage_floor": {"E": (3e-8, 6e-8), "p": (2e-6, 9e-5)}}
     md = precision.to_markdown(res, "python3 precision.py --n 32")
     assert md.startswith("# Precision")
     assert "| parity | 1.00e-07 |" in md and "| fast |" in md
+    assert "| parity | 1.00e-07 | 2.00e-05 | 4.00e-06 | 5.00e-05 |" in md
+    assert "| binary32 storage floor | 3.00e-08 | 6.00e-08 | 2.00e-06 | 9.00e-05 |" in md
     assert "python3 precision.py --n 32" in md
 
 
