@@ -1059,7 +1059,7 @@ int batch_enqueue_step(mpcf_batch* b, double cfl, double dt_max)
         b->L->b_sos(d, b->tot_sos[grp], cpu_form, cfl, dt_max, st);
         for (int s = 0; s < 3; ++s) {
             b->L->b_xy(d, b->tot_xy[grp], kA[s], kB[s], st);
-            b->L->b_z(d, b->tot_z[grp], kA[s], kB[s], s == 2, cpu_form, st);
+            b->L->b_z(d, b->tot_z[grp], kA[s], kB[s], s == 2, cpu_form, 1, st);
         }
         if (grp > 0) HIP_TRY(hipEventRecord(b->join[grp], st));
     }

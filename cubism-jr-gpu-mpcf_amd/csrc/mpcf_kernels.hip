@@ -1221,9 +1221,13 @@ void kb_sweep_xy(const BatchDev B, const Real a, const Real b)
 }
 
 // the z sweep finishes the stage; the last block of each case to finish
-// flips the case's current-buffer index (every block has read it by then)
+// flips the case's current-buffer index (every block has read it before it
+// counts itself: no fence is needed, and a device-scope fence here would
+// write back and invalidate the XCD's L2 once per block).
 // fold != 0 (a step's last stage): the sweep also folds the maximum speed of
-// sound of the state it writes, for the next step's kb_sos
+// sound of the state it writes, for the next step's kb_sos.  MODE_STAGE_NOR:
+// R is not stored (a caller that never reads R after a step)
+template <int MODE>
 __global__ __launch_bounds__(256)
 void kb_sweep_z(const BatchDev B, const Real a, const Real b, const int fold, const int cpu_form)
 {
@@ -1232,10 +1236,9 @@ void kb_sweep_z(const BatchDev B, const Real a, const Real b, const int fold, co
     const int local = blockIdx.x - B.off_z[c];
     StageArgs A = case_args(C, B.cur[c], a, b);
     if (fold) { A.sos_next = C.st; A.sos_cpu = cpu_form; }
-    sweep_yz_body<2, MODE_STAGE>(C.g, A, C.chunk_z, (local % C.nbx_z) * 256 + threadIdx.x, local / C.nbx_z);
+    sweep_yz_body<2, MODE>(C.g, A, C.chunk_z, (local % C.nbx_z) * 256 + threadIdx.x, local / C.nbx_z);
     __syncthreads();
     if (threadIdx.x == 0) {
-        __threadfence();
         const unsigned nblk = (unsigned)(B.off_z[c + 1] - B.off_z[c]);
         if (atomicAdd(&C.st->z_count, 1u) == nblk - 1u) {
             atomicExch(&B.cur[c], 1 - B.cur[c]);
@@ -1553,10 +1556,12 @@ void batch_xy(const BatchDev& B, const int blocks, const double a, const double 
     hipLaunchKernelGGL(kb_sweep_xy, dim3(blocks), dim3(64, 4), 0, s, B, (Real)a, (Real)b);
 }
 
+// storeR == 0: R is not stored (MODE_STAGE_NOR)
 void batch_z(const BatchDev& B, const int blocks, const double a, const double b, const int fold, const int cpu_form,
-             hipStream_t s)
+             const int storeR, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_sweep_z, dim3(blocks), dim3(256), 0, s, B, (Real)a, (Real)b, fold, cpu_form);
+    if (storeR) hipLaunchKernelGGL(kb_sweep_z<MODE_STAGE>, dim3(blocks), dim3(256), 0, s, B, (Real)a, (Real)b, fold, cpu_form);
+    else        hipLaunchKernelGGL(kb_sweep_z<MODE_STAGE_NOR>, dim3(blocks), dim3(256), 0, s, B, (Real)a, (Real)b, fold, cpu_form);
 }
 // the two launches of one stage; blocks[0] workgroups for the x+y items,
 // blocks[1] for the z items

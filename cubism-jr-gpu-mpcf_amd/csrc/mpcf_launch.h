@@ -113,7 +113,8 @@ void launch_test_weno(int n, const void* in5, void* m, void* p, hipStream_t s); 
 void launch_test_face(int n, int dir, const void* st, void* out, hipStream_t s);                           \
 void batch_sos(const BatchDev& B, int blocks, int cpu_form, double cfl, double dt_max, hipStream_t s);    \
 void batch_xy(const BatchDev& B, int blocks, double a, double b, hipStream_t s);                           \
-void batch_z(const BatchDev& B, int blocks, double a, double b, int fold, int cpu_form, hipStream_t s);    \
+void batch_z(const BatchDev& B, int blocks, double a, double b, int fold, int cpu_form, int storeR,         \
+             hipStream_t s);                                                                               \
 void persist_stage(const PersistDev& P, const int blocks[2], int threads, double a, double b, int fold,     \
                    int cpu_form, hipStream_t s);                                                           \
 int persist_occupancy(int threads, int phase);                                                             \

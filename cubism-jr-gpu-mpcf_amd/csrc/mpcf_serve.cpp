@@ -406,10 +406,21 @@ void choose_active(mpcf_server* S, std::vector<Req*>& act)
         }
         long long budget = S->fill_cells;
         if (S->policy == MPCF_SERVE_FIFO_SLACK) {
+            std::stable_sort(live.begin(), live.end(), by_age);
+            if (live.size() <= 2) {
+                // light load: the oldest request steps as if alone, except for
+                // a companion whose remaining work is at most a tenth of its
+                // own (it finishes quickly and delays the oldest by at most
+                // that much), which would otherwise wait for the whole of it
+                auto work = [](const Req* r) { return r->cells() * (long long)r->left; };
+                act.push_back(live[0]);
+                if (live.size() == 2 && 10 * work(live[1]) <= work(live[0])) act.push_back(live[1]);
+                std::sort(act.begin(), act.end(), [](const Req* a, const Req* b) { return a->seq < b->seq; });
+                return;
+            }
             // the oldest request steps at close to its own speed: companions
             // may add at most a quarter of its cells (or fill the floor budget
             // when it is small)
-            std::stable_sort(live.begin(), live.end(), by_age);
             budget = std::max(budget, live.front()->cells() + live.front()->cells() / 4);
         }
         long long cells = 0;
@@ -536,7 +547,9 @@ bool enqueue_step_kernels(mpcf_server* S)
         S->L->b_sos(d, S->tot[grp][2], cpu_form, S->cfl, 1e30, st);
         for (int s = 0; s < 3; ++s) {
             S->L->b_xy(d, S->tot[grp][0], kA[s], kB[s], st);
-            S->L->b_z(d, S->tot[grp][1], kA[s], kB[s], s == 2, cpu_form, st);
+            // a request's R is never handed back, and the next step's first
+            // stage multiplies it by a = 0: the last stage does not store it
+            S->L->b_z(d, S->tot[grp][1], kA[s], kB[s], s == 2, cpu_form, s != 2, st);
         }
         if (grp > 0 && hipEventRecord(S->join[grp], st) != hipSuccess) return false;
     }

@@ -587,7 +587,9 @@ enum {
     MPCF_SERVE_SRPT_FILL = 2,  /* least remaining work (cells x steps) first, same fill       */
     MPCF_SERVE_HYBRID = 3,     /* the oldest request, then least remaining work, same fill    */
     MPCF_SERVE_FIFO_SLACK = 4  /* oldest first; budget max(fill_cells, 1.25 x the oldest's
-                                  cells): the oldest steps at close to its solo speed        */
+                                  cells): the oldest steps at close to its solo speed.  With
+                                  two live requests or fewer, the oldest steps alone unless
+                                  the other's remaining work is at most a tenth of its own */
 };
 
 /* server flag: launch each step eagerly instead of replaying a hipGraph */
