@@ -21,7 +21,9 @@
 //   active is the policy: all live cases; or, filled up to a cell budget, the
 //   oldest first, those with the least remaining work first, or the oldest
 //   one and then the least remaining work; or the oldest first with the
-//   budget raised to 1.25x the oldest request's cells.
+//   budget raised to 1.25x the oldest request's cells.  A request of 2^18
+//   cells or more that steps alone runs the whole-block kernels of
+//   mpcf_step instead, enqueued eagerly.
 // * Each case's current-buffer index is mirrored on the host (a step flips it
 //   once), so membership changes need no read-back.  At most `depth` steps are
 //   in flight; the engine waits on the oldest beyond that.
@@ -121,6 +123,12 @@ struct Req
     mpcf_ctx* c = nullptr;
     int left = 0;
     long long seq = 0;                 // admission order
+    // a step has been enqueued: its last z sweep folded the maximum speed of
+    // sound of the state it wrote into the step state's next_sos_bits
+    bool stepped = false;
+    // the step state's device flag `folded` is set (the batched speed-of-sound
+    // launch then takes dt from the folded maximum instead of reading u)
+    bool dev_folded = false;
     long long cells() const { return (long long)nx * ny * nz; }
 };
 
@@ -486,7 +494,14 @@ bool rebuild(mpcf_server* S)
     for (int grp = 0; grp < kServerGroups; ++grp)
         hoff[grp * tab] = hoff[grp * tab + (cap + 1)] = hoff[grp * tab + 2 * (cap + 1)] = 0;
     for (int i = 0; i < n; ++i) {
-        mpcf_ctx* c = S->active[i]->c;
+        Req* r = S->active[i];
+        mpcf_ctx* c = r->c;
+        if (r->stepped && !r->dev_folded) {
+            // its steps so far ran alone on the whole-block kernels, which
+            // fold the maximum but leave the batch's flag to the host
+            if (hipMemsetD32Async((hipDeviceptr_t)&c->st->folded, 1, 1, S->stream) != hipSuccess) return false;
+            r->dev_folded = true;
+        }
         int grp = 0;
         while (i >= first[grp + 1]) ++grp;
         const int j = i - first[grp];                                      // index within the group
@@ -555,6 +570,52 @@ bool enqueue_step_kernels(mpcf_server* S)
     }
     for (int grp = 1; grp < kServerGroups; ++grp)
         if (S->dev[grp].n > 0 && hipStreamWaitEvent(S->stream, S->join[grp], 0) != hipSuccess) return false;
+    return true;
+}
+
+// A request stepped alone that is large enough to fill the GPU by itself runs
+// the kernels a whole-block mpcf_step launches (the batched launches cost it
+// 6-12 % at 128^3-192^3: the per-workgroup case lookup and the tile order of
+// a mixed batch).  Its buffers alternate between steps, so these launches are
+// enqueued eagerly, not replayed from a graph; below this size the launch
+// overhead outweighs the difference and the batch graph steps it.
+constexpr long long kSoloCells = 1LL << 18;
+
+bool solo_step(const mpcf_server* S)
+{
+    return S->active.size() == 1 && S->active[0]->cells() >= kSoloCells;
+}
+
+// one step of a lone request on the whole-block kernels: dt (from the maximum
+// its previous step folded, or a full reduction before its first step), then
+// per stage the x+y sweep(s) and the z sweep, the last stage without the R
+// store and folding the next step's maximum
+bool enqueue_solo_step(mpcf_server* S, Req* r)
+{
+    mpcf_ctx* c = r->c;
+    const int cpu_form = (S->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0;
+    if (r->stepped) S->L->sos_folded(c->st, S->cfl, c->h, 1e30, S->stream);
+    else S->L->sos(c->g, c->u[c->cur], c->st, cpu_form, 1, S->cfl, c->h, 1e30, S->stream);
+    const bool split = !(S->flags & MPCF_FLAG_FAST);   // as a context's default
+    int cur = c->cur;
+    for (int s = 0; s < 3; ++s) {
+        mpcf::SweepParams p;
+        p.u = c->u[cur]; p.R = c->R; p.acc = c->acc; p.uo = c->u[1 - cur];
+        p.a = kA[s]; p.b = kB[s]; p.dtinvh = 0.0; p.st = c->st;
+        p.sos_next = s == 2 ? c->st : nullptr;
+        p.sos_cpu = cpu_form;
+        p.bcmask = mpcf::detail::bc_mask(c->bc);
+        if (split) {
+            S->L->sweep(c->g, 0, p, 1, S->stream);
+            S->L->sweep(c->g, 1, p, 1, S->stream);
+        } else {
+            S->L->sweep(c->g, 3, p, 1, S->stream);
+        }
+        S->L->sweep(c->g, 2, p, s == 2 ? 2 : 1, S->stream);
+        cur = 1 - cur;
+    }
+    if (hipGetLastError() != hipSuccess) return false;
+    ++S->st_steps;
     return true;
 }
 
@@ -649,9 +710,15 @@ void engine_main(mpcf_server* S)
             S->active_seq = seq;
             if (!rebuild(S)) { set_error(S, MPCF_EHIP, "server: table upload failed"); break; }
         }
-        if (!enqueue_step(S)) { set_error(S, MPCF_EHIP, "server: step launch failed"); break; }
+        const bool solo = solo_step(S);
+        if (!(solo ? enqueue_solo_step(S, S->active[0]) : enqueue_step(S))) {
+            set_error(S, MPCF_EHIP, "server: step launch failed");
+            break;
+        }
         S->st_peak_live = std::max<long long>(S->st_peak_live, (long long)S->active.size());
         for (Req* r : S->active) {
+            r->stepped = true;
+            if (!solo) r->dev_folded = true;   // the batched z sweep sets the flag
             r->c->cur ^= 1;
             if (--r->left == 0 && !finish(S, r)) { set_error(S, MPCF_EHIP, "server: download failed"); return; }
         }

@@ -592,7 +592,10 @@ enum {
                                   the other's remaining work is at most a tenth of its own */
 };
 
-/* server flag: launch each step eagerly instead of replaying a hipGraph */
+/* server flag: launch each step eagerly instead of replaying a hipGraph.
+ * Without it, the batch steps are replayed from a graph; a request of 2^18
+ * cells or more that steps alone runs the whole-block kernels of mpcf_step,
+ * enqueued eagerly either way. */
 enum { MPCF_FLAG_SERVE_EAGER = 64u };
 
 /* Create a server on the current device: `hbm_bytes` of device memory for

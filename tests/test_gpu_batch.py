@@ -184,6 +184,51 @@ def test_server_results_equal_alone(mpcf, policy, fill, graph):
         assert np.array_equal(got[j], u), j
 
 
+def test_server_lone_request_runs_whole_block_kernels(mpcf):
+    """A lone request of 2^18 cells or more steps on the whole-block kernels,
+    enqueued eagerly (no graph is captured); a smaller lone one replays the
+    batch graph.  Both return exactly the state they reach alone."""
+    big = mpcf.ic_bubble_collapse(72, 64, 64, 0.2, 100.0)        # 294912 cells
+    small = states.random_smooth(20, 20, 20, seed=1)
+    for s, n, solo in ((big, 7, True), (small, 4, False)):
+        say("server lone request", s.shape[1:], "solo" if solo else "batch graph")
+        with mpcf.Server(1 << 30, policy="fifo_slack", fill_cells=1 << 20, max_live=4, fast=True, graph=True) as S:
+            i = S.submit(s, n, h=1.0 / max(s.shape[1:]))
+            S.wait(60.0)
+            st = S.stats()
+            got = S.results[i]
+        assert st["steps"] == n and st["captures"] == (0 if solo else 1), st
+        u, _, _ = alone(mpcf, s, [0] * 6, n, fast=True)
+        assert np.array_equal(got, u)
+
+
+@pytest.mark.parametrize("policy", ["fifo_slack", "all"])
+def test_server_lone_request_joined_and_left(mpcf, policy):
+    """A large request steps alone (whole-block kernels), is joined by small
+    ones (batch graph), and finishes alone again; every result is exactly the
+    state it reaches alone."""
+    import time
+    big = mpcf.ic_bubble_collapse(72, 64, 64, 0.2, 100.0)
+    small = cases(mpcf)[:3]
+    steps = [3, 5, 2]
+    say("server lone request joined and left,", policy)
+    with mpcf.Server(1 << 30, policy=policy, fill_cells=1 << 22, max_live=4, fast=True, graph=True) as S:
+        ib = S.submit(big, 150, h=1.0 / 72)
+        time.sleep(0.004)        # the large request runs alone for a while
+        ids = [S.submit(s, n, h=1.0 / max(s.shape[1:]), bc=bc) for (s, bc), n in zip(small, steps)]
+        S.wait(60.0)
+        st = S.stats()
+        got_big = S.results[ib]
+        got = [S.results[i] for i in ids]
+    say("  stats", st)
+    assert st["completed"] == 4 and st["steps"] >= 150, st
+    u, _, _ = alone(mpcf, big, [0] * 6, 150, fast=True)
+    assert np.array_equal(got_big, u)
+    for j, ((s, bc), n) in enumerate(zip(small, steps)):
+        u, _, _ = alone(mpcf, s, bc, n, fast=True)
+        assert np.array_equal(got[j], u), j
+
+
 def test_server_budget_queues_requests(mpcf):
     """A budget that holds one case at a time still completes every request."""
     s = states.random_smooth(20, 20, 20, seed=1)
