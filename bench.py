@@ -60,10 +60,14 @@ KERNEL_BYTES_PER_CELL = {
     # FAST build: the x+y part is folded into R where it is computed; 3 G/P
     # accumulators remain for the z sweep
     "fast": {
-        "k_sweep_xy": 28 + 28 + 28 + 12,          # read u, R; write R, 3 accumulators
+        # read u, R (not in a step's first stage: a = 0); write R, 3 accumulators;
+        # the mean over the three stages
+        "k_sweep_xy": 28 + 28 * 2 / 3 + 28 + 12,
         "k_sweep_x": 28 + 40,                     # read u, write 10 accumulators
         "k_sweep_yz<1>": 28 + 40 + 28 + 28 + 12,  # read u, accumulators, R; write R, 3 accumulators
-        "k_sweep_yz<2>": 28 + 12 + 28 + 28 + 28,  # read u, 3 accumulators, R; write R, u'
+        # read u, 3 accumulators, R; write R (not in the last stage of a step
+        # that another step of the same call follows), u'
+        "k_sweep_yz<2>": 28 + 12 + 28 + 28 * 2 / 3 + 28,
     },
 }
 # measured HBM bytes per cell of each kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE

@@ -669,7 +669,10 @@ __device__ __forceinline__ CellRec reconstruct_lanes(const Prim& c)
     return o;
 }
 
-// wave `wv` of the x+y sweep over rows [bz * chunk, bz * chunk + chunk)
+// wave `wv` of the x+y sweep over rows [bz * chunk, bz * chunk + chunk).
+// FIRST (folded builds, a = 0 known at launch): R is not read -- the stage
+// multiplies it by zero -- which takes 28 B/cell from that stage's traffic
+template <bool FIRST = false>
 __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A, const int chunk,
                                               const int wv, const int bz)
 {
@@ -712,7 +715,7 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
     Real nxt[NU];
     Real nsg = load_row(A, e0, c0 + 1, g.ny, s, 2, nxt);
     // a first stage stores its x+y part unscaled and does not read dt
-    const bool unscaled = unscaled_stage(A.a);
+    const bool unscaled = FIRST || unscaled_stage(A.a);
     const Real dtinvh = unscaled ? Real(0.0f) : stage_dtinvh(A);
     // one march step at row k.  FLUX: the face below row k is solved
     // (k >= c0); EMIT: row k-1 is finished and written (k >= c0 + 1); PF: row
@@ -724,7 +727,7 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
         // folded builds: R of row k-1, emitted below (every lane's address is
         // an interior cell, stored or not)
         Real Ro[NVAR];
-        if (kFold && EMIT) {
+        if (kFold && EMIT && !FIRST) {
 #pragma unroll
             for (int f = 0; f < NVAR; ++f) Ro[f] = A.R[f*fs + e0 + (k - 1) * s];
         }
@@ -747,7 +750,8 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
 #pragma unroll
                     for (int f = 0; f < NVAR; ++f) {
                         const Real t = ((Fxr.f[f] - Fxl.f[f]) + Fy.f[f]) - m.Fl.f[f];
-                        if (kFold) A.R[f*fs + e] = unscaled ? A.a*Ro[f] - t : A.a*Ro[f] - dtinvh*t;
+                        // first stage: -t, which is 0 R - t up to the sign of a zero
+                        if (kFold) A.R[f*fs + e] = unscaled ? -t : A.a*Ro[f] - dtinvh*t;
                         else       acc[f*fs + e] = t;
                     }
                     acc[A_SUMG*fs + e] = (rx.p.G + rx.m.G) + cGy;
@@ -788,12 +792,13 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
 #else
 #define MPCF_XY_WAVES
 #endif
+template <bool FIRST>
 __global__ __launch_bounds__(256) MPCF_XY_WAVES
 void k_sweep_xy(const Geom g, const StageArgs A, const int chunk)
 {
     const int gx = gridDim.x;
     const int t = xcd_order(blockIdx.x + blockIdx.y * gx, gx * gridDim.y);
-    sweep_xy_body(g, A, chunk, (t % gx) * 4 + threadIdx.y, t / gx);
+    sweep_xy_body<FIRST>(g, A, chunk, (t % gx) * 4 + threadIdx.y, t / gx);
 }
 
 // ---------------------------------------------------------------------------
@@ -1207,6 +1212,7 @@ void kb_sos(const BatchDev B, const double cfl, const double dt_max)
     sos_body<CPU_FORM>(C.g, (const Real*)C.u[B.cur[c]], C.g.fstride, C.st, local, C.nb_sos, wmax, wbad, d);
 }
 
+template <bool FIRST>
 __global__ __launch_bounds__(256) MPCF_XY_WAVES
 void kb_sweep_xy(const BatchDev B, const Real a, const Real b)
 {
@@ -1217,7 +1223,7 @@ void kb_sweep_xy(const BatchDev B, const Real a, const Real b)
     const CaseDesc& C = B.cases[c];
     const int local = blockIdx.x - B.off_xy[c];
     const StageArgs A = case_args(C, B.cur[c], a, b);
-    sweep_xy_body(C.g, A, C.chunk_xy, (local % C.nbx_xy) * 4 + threadIdx.y, local / C.nbx_xy);
+    sweep_xy_body<FIRST>(C.g, A, C.chunk_xy, (local % C.nbx_xy) * 4 + threadIdx.y, local / C.nbx_xy);
 }
 
 // the z sweep finishes the stage; the last block of each case to finish
@@ -1445,7 +1451,9 @@ void launch_sweep(const Geom& g, const int dir, const SweepParams& p, const int 
         const int chunk = pick_chunk(g.ny, (int64_t)g.nx * nzr, alone);
         const int waves = ((g.nx + 6) * nzr + XY_TILE - 1) / XY_TILE;
         const dim3 grid((waves + 3) / 4, (g.ny + chunk - 1) / chunk);
-        hipLaunchKernelGGL(k_sweep_xy, grid, dim3(64, 4), 0, s, g, A, chunk);
+        // a first stage of a folded build does not read R
+        if (kFold && A.a == Real(0.0f)) hipLaunchKernelGGL(k_sweep_xy<true>, grid, dim3(64, 4), 0, s, g, A, chunk);
+        else                            hipLaunchKernelGGL(k_sweep_xy<false>, grid, dim3(64, 4), 0, s, g, A, chunk);
     } else if (dir == 0) {
         const int npen = g.ny * nzr;
         const int chunk = pick_chunk(g.nx, npen, 1);
@@ -1553,7 +1561,8 @@ void batch_sos(const BatchDev& B, const int blocks, const int cpu_form, const do
 
 void batch_xy(const BatchDev& B, const int blocks, const double a, const double b, hipStream_t s)
 {
-    hipLaunchKernelGGL(kb_sweep_xy, dim3(blocks), dim3(64, 4), 0, s, B, (Real)a, (Real)b);
+    if (kFold && a == 0.0) hipLaunchKernelGGL(kb_sweep_xy<true>, dim3(blocks), dim3(64, 4), 0, s, B, (Real)a, (Real)b);
+    else                   hipLaunchKernelGGL(kb_sweep_xy<false>, dim3(blocks), dim3(64, 4), 0, s, B, (Real)a, (Real)b);
 }
 
 // storeR == 0: R is not stored (MODE_STAGE_NOR)
