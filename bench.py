@@ -423,6 +423,11 @@ def main():
             "avg_launch_ms": dom_ms / dom_cnt,
             "hbm_stage_GBs": hbm_stage,
             "hbm_stage_frac": hbm_stage / PEAK_HBM_GBS,
+            # the published FP32 peak counts packed (two-wide) FMAs; every build
+            # here is compiled without packed FP32, so one FMA per lane per
+            # cycle, half of it, is the issue ceiling these kernels can reach
+            "peak_unpacked_fp32": PEAK_FP32_TFLOPS / 2 if unit == "TFLOP/s" else None,
+            "frac_unpacked_fp32": achieved / (PEAK_FP32_TFLOPS / 2) if unit == "TFLOP/s" else None,
         },
         "kernels": kernels,
         "flop_per_cell_stage": fl["cell_stage"],

@@ -184,21 +184,23 @@ def test_server_results_equal_alone(mpcf, policy, fill, graph):
         assert np.array_equal(got[j], u), j
 
 
-def test_server_lone_request_runs_whole_block_kernels(mpcf):
-    """A lone request of 2^18 cells or more steps on the whole-block kernels,
-    enqueued eagerly (no graph is captured); a smaller lone one replays the
-    batch graph.  Both return exactly the state they reach alone."""
+@pytest.mark.parametrize("fast", [True, False], ids=["fast", "bitexact"])
+def test_server_lone_request_runs_whole_block_kernels(mpcf, fast):
+    """A lone request of 2^18 cells or more steps on the whole-block kernels
+    (the bit-exact build: the split x and y sweeps), enqueued eagerly (no graph
+    is captured); a smaller lone one replays the batch graph.  Both return
+    exactly the state they reach alone."""
     big = mpcf.ic_bubble_collapse(72, 64, 64, 0.2, 100.0)        # 294912 cells
     small = states.random_smooth(20, 20, 20, seed=1)
     for s, n, solo in ((big, 7, True), (small, 4, False)):
-        say("server lone request", s.shape[1:], "solo" if solo else "batch graph")
-        with mpcf.Server(1 << 30, policy="fifo_slack", fill_cells=1 << 20, max_live=4, fast=True, graph=True) as S:
+        say("server lone request", s.shape[1:], "solo" if solo else "batch graph", "fast" if fast else "bit-exact")
+        with mpcf.Server(1 << 30, policy="fifo_slack", fill_cells=1 << 20, max_live=4, fast=fast, graph=True) as S:
             i = S.submit(s, n, h=1.0 / max(s.shape[1:]))
             S.wait(60.0)
             st = S.stats()
             got = S.results[i]
         assert st["steps"] == n and st["captures"] == (0 if solo else 1), st
-        u, _, _ = alone(mpcf, s, [0] * 6, n, fast=True)
+        u, _, _ = alone(mpcf, s, [0] * 6, n, fast=fast)
         assert np.array_equal(got, u)
 
 
