@@ -140,22 +140,26 @@ template <typename T>
 MPCF_HD RecT<T> weno_fast_diff(const T b, const T c, const T d, const T dab, const T dbc, const T dcd, const T dde)
 {
     RecT<T> out;
-    const T eps = T(1e-6);
     const T Db = dbc - dab, Dc = dcd - dbc, Dd = dde - dcd;   // second differences
     const T tA = Db + 2.0f*dbc;                               // a - 4b + 3c
     const T tB = dbc + dcd;                                   // d - b
     const T tC = Dd - 2.0f*dcd;                               // 3c - 4d + e
-    const T k13_12 = T(13./12.);
-    const T iA = Db*(k13_12*Db) + ((0.25f*tA)*tA + eps);
-    const T iB = Dc*(k13_12*Dc) + ((0.25f*tB)*tB + eps);
-    const T iC = Dd*(k13_12*Dd) + ((0.25f*tC)*tC + eps);
+    // the indicators times 4, 13/3 D^2 + t^2 + 4 eps: each is exactly 4 times
+    // 13/12 D^2 + 1/4 t^2 + eps (a power-of-two scale of every operand), with
+    // one product fewer; the normalised weights depend only on their ratios
+    const T k13_3 = 4.0f*T(13./12.);
+    const T eps4 = 4.0f*T(1e-6);
+    const T iA = Db*(k13_3*Db) + (tA*tA + eps4);
+    const T iB = Dc*(k13_3*Dc) + (tB*tB + eps4);
+    const T iC = Dd*(k13_3*Dd) + (tC*tC + eps4);
     const T rA = mrcp(iA*iA), rB = mrcp(iB*iB), rC = mrcp(iC*iC);
     const T aB = 6.0f*rB, mC = 3.0f*rC, pA = 3.0f*rA;
-    // minus-state candidates (window a..e), 6 (q_k - c)
-    const T nm0 = 5.0f*dbc - 2.0f*dab, nm1 = dbc + 2.0f*dcd, nm2 = 4.0f*dcd - dde;
+    // minus-state candidates (window a..e), 6 (q_k - c); the first,
+    // 5 (c-b) - 2 (b-a), as 2 tA - (c-b)
+    const T nm0 = 2.0f*tA - dbc, nm1 = dbc + 2.0f*dcd, nm2 = 4.0f*dcd - dde;
     // plus-state candidates (the mirror), 6 (q_k - c); the middle one as
-    // -(its value)
-    const T np0 = 2.0f*dde - 5.0f*dcd, np1n = dcd + 2.0f*dbc, np2 = dab - 4.0f*dbc;
+    // -(its value); the first, 2 (e-d) - 5 (d-c), as 2 tC + (d-c)
+    const T np0 = 2.0f*tC + dcd, np1n = dcd + 2.0f*dbc, np2 = dab - 4.0f*dbc;
     const T k1_6 = T(1./6.);
     const T om = (rA*nm0 + aB*nm1 + mC*nm2) * mrcp(rA + aB + mC);
     const T op = (rC*np0 - aB*np1n + pA*np2) * mrcp(rC + aB + pA);
@@ -181,10 +185,10 @@ MPCF_HD RecT<T> weno_cell(const T a, const T b, const T c, const T d, const T e)
         //  * the indicators in Jiang-Shu form 13/12 D^2 + 1/4 t^2 (expanding
         //    gives the reference's coefficients 4/3, -19/3, 11/3, 25/3, -31/3,
         //    10/3 and 4/3, -13/3, 5/3, 13/3, -13/3, 4/3 term by term), eps
-        //    folded in;
+        //    folded in, and evaluated times 4 (13/3 D^2 + t^2 + 4 eps);
         //  * the weights only normalised, omega_k = alpha_k / sum(alpha): the
-        //    common 1/10 of alpha_k = d_k / (10 is_k^2) dropped, one reciprocal
-        //    of the sum per state;
+        //    common factors of alpha_k = d_k / (10 is_k^2) dropped, one
+        //    reciprocal of the sum per state;
         //  * the candidates as c + (differences)/6, e.g. (2a - 7b + 11c)/6 =
         //    c + (-2 (b-a) + 5 (c-b))/6, so a state is
         //    c + sum(alpha_k n_k) / (6 sum(alpha));
