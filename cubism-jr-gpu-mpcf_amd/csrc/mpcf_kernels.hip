@@ -19,7 +19,6 @@
 // summation order (((((fxp-fxm)+fyp)-fym)+fzp)-fzm) (GPUkernels.cu:1111).
 #include <hip/hip_runtime.h>
 #include <type_traits>
-#include <type_traits>
 #include "mpcf_scheme.h"
 #include "mpcf_launch.h"
 
