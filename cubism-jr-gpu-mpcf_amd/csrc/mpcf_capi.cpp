@@ -424,20 +424,30 @@ int transfer(mpcf_ctx* c, const int which, void* const* fields, const int host_f
     char* blk = (char*)(which == MPCF_DATA ? c->u[c->cur] : c->R);
     // E of an E_PAIR solution: the pair (hi in field 4, lo in field 7)
     const bool pair = which == MPCF_DATA && (c->flags & MPCF_FLAG_E_PAIR);
-    for (int q = 0; q < 7; ++q) {
-        void* dq = blk + q * fb;
-        if (to_device) {
-            HIP_TRY(hipMemcpyAsync(c->acc, fields[q], n * hb, hipMemcpyHostToDevice, c->stream));
-            if (pair && q == 4) c->L->pack_epair(c->g, dq, blk + 7 * fb, c->acc, 1, host_f64, c->stream);
-            else                c->L->pack(c->g, dq, c->acc, 1, 1, host_f64, c->stream);
-            LAUNCH_CHECK("pack");
-        } else {
-            if (pair && q == 4) c->L->pack_epair(c->g, dq, blk + 7 * fb, c->acc, 0, host_f64, c->stream);
-            else                c->L->pack(c->g, dq, c->acc, 0, 1, host_f64, c->stream);
-            LAUNCH_CHECK("unpack");
-            HIP_TRY(hipMemcpyAsync(fields[q], c->acc, n * hb, hipMemcpyDeviceToHost, c->stream));
+    // the accumulator area stages dense fields.  Uploads stage all seven at
+    // once (binary32 host data), or in groups of at least five (binary64 host
+    // data into a binary32 block), with one wait per group; downloads go one
+    // field at a time (measured faster for the copy into pageable memory)
+    const size_t per = n * hb;
+    const int group = to_device ? (int)std::min<size_t>(7, mpcf::NACC * fb / per) : 1;
+    for (int q0 = 0; q0 < 7; q0 += group) {
+        const int q1 = std::min(7, q0 + group);
+        for (int q = q0; q < q1; ++q) {
+            void* dq = blk + q * fb;
+            char* st = (char*)c->acc + (size_t)(q - q0) * per;
+            if (to_device) {
+                HIP_TRY(hipMemcpyAsync(st, fields[q], per, hipMemcpyHostToDevice, c->stream));
+                if (pair && q == 4) c->L->pack_epair(c->g, dq, blk + 7 * fb, st, 1, host_f64, c->stream);
+                else                c->L->pack(c->g, dq, st, 1, 1, host_f64, c->stream);
+                LAUNCH_CHECK("pack");
+            } else {
+                if (pair && q == 4) c->L->pack_epair(c->g, dq, blk + 7 * fb, st, 0, host_f64, c->stream);
+                else                c->L->pack(c->g, dq, st, 0, 1, host_f64, c->stream);
+                LAUNCH_CHECK("unpack");
+                HIP_TRY(hipMemcpyAsync(fields[q], st, per, hipMemcpyDeviceToHost, c->stream));
+            }
         }
-        HIP_TRY(hipStreamSynchronize(c->stream));   // the staging area is reused by the next field
+        HIP_TRY(hipStreamSynchronize(c->stream));   // the staging area is reused by the next group
     }
     return MPCF_OK;
 }
