@@ -92,8 +92,11 @@ struct Timed
 // pair wins when it is VALU-bound (bit-exact build), unless forced
 bool split_xy(const mpcf_ctx* c)
 {
+    // default: the fused x+y sweep, except in the binary64 build (fused and
+    // split give the same results; the fused one reads u once and writes the
+    // accumulators once, the split x sweep's LDS-staged runs over-fetch)
     return (c->flags & MPCF_FLAG_SPLIT_XY) ? true : (c->flags & MPCF_FLAG_FUSED_XY) ? false
-                                                   : !(c->flags & MPCF_FLAG_FAST);
+                                                   : (c->flags & MPCF_FLAG_FP64) != 0;
 }
 
 // R <- a R - dtinvh L(u) (stage == 0), or that and u_out <- u + b R (stage == 1;
@@ -121,8 +124,8 @@ void run_rhs(mpcf_ctx* c, const void* u, void* uo, double a, double b, double dt
 int pipeline_slabs(const mpcf_ctx* c)
 {
     if (c->pipe_req >= 0) return c->pipe_req;
-    // the split sweeps (bit-exact build) are all VALU-bound: slabs give them
-    // nothing to overlap (measured on MI355X at 512^3: whole-block 59.4 ms a
+    // the split sweeps are all VALU-bound: slabs give them nothing to overlap
+    // (measured on MI355X at 512^3, bit-exact build: whole-block 59.4 ms a
     // step, 4 slabs 60.1, 8 slabs 61.4)
     if (split_xy(c)) return 0;
     // a slab needs about 64 z-planes of a 256^2-or-larger section to fill the

@@ -257,7 +257,7 @@ class Block:
             flags |= FLAG_SPLIT_XY
         elif split_xy is False:
             flags |= FLAG_FUSED_XY
-        self.fused_xy = (split_xy is False) or (split_xy is None and fast)
+        self.fused_xy = (split_xy is False) or (split_xy is None and not fp64)
         h_ = C.c_void_p()
         _check(lib().mpcf_create(C.byref(h_), nx, ny, nz, float(h), flags))
         self._h = h_

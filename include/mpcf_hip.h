@@ -54,8 +54,8 @@ enum {
     MPCF_FLAG_SOS_CPU = 2u,  /* speed of sound with p recovered in binary64 as MaxSpeedOfSound_CPP
                                 (source/MaxSpeedOfSound.cpp:43) instead of the GPU form (GPUkernels.cu:1206) */
     MPCF_FLAG_SPLIT_XY = 4u, /* run x and y as two sweeps ...                                          */
-    MPCF_FLAG_FUSED_XY = 8u, /* ... or as one fused x+y sweep (same results either way).  Default: fused
-                                with MPCF_FLAG_FAST, split otherwise (the faster one for each build).  */
+    MPCF_FLAG_FUSED_XY = 8u, /* ... or as one fused x+y sweep (same results either way).  Default: fused,
+                                except split with MPCF_FLAG_FP64.                                      */
     MPCF_FLAG_P64 = 16u,     /* selective binary64 on binary32 storage: the FAST arithmetic, with the
                                 pressure recovery p = (E - KE - P)/G and the speed-of-sound / dt
                                 reduction in binary64 (implies MPCF_FLAG_FAST)                      */

@@ -368,11 +368,13 @@ def sweep_launches(mpcf, b, steps=2):
 
 def test_pipeline_slab_choice(mpcf):
     """The automatic choice: min(8, NZ/64) slabs for blocks of 2^24 cells or
-    more with the fused x+y sweep; whole-block sweeps below that, for the
-    split sweeps and under MPCF_FLAG_NO_PIPELINE; forced counts as asked,
-    split sweeps included."""
+    more with the fused x+y sweep (the default of the FAST and the bit-exact
+    binary32 builds); whole-block sweeps below that, for the split sweeps and
+    under MPCF_FLAG_NO_PIPELINE; forced counts as asked, split sweeps
+    included."""
     cases = [((128, 128, 128), dict(fast=True), 1), ((256, 256, 256), dict(fast=True), 4),
-             ((512, 512, 64), dict(fast=True), 1), ((256, 256, 256), dict(), 1),
+             ((512, 512, 64), dict(fast=True), 1), ((256, 256, 256), dict(split_xy=True), 1),
+             ((256, 256, 256), dict(), 4),
              ((64, 64, 96), dict(pipeline=3), 3),
              ((256, 256, 256), dict(fast=True, pipeline=False), 1), ((64, 64, 96), dict(fast=True, pipeline=6), 6)]
     for (nx, ny, nz), kw, want in cases:
