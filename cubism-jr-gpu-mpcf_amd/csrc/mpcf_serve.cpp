@@ -596,7 +596,6 @@ bool enqueue_solo_step(mpcf_server* S, Req* r)
     const int cpu_form = (S->flags & MPCF_FLAG_SOS_CPU) ? 1 : 0;
     if (r->stepped) S->L->sos_folded(c->st, S->cfl, c->h, 1e30, S->stream);
     else S->L->sos(c->g, c->u[c->cur], c->st, cpu_form, 1, S->cfl, c->h, 1e30, S->stream);
-    const bool split = !(S->flags & MPCF_FLAG_FAST);   // as a context's default
     int cur = c->cur;
     for (int s = 0; s < 3; ++s) {
         mpcf::SweepParams p;
@@ -605,12 +604,7 @@ bool enqueue_solo_step(mpcf_server* S, Req* r)
         p.sos_next = s == 2 ? c->st : nullptr;
         p.sos_cpu = cpu_form;
         p.bcmask = mpcf::detail::bc_mask(c->bc);
-        if (split) {
-            S->L->sweep(c->g, 0, p, 1, S->stream);
-            S->L->sweep(c->g, 1, p, 1, S->stream);
-        } else {
-            S->L->sweep(c->g, 3, p, 1, S->stream);
-        }
+        S->L->sweep(c->g, 3, p, 1, S->stream);                // the fused x+y sweep
         S->L->sweep(c->g, 2, p, s == 2 ? 2 : 1, S->stream);
         cur = 1 - cur;
     }

@@ -186,10 +186,10 @@ def test_server_results_equal_alone(mpcf, policy, fill, graph):
 
 @pytest.mark.parametrize("fast", [True, False], ids=["fast", "bitexact"])
 def test_server_lone_request_runs_whole_block_kernels(mpcf, fast):
-    """A lone request of 2^18 cells or more steps on the whole-block kernels
-    (the bit-exact build: the split x and y sweeps), enqueued eagerly (no graph
-    is captured); a smaller lone one replays the batch graph.  Both return
-    exactly the state they reach alone."""
+    """A lone request of 2^18 cells or more steps on the whole-block kernels,
+    enqueued eagerly (no graph is captured); a smaller lone one replays the
+    batch graph.  Both return exactly the state they reach alone, in the FAST
+    and the bit-exact build."""
     big = mpcf.ic_bubble_collapse(72, 64, 64, 0.2, 100.0)        # 294912 cells
     small = states.random_smooth(20, 20, 20, seed=1)
     for s, n, solo in ((big, 7, True), (small, 4, False)):
