@@ -438,10 +438,11 @@ PinnedChunks& pinned_chunks()
 
 // A host copy split over a pool of worker threads (the copy and the first
 // touch of the destination's pages are the cost).  The pool is created on
-// first use and lives as long as the process; one copy runs at a time (the
-// callers hold the staging mutex).
+// first use and lives as long as the process; one copy runs at a time
+// (`busy` serialises the callers).
 struct CopyPool
 {
+    std::mutex busy;
     std::mutex mu;
     std::condition_variable go, done;
     std::vector<std::thread> workers;
@@ -479,6 +480,7 @@ struct CopyPool
             std::memcpy(d, sp, bytes);
             return;
         }
+        std::lock_guard<std::mutex> one(busy);
         {
             std::lock_guard<std::mutex> lk(mu);
             dst = d; src = sp; n = bytes; parts = p; per = (bytes + p - 1) / p;
@@ -506,6 +508,15 @@ CopyPool& copy_pool()
     }();
     return *pool;
 }
+
+}  // namespace
+
+void mpcf::detail::parallel_copy(void* dst, const void* src, const size_t n)
+{
+    copy_pool().copy((char*)dst, (const char*)src, n);
+}
+
+namespace {
 
 // n bytes at device address src into host memory dst, ordered on stream s and
 // complete on return

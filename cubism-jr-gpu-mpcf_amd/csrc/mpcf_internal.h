@@ -18,6 +18,10 @@ namespace detail {
 
 // records msg as this thread's last error (mpcf_last_error) and returns code
 int fail(int code, const std::string& msg);
+
+// host memcpy of n bytes split over a pool of worker threads (created on first
+// use, kept until the process ends); concurrent callers take turns
+void parallel_copy(void* dst, const void* src, size_t n);
 const char* last_error();
 
 }  // namespace detail

@@ -290,10 +290,12 @@ void copier_main(mpcf_server* S)
             j = S->copy_jobs.front();
             S->copy_jobs.pop_front();
         }
+        // split over the library's pool of copy threads (the copy and the
+        // first touch of the caller's pages are the cost of a large request)
         const size_t n = (size_t)j.r->cells();
         for (int q = 0; q < 7; ++q) {
-            if (j.to_pinned) std::memcpy(j.pinned + q * n, j.r->in[q], n * sizeof(float));
-            else             std::memcpy(j.r->out[q], j.pinned + q * n, n * sizeof(float));
+            if (j.to_pinned) mpcf::detail::parallel_copy(j.pinned + q * n, j.r->in[q], n * sizeof(float));
+            else             mpcf::detail::parallel_copy(j.r->out[q], j.pinned + q * n, n * sizeof(float));
         }
         j.r->copied.store(true, std::memory_order_release);
         std::lock_guard<std::mutex> lk(S->mu);
