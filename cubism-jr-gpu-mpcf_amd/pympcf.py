@@ -856,8 +856,9 @@ class Server:
                 "fifo_slack": SERVE_FIFO_SLACK}
 
     def __init__(self, hbm_bytes, policy="fifo_fill", fill_cells=0, max_live=256, cfl=0.3, fast=True,
-                 p64=False, graph=True):
+                 p64=False, graph=True, sos_cpu=False):
         flags = (FLAG_FAST if fast or p64 else 0) | (FLAG_P64 if p64 else 0) | (0 if graph else FLAG_SERVE_EAGER)
+        flags |= FLAG_SOS_CPU if sos_cpu else 0
         h_ = C.c_void_p()
         _check(lib().mpcf_server_create(C.byref(h_), flags, int(hbm_bytes), self.POLICIES[policy],
                                         int(fill_cells), int(max_live), float(cfl)))

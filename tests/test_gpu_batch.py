@@ -184,23 +184,34 @@ def test_server_results_equal_alone(mpcf, policy, fill, graph):
         assert np.array_equal(got[j], u), j
 
 
-@pytest.mark.parametrize("fast", [True, False], ids=["fast", "bitexact"])
-def test_server_lone_request_runs_whole_block_kernels(mpcf, fast):
+LONE_BUILDS = {"fast": dict(fast=True), "bitexact": dict(fast=False), "fast-soscpu": dict(fast=True, sos_cpu=True),
+               "p64": dict(fast=True, p64=True)}
+
+
+@pytest.mark.parametrize("build", list(LONE_BUILDS))
+def test_server_lone_request_runs_whole_block_kernels(mpcf, build):
     """A lone request of 2^18 cells or more steps on the whole-block kernels,
     enqueued eagerly (no graph is captured); a smaller lone one replays the
-    batch graph.  Both return exactly the state they reach alone, in the FAST
-    and the bit-exact build."""
+    batch graph.  Both return exactly the state they reach alone, in the FAST,
+    bit-exact and selective-binary64 builds and with the CPU form of the speed
+    of sound."""
+    kw = LONE_BUILDS[build]
     big = mpcf.ic_bubble_collapse(72, 64, 64, 0.2, 100.0)        # 294912 cells
     small = states.random_smooth(20, 20, 20, seed=1)
     for s, n, solo in ((big, 7, True), (small, 4, False)):
-        say("server lone request", s.shape[1:], "solo" if solo else "batch graph", "fast" if fast else "bit-exact")
-        with mpcf.Server(1 << 30, policy="fifo_slack", fill_cells=1 << 20, max_live=4, fast=fast, graph=True) as S:
+        say("server lone request", s.shape[1:], "solo" if solo else "batch graph", build)
+        with mpcf.Server(1 << 30, policy="fifo_slack", fill_cells=1 << 20, max_live=4, graph=True, **kw) as S:
             i = S.submit(s, n, h=1.0 / max(s.shape[1:]))
             S.wait(60.0)
             st = S.stats()
             got = S.results[i]
         assert st["steps"] == n and st["captures"] == (0 if solo else 1), st
-        u, _, _ = alone(mpcf, s, [0] * 6, n, fast=fast)
+        nz, ny, nx = s.shape[1:]
+        with mpcf.Block(nx, ny, nz, 1.0 / max(nx, ny, nz), split_xy=False, **kw) as b:
+            b.data = s
+            for _ in range(n):
+                b.step(0.3)
+            u = b.data
         assert np.array_equal(got, u)
 
 
