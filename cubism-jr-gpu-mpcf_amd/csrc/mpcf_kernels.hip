@@ -179,27 +179,76 @@ __device__ __forceinline__ int bc_index(const int i, const int n, const int bcma
     return i;
 }
 
-// raw conserved values of the cell at index `row` of a marched column (e0 =
+// A lane's element index split into a part that is the same for every lane of
+// the wave (base, a scalar) and the lane's byte offset from it (lo, small and
+// non-negative).  The sweeps access their fields through buffer resources
+// built on wave-uniform element pointers (the field, row and wave parts of
+// the index, added by the scalar unit), with the lane's 32-bit offset: as flat
+// global accesses the same loads and stores took one 64-bit vector add each,
+// a few percent of the marches' vector instructions.
+struct LaneIdx { int64_t base; uint32_t lo; };
+
+// e: the lane's element; floor_e: an element no greater than the element of
+// any active lane of the wave (the first active lane's value is taken)
+__device__ __forceinline__ LaneIdx lane_idx(const int e, const int floor_e)
+{
+    const int b = __builtin_amdgcn_readfirstlane(floor_e);
+    return LaneIdx{b, (uint32_t)(e - b) * (uint32_t)sizeof(Real)};
+}
+
+// raw buffer resource at a wave-uniform address: no stride, no record limit,
+// 32-bit data format (a zero format would read as zero)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc_at(const void* p)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, -1, 0x00020000);
+}
+
+// element p[u] + lane offset lo (u wave-uniform).  BUF: through a buffer
+// resource; else a flat global access (the y/z sweeps: their marches keep
+// more scalar state live, and a resource per access runs them out of scalar
+// registers)
+template <bool BUF, typename V>
+__device__ __forceinline__ V ld_lane(const V* p, const int64_t u, const uint32_t lo)
+{
+    if constexpr (!BUF) return p[u + lo / sizeof(V)];
+    else if constexpr (sizeof(V) == 4) return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b32(rsrc_at(p + u), lo, 0, 0));
+    else return __builtin_bit_cast(V, __builtin_amdgcn_raw_buffer_load_b64(rsrc_at(p + u), lo, 0, 0));
+}
+
+// store; STREAM: non-temporal, for data that no kernel reads again soon --
+// the solution and register written by the z sweep (read by the next stage, a
+// whole block later) and the G/P accumulators of the x+y sweep (read once by
+// the z sweep of the slab).  The hint keeps these streams from displacing
+// lines in the caches; at 512^3 it takes the pipelined step from 20.66 to
+// 20.0 ms.
+template <bool BUF, bool STREAM = false, typename V>
+__device__ __forceinline__ void st_lane(V* p, const int64_t u, const uint32_t lo, const V v)
+{
+    constexpr int aux = STREAM ? 2 : 0;   // bit 1: nt
+    if constexpr (!BUF) {
+        if (STREAM) __builtin_nontemporal_store(v, &p[u + lo / sizeof(V)]);
+        else        p[u + lo / sizeof(V)] = v;
+    } else if constexpr (sizeof(V) == 4)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, v), rsrc_at(p + u), lo, 0, aux);
+    else
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(__attribute__((ext_vector_type(2))) unsigned, v),
+                                              rsrc_at(p + u), lo, 0, aux);
+}
+
+// raw conserved values of the cell at index `row` of a marched column (L =
 // element of row 0, s = element stride of the axis), boundary-mapped.  Returns
 // the sign the axis' normal momentum takes; the caller applies it where the
 // values are consumed, so that a prefetch is not waited on when it is issued.
-__device__ __forceinline__ Real load_row(const StageArgs& A, const int e0, const int row, const int n, const int s,
+template <bool BUF>
+__device__ __forceinline__ Real load_row(const StageArgs& A, const LaneIdx& L, const int row, const int n, const int s,
                                          const int face_lo, Real* out)
 {
     Real sg;
     const int r = bc_index(row, n, A.bcmask, face_lo, sg);
 #pragma unroll
-    for (int q = 0; q < NU; ++q) out[q] = A.u[q*A.fs + e0 + r * s];
+    for (int q = 0; q < NU; ++q) out[q] = ld_lane<BUF>(A.u, L.base + q*A.fs + (int64_t)r * s, L.lo);
     return sg;
 }
-
-// a store that no kernel reads again soon: the solution and register written
-// by the z sweep (read by the next stage, a whole block later) and the G/P
-// accumulators of the x+y sweep (read once by the z sweep of the slab).  The
-// non-temporal hint keeps these streams from displacing lines in the caches;
-// at 512^3 it takes the pipelined step from 20.66 to 20.0 ms.
-template <typename V>
-__device__ __forceinline__ void store_stream(V* p, const V v) { __builtin_nontemporal_store(v, p); }
 
 // primitives of a loaded row of NU conserved values
 __device__ __forceinline__ Prim prim_of(const Real* w)
@@ -217,52 +266,55 @@ struct EmitIn
 };
 
 template <int MODE>
-__device__ __forceinline__ void emit_load(const StageArgs& A, const int e, EmitIn<MODE>& in)
+__device__ __forceinline__ void emit_load(const StageArgs& A, const LaneIdx& L, const int64_t row, EmitIn<MODE>& in)
 {
     const int64_t fs = A.fs;
+    const int64_t e = L.base + row;   // the wave-uniform part of the cell's element
     if (MODE == MODE_FIRST) return;
     if (!kFold || MODE == MODE_ACC) {
 #pragma unroll
-        for (int q = 0; q < NACC; ++q) in.acc[q] = A.acc[q*fs + e];
+        for (int q = 0; q < NACC; ++q) in.acc[q] = ld_lane<false>(A.acc, q*fs + e, L.lo);
     } else {
         // folded: only the G/P accumulators remain
-        in.acc[A_SUMG] = A.acc[A_SUMG*fs + e];
-        in.acc[A_SUMP] = A.acc[A_SUMP*fs + e];
-        in.acc[A_DIVU] = A.acc[A_DIVU*fs + e];
+        in.acc[A_SUMG] = ld_lane<false>(A.acc, A_SUMG*fs + e, L.lo);
+        in.acc[A_SUMP] = ld_lane<false>(A.acc, A_SUMP*fs + e, L.lo);
+        in.acc[A_DIVU] = ld_lane<false>(A.acc, A_DIVU*fs + e, L.lo);
     }
     if (MODE == MODE_CONV || is_stage(MODE) || (kFold && MODE == MODE_ACC)) {
 #pragma unroll
-        for (int q = 0; q < NVAR; ++q) in.R[q] = A.R[q*fs + e];
+        for (int q = 0; q < NVAR; ++q) in.R[q] = ld_lane<false>(A.R, q*fs + e, L.lo);
     }
 }
 
 // write the contribution of one cell.  F = flux at the cell's right face,
 // Fl = flux at its left face; cG/cP = plus-state + minus-state of G/P.
 template <int MODE>
-__device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, const int e, const EmitIn<MODE>& in,
+__device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, const LaneIdx& L, const int64_t row, const EmitIn<MODE>& in,
                                      const FaceFlux& F, const FaceFlux& Fl, const Real cG, const Real cP,
                                      SosT& smax, bool& bad)
 {
     Real* __restrict__ acc = A.acc;
     const int64_t fs = A.fs;
+    const int64_t e = L.base + row;   // the wave-uniform part of the cell's element
+    const uint32_t lo = L.lo;
     if (MODE == MODE_FIRST) {
         // _xflux + _xextraterm_hllc ("=" assignment, GPUkernels.cu:57-59)
 #pragma unroll
-        for (int q = 0; q < NVAR; ++q) acc[q*fs + e] = F.f[q] - Fl.f[q];
-        acc[A_SUMG*fs + e] = cG;
-        acc[A_SUMP*fs + e] = cP;
-        acc[A_DIVU*fs + e] = F.vel - Fl.vel;
+        for (int q = 0; q < NVAR; ++q) st_lane<false>(acc, q*fs + e, lo, F.f[q] - Fl.f[q]);
+        st_lane<false>(acc, A_SUMG*fs + e, lo, cG);
+        st_lane<false>(acc, A_SUMP*fs + e, lo, cP);
+        st_lane<false>(acc, A_DIVU*fs + e, lo, F.vel - Fl.vel);
     } else if (MODE == MODE_ACC) {
         // _yextraterm_hllc ("+=", GPUkernels.cu:439-441)
 #pragma unroll
         for (int q = 0; q < NVAR; ++q) {
             const Real t = (in.acc[q] + F.f[q]) - Fl.f[q];
-            if (kFold) A.R[q*fs + e] = unscaled_stage(A.a) ? A.a*in.R[q] - t : A.a*in.R[q] - dtinvh*t;
-            else       acc[q*fs + e] = t;
+            if (kFold) st_lane<false>(A.R, q*fs + e, lo, unscaled_stage(A.a) ? A.a*in.R[q] - t : A.a*in.R[q] - dtinvh*t);
+            else       st_lane<false>(acc, q*fs + e, lo, t);
         }
-        acc[A_SUMG*fs + e] = in.acc[A_SUMG] + cG;
-        acc[A_SUMP*fs + e] = in.acc[A_SUMP] + cP;
-        acc[A_DIVU*fs + e] = in.acc[A_DIVU] + (F.vel - Fl.vel);
+        st_lane<false>(acc, A_SUMG*fs + e, lo, in.acc[A_SUMG] + cG);
+        st_lane<false>(acc, A_SUMP*fs + e, lo, in.acc[A_SUMP] + cP);
+        st_lane<false>(acc, A_DIVU*fs + e, lo, in.acc[A_DIVU] + (F.vel - Fl.vel));
     } else {
         // _zextraterm_hllc + _divergence (+ _update), GPUkernels.cu:812-814,1104-1136,1164-1170
         const Real sG = in.acc[A_SUMG] + cG;
@@ -284,7 +336,7 @@ __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, cons
             // subtraction, so it rounds as the x+y sweep's product would)
             const Real Rx = !kFold ? A.a*in.R[q] : unscaled_stage(A.a) ? __fmul_rn(dtinvh, in.R[q]) : in.R[q];
             const Real Rn = Rx - rhs;
-            if (MODE != MODE_STAGE_NOR) store_stream(&A.R[q*fs + e], Rn);
+            if (MODE != MODE_STAGE_NOR) st_lane<false, true>(A.R, q*fs + e, lo, Rn);
             if (is_stage(MODE)) {
                 if (kEPair && q == FE) {
                     // E + b R in binary64, stored as the rounded sum and its
@@ -292,12 +344,12 @@ __device__ __forceinline__ void emit(const StageArgs& A, const Real dtinvh, cons
                     const double Ed = (double(in.u[FE]) + double(in.u[FELO])) + double(A.b) * double(Rn);
                     const Real hi = Real(Ed);
                     un[q] = hi;
-                    store_stream(&A.uo[q*fs + e], hi);
-                    store_stream(&A.uo[FELO*fs + e], Real(Ed - double(hi)));
+                    st_lane<false, true>(A.uo, q*fs + e, lo, hi);
+                    st_lane<false, true>(A.uo, FELO*fs + e, lo, Real(Ed - double(hi)));
                     Eun = Ed;
                 } else {
                     un[q] = A.b*Rn + in.u[q];
-                    store_stream(&A.uo[q*fs + e], un[q]);
+                    st_lane<false, true>(A.uo, q*fs + e, lo, un[q]);
                 }
             }
         }
@@ -329,12 +381,12 @@ struct March
     // std::bool_constant (the peeled steps of an unrolled march)
     template <class FLUX, class EMIT>
     __device__ __forceinline__ void step(const StageArgs& A, const Real dtinvh, const FLUX flux, const EMIT em,
-                                         const int e_km1, const EmitIn<MODE>& in)
+                                         const LaneIdx& L, const int64_t row_km1, const EmitIn<MODE>& in)
     {
         const CellRec r = reconstruct<Real, kFast>(w0, w1, w2, w3, w4);
         if (flux) {
             const FaceFlux F = riemann<D, Real, kFast>(pm, r.p);
-            if (em) emit<MODE>(A, dtinvh, e_km1, in, F, Fl, pGp + pm.G, pPp + pm.P, smax, bad);
+            if (em) emit<MODE>(A, dtinvh, L, row_km1, in, F, Fl, pGp + pm.G, pPp + pm.P, smax, bad);
             Fl = F;
         }
         pm = r.m; pGp = r.p.G; pPp = r.p.P;
@@ -374,6 +426,8 @@ __device__ __forceinline__ SosT sweep_yz_column(const Geom& g, const StageArgs& 
     const int c1 = min(c0 + chunk, D == 2 ? A.z1 : nd);
     const int s = g.stride(D);
     const int e0 = D == 1 ? g.idx(x, 0, t) : g.idx(x, t, 0);
+    // the columns of a wave are consecutive: the first active lane's element is the least
+    const LaneIdx L = lane_idx(e0, e0);
     // the y sweep of a first stage stores its part unscaled and does not read dt
     const Real dtinvh = (MODE == MODE_ACC && unscaled_stage(A.a)) ? Real(0.0f) : stage_dtinvh(A);
     const int64_t fs = A.fs;
@@ -385,10 +439,10 @@ __device__ __forceinline__ SosT sweep_yz_column(const Geom& g, const StageArgs& 
     // raw conserved values of cells k-1 .. k+2 (kept for the register update;
     // the emitted cell k-1 is always interior, so its raw values are exact)
     Real r0[NU], r1[NU], r2[NU], r3[NU], w[NU];
-    r1[NEGQ] *= load_row(A, e0, c0 - 2, nd, s, FLO, r1);
-    r2[NEGQ] *= load_row(A, e0, c0 - 1, nd, s, FLO, r2);
-    r3[NEGQ] *= load_row(A, e0, c0, nd, s, FLO, r3);
-    w[NEGQ] *= load_row(A, e0, c0 - 3, nd, s, FLO, w);
+    r1[NEGQ] *= load_row<false>(A, L, c0 - 2, nd, s, FLO, r1);
+    r2[NEGQ] *= load_row<false>(A, L, c0 - 1, nd, s, FLO, r2);
+    r3[NEGQ] *= load_row<false>(A, L, c0, nd, s, FLO, r3);
+    w[NEGQ] *= load_row<false>(A, L, c0 - 3, nd, s, FLO, w);
     m.w1 = prim_of(w);
     m.w2 = prim_of(r1);
     m.w3 = prim_of(r2);
@@ -396,7 +450,7 @@ __device__ __forceinline__ SosT sweep_yz_column(const Geom& g, const StageArgs& 
     // software pipeline: raw values of the next incoming cell and the emit
     // operands of the next step are loaded one step ahead
     Real nxt[NU];
-    Real nsg = load_row(A, e0, c0 + 1, nd, s, FLO, nxt);
+    Real nsg = load_row<false>(A, L, c0 + 1, nd, s, FLO, nxt);
     EmitIn<MODE> ein, ein_next;
     // one march step at row k.  FLUX: face k-1/2 is solved (k >= c0); EMIT:
     // cell k-1 is written (k >= c0 + 1); PF: row k+3 is prefetched and the
@@ -407,14 +461,14 @@ __device__ __forceinline__ SosT sweep_yz_column(const Geom& g, const StageArgs& 
 #pragma unroll
         for (int q = 0; q < NU; ++q) { r0[q] = r1[q]; r1[q] = r2[q]; r2[q] = r3[q]; r3[q] = nxt[q]; }
         const Prim q = prim_of(nxt);  // cell k+2
-        if (PF) nsg = load_row(A, e0, k + 3, nd, s, FLO, nxt);
-        if (PF && FLUX) emit_load<MODE>(A, e0 + k * s, ein_next);   // emitted at step k+1
+        if (PF) nsg = load_row<false>(A, L, k + 3, nd, s, FLO, nxt);
+        if (PF && FLUX) emit_load<MODE>(A, L, (int64_t)k * s, ein_next);   // emitted at step k+1
         if (kRaw) {
 #pragma unroll
             for (int f = 0; f < NU; ++f) ein.u[f] = r0[f];                   // cell k-1
         }
         m.push(q);
-        m.step(A, dtinvh, FLUX, EMIT, e0 + (k - 1) * s, ein);
+        m.step(A, dtinvh, FLUX, EMIT, L, (int64_t)(k - 1) * s, ein);
         ein = ein_next;
     };
     if constexpr (kUnrollMarch) {
@@ -704,6 +758,9 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
     const int c1 = min(c0 + chunk, g.ny);
     const int s = g.px;
     const int e0 = g.idx(xm, 0, z);
+    // every lane's element is at least that of the first ghost column of lane
+    // 0's plane (lanes left of x = 0 read it on a halo face)
+    const LaneIdx L = lane_idx(e0, g.idx(-GHOST, 0, z));
     const int64_t fs = A.fs;
     Real* __restrict__ acc = A.acc;
     // boundary signs of a loaded row: y-momentum by the row's mapping,
@@ -715,12 +772,12 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
 
     March<1, MODE_ACC> m;
     Real w[NU];
-    fix(w, load_row(A, e0, c0 - 3, g.ny, s, 2, w)); m.w1 = prim_of(w);
-    fix(w, load_row(A, e0, c0 - 2, g.ny, s, 2, w)); m.w2 = prim_of(w);
-    fix(w, load_row(A, e0, c0 - 1, g.ny, s, 2, w)); m.w3 = prim_of(w);
-    fix(w, load_row(A, e0, c0, g.ny, s, 2, w));     m.w4 = prim_of(w);
+    fix(w, load_row<true>(A, L, c0 - 3, g.ny, s, 2, w)); m.w1 = prim_of(w);
+    fix(w, load_row<true>(A, L, c0 - 2, g.ny, s, 2, w)); m.w2 = prim_of(w);
+    fix(w, load_row<true>(A, L, c0 - 1, g.ny, s, 2, w)); m.w3 = prim_of(w);
+    fix(w, load_row<true>(A, L, c0, g.ny, s, 2, w));     m.w4 = prim_of(w);
     Real nxt[NU];
-    Real nsg = load_row(A, e0, c0 + 1, g.ny, s, 2, nxt);
+    Real nsg = load_row<true>(A, L, c0 + 1, g.ny, s, 2, nxt);
     // a first stage stores its x+y part unscaled and does not read dt
     const bool unscaled = FIRST || unscaled_stage(A.a);
     const Real dtinvh = unscaled ? Real(0.0f) : stage_dtinvh(A);
@@ -730,13 +787,13 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
     auto step = [&](auto FLUX, auto EMIT, auto PF, const int k) {
         fix(nxt, nsg);
         const Prim q = prim_of(nxt);  // row k+2
-        if (PF) nsg = load_row(A, e0, k + 3, g.ny, s, 2, nxt);
+        if (PF) nsg = load_row<true>(A, L, k + 3, g.ny, s, 2, nxt);
         // folded builds: R of row k-1, emitted below (every lane's address is
         // an interior cell, stored or not)
         Real Ro[NVAR];
         if (kFold && EMIT && !FIRST) {
 #pragma unroll
-            for (int f = 0; f < NVAR; ++f) Ro[f] = A.R[f*fs + e0 + (k - 1) * s];
+            for (int f = 0; f < NVAR; ++f) Ro[f] = ld_lane<true>(A.R, L.base + f*fs + (int64_t)(k - 1) * s, L.lo);
         }
         m.push(q);
         const CellRec ry = reconstruct<Real, kFast>(m.w0, m.w1, m.w2, m.w3, m.w4);   // row k, y-direction
@@ -752,18 +809,18 @@ __device__ __forceinline__ void sweep_xy_body(const Geom& g, const StageArgs& A,
                 for (int f = 0; f < NVAR; ++f) Fxr.f[f] = from_right(Fxl.f[f]);
                 Fxr.vel = from_right(Fxl.vel);
                 if (store) {
-                    const int e = e0 + (k - 1) * s;
+                    const int64_t e = L.base + (int64_t)(k - 1) * s;   // wave-uniform part
                     const Real cGy = m.pGp + m.pm.G, cPy = m.pPp + m.pm.P;
 #pragma unroll
                     for (int f = 0; f < NVAR; ++f) {
                         const Real t = ((Fxr.f[f] - Fxl.f[f]) + Fy.f[f]) - m.Fl.f[f];
                         // first stage: -t, which is 0 R - t up to the sign of a zero
-                        if (kFold) A.R[f*fs + e] = unscaled ? -t : A.a*Ro[f] - dtinvh*t;
-                        else       acc[f*fs + e] = t;
+                        if (kFold) st_lane<true>(A.R, f*fs + e, L.lo, unscaled ? -t : A.a*Ro[f] - dtinvh*t);
+                        else       st_lane<true>(acc, f*fs + e, L.lo, t);
                     }
-                    store_stream(&acc[A_SUMG*fs + e], (rx.p.G + rx.m.G) + cGy);
-                    store_stream(&acc[A_SUMP*fs + e], (rx.p.P + rx.m.P) + cPy);
-                    store_stream(&acc[A_DIVU*fs + e], (Fxr.vel - Fxl.vel) + (Fy.vel - m.Fl.vel));
+                    st_lane<true, true>(acc, A_SUMG*fs + e, L.lo, (rx.p.G + rx.m.G) + cGy);
+                    st_lane<true, true>(acc, A_SUMP*fs + e, L.lo, (rx.p.P + rx.m.P) + cPy);
+                    st_lane<true, true>(acc, A_DIVU*fs + e, L.lo, (Fxr.vel - Fxl.vel) + (Fy.vel - m.Fl.vel));
                 }
             }
             m.Fl = Fy;
