@@ -18,6 +18,28 @@ def test_host_weno_equals_oracle(mpcf, oracle):
     assert np.array_equal(p, rp)
 
 
+def test_host_weno_fast_close_to_oracle(mpcf, oracle):
+    """The FAST reconstruction (indicators on first differences, weights
+    applied to third differences about the central candidate, median-of-three
+    clip), compiled for the host with exact division, against the oracle's
+    reference-arithmetic WENO on noise, smooth, jump, near-constant, constant
+    and linear windows: every state within 1e-5 of its window's largest
+    magnitude.  Rounding alone stays near 1e-6 (noise windows; near-constant
+    large windows differ by a few ulp, where the reference's expanded
+    quadratics cancel), while a wrong weight, candidate or clip moves a state
+    by a sizeable fraction of the window."""
+    w = weno_windows(20000, 5)
+    m, p = mpcf.host_weno(w, fast=True)
+    rm, rp = oracle.weno(w)
+    scale = np.abs(w.astype(np.float64)).max(axis=1)
+    for got, ref in ((m, rm), (p, rp)):
+        err = np.abs(got.astype(np.float64) - ref.astype(np.float64)) / scale
+        assert err.max() < 1e-5, err.max()
+    # constant windows reconstruct exactly
+    k = np.all(w == w[:, :1], axis=1)
+    assert k.any() and np.array_equal(m[k], w[k, 2]) and np.array_equal(p[k], w[k, 2])
+
+
 @pytest.mark.parametrize("d", [0, 1, 2])
 def test_host_face_equals_oracle(mpcf, oracle, d):
     st = states.random_stencils(30000, seed=40 + d)
