@@ -131,12 +131,16 @@ int pipeline_slabs(const mpcf_ctx* c)
     // (measured on MI355X at 512^3, bit-exact build: whole-block 59.4 ms a
     // step, 4 slabs 60.1, 8 slabs 61.4)
     if (split_xy(c)) return 0;
-    // a slab needs about 64 z-planes of a 256^2-or-larger section to fill the
-    // chip on its own stream (measured on MI355X, fast build: 128^3 runs best
-    // whole, 256^3 gains 3-8% with 3-4 slabs, 384^3 20% and 512^3 14% with 8);
-    // more than 8 slabs gain nothing more at 512^3
+    // blocks of 2^24 cells or more, slabs by depth as measured on MI355X (fast
+    // build, cubes, ms a step): 256^3 4 slabs (2.66; 2.74 with 2, 2.81
+    // whole); 320^3 3 (4.83; 4.97 with 4, 4.98 with 5); 384^3 3 (8.29; 8.34
+    // with 4, 8.49 with 6); 448^3 4 (13.0; 13.2 with 7); 512^3 8 (19.8;
+    // 20.0-20.1 with 6 or 10, 20.3 with 12); 640^3 8 (10 slabs 3 % faster on
+    // one box, equal on another); 768^3 and 1024^3 8 (12 slabs were 6 %
+    // slower at 1024^3).  128^3 runs best whole.
     const int64_t cells = (int64_t)c->g.nx * c->g.ny * c->g.nz;
-    const int S = std::min(8, c->g.nz / 64);
+    const int nz = c->g.nz;
+    const int S = nz < 320 ? nz / 64 : nz < 512 ? nz / 100 : std::min(8, nz / 64);
     return (cells >= (int64_t(1) << 24) && S >= 2) ? S : 0;
 }
 

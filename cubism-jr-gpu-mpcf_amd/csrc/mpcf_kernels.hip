@@ -1510,7 +1510,9 @@ void launch_sweep(const Geom& g, const int dir, const SweepParams& p, const int 
     A.sos_next = p.sos_next; A.sos_cpu = p.sos_cpu;
     A.z0 = p.z0; A.z1 = p.z1 < 0 ? g.nz : p.z1;
     const int nzr = A.z1 - A.z0;
-    const int alone = (A.z0 == 0 && A.z1 == g.nz) ? 2 : 1;   // a whole-block launch (not a slab)
+    // a whole-block launch (not a slab) targets four times a slab's waves:
+    // at 256^3-512^3 its steps are 0.6-2.7 % faster than with twice
+    const int alone = (A.z0 == 0 && A.z1 == g.nz) ? 4 : 1;
     if (dir == 3) {
         const int chunk = pick_chunk(g.ny, (int64_t)g.nx * nzr, alone);
         const int waves = ((g.nx + 6) * nzr + XY_TILE - 1) / XY_TILE;
@@ -1600,7 +1602,7 @@ void launch_test_face(const int n, const int dir, const void* stv, void* outv, h
 
 int default_chunk(const int nd, const long long columns) { return pick_chunk(nd, columns, 1); }
 // the march length of a whole-block launch that runs alone
-int whole_chunk(const int nd, const long long columns) { return pick_chunk(nd, columns, 2); }
+int whole_chunk(const int nd, const long long columns) { return pick_chunk(nd, columns, 4); }
 
 // grid of the fused x+y sweep: dims[0] blocks of four waves over the padded
 // rows of all planes, dims[2] y-chunks (dims[1] = 1)
