@@ -181,11 +181,12 @@ __device__ __forceinline__ int bc_index(const int i, const int n, const int bcma
 
 // A lane's element index split into a part that is the same for every lane of
 // the wave (base, a scalar) and the lane's byte offset from it (lo, small and
-// non-negative).  The sweeps access their fields through buffer resources
-// built on wave-uniform element pointers (the field, row and wave parts of
-// the index, added by the scalar unit), with the lane's 32-bit offset: as flat
-// global accesses the same loads and stores took one 64-bit vector add each,
-// a few percent of the marches' vector instructions.
+// non-negative).  The fused x+y sweep accesses its fields through buffer
+// resources built on wave-uniform element pointers (the field, row and wave
+// parts of the index, added by the scalar unit), with the lane's 32-bit
+// offset: as flat global accesses the same loads and stores took one 64-bit
+// vector add each, 3 % of its vector instructions.  The y and z sweeps use the
+// same split with flat global accesses (see ld_lane).
 struct LaneIdx { int64_t base; uint32_t lo; };
 
 // e: the lane's element; floor_e: an element no greater than the element of
