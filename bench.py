@@ -180,13 +180,16 @@ def cpu_baseline(full: bool):
         b.data = f
         del f
         h = 1.0 / n
+        t0 = time.perf_counter()
         b.step(h, 0.3, 1e30, [0] * 6)          # warm-up step
+        log(f"CPU baseline {n}^3: warm-up step {time.perf_counter() - t0:.2f} s")
         reps = []
-        for _ in range(3):
+        for i in range(3):
             t0 = time.perf_counter()
             for _ in range(steps):
                 b.step(h, 0.3, 1e30, [0] * 6)
             reps.append(time.perf_counter() - t0)
+            log(f"CPU baseline {n}^3: repetition {i + 1} of 3, {steps} steps in {reps[-1]:.2f} s")
         med = statistics.median(reps)
         sizes[str(n)] = {"steps": steps, "repetitions_s": reps, "median_s_per_step": med / steps,
                          "cell_updates_per_s": n ** 3 * 3 * steps / med}
