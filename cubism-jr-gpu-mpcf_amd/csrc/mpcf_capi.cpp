@@ -131,6 +131,11 @@ int pipeline_slabs(const mpcf_ctx* c)
     // (measured on MI355X at 512^3, bit-exact build: whole-block 59.4 ms a
     // step, 4 slabs 60.1, 8 slabs 61.4)
     if (split_xy(c)) return 0;
+    // so is the bit-exact binary32 build's fused sweep, whose division and
+    // binary64 weight sequences make both sweeps VALU-bound (512^3: whole-
+    // block 57.05 ms a step, 4 slabs 57.6, 8 slabs 57.9; 384^3: whole 25.16,
+    // 3 slabs 25.17, 6 slabs 25.3)
+    if (!(c->flags & MPCF_FLAG_FAST)) return 0;
     // blocks of 2^24 cells or more, slabs by depth as measured on MI355X (fast
     // build, cubes, ms a step): 256^3 4 slabs (2.66; 2.74 with 2, 2.81
     // whole); 320^3 3 (4.83; 4.97 with 4, 4.98 with 5); 384^3 3 (8.29; 8.34

@@ -101,10 +101,11 @@ int mpcf_destroy(mpcf_ctx* ctx);
  * a second stream of the context while the x and y sweeps (fused or split) of
  * the next slab run on the context's stream; the context's stream is ordered
  * after the step's last sweep.  0: whole-block sweeps.  -1 (default; 0 under
- * MPCF_FLAG_NO_PIPELINE): with the fused x+y sweep, for blocks of 2^24 cells
- * or more, NZ/64 slabs for NZ < 320, NZ/100 for NZ < 512 and min(8, NZ/64)
+ * MPCF_FLAG_NO_PIPELINE): in the FAST builds, for blocks of 2^24 cells or
+ * more, NZ/64 slabs for NZ < 320, NZ/100 for NZ < 512 and min(8, NZ/64)
  * above (the fastest counts measured on cubes), whole-block when that is
- * below 2 or the block is smaller; with the split sweeps, whole-block.  slabs <= min(64, NZ).  No counterpart in the
+ * below 2 or the block is smaller; in the bit-exact and binary64 builds and
+ * with the split sweeps, whole-block (their sweeps are all VALU-bound).  slabs <= min(64, NZ).  No counterpart in the
  * reference, whose stages run chunk by chunk on the host (GPUlab.cpp:703-793). */
 int mpcf_set_pipeline(mpcf_ctx* ctx, int slabs);
 

@@ -367,17 +367,16 @@ def sweep_launches(mpcf, b, steps=2):
 
 
 def test_pipeline_slab_choice(mpcf):
-    """The automatic choice for blocks of 2^24 cells or more with the fused
-    x+y sweep (the default of the FAST and the bit-exact binary32 builds):
-    NZ/64 slabs for NZ < 320, NZ/100 for NZ < 512, min(8, NZ/64) above;
-    whole-block sweeps below 2 slabs or 2^24 cells, for the split sweeps and
-    under MPCF_FLAG_NO_PIPELINE; forced counts as asked, split sweeps
-    included."""
+    """The automatic choice for blocks of 2^24 cells or more in the FAST
+    builds (fused x+y sweep): NZ/64 slabs for NZ < 320, NZ/100 for NZ < 512,
+    min(8, NZ/64) above; whole-block sweeps below 2 slabs or 2^24 cells, in
+    the bit-exact build (fused or split), for the split sweeps and under
+    MPCF_FLAG_NO_PIPELINE; forced counts as asked, split sweeps included."""
     cases = [((128, 128, 128), dict(fast=True), 1), ((256, 256, 256), dict(fast=True), 4),
              ((256, 256, 384), dict(fast=True), 3), ((160, 160, 1024), dict(fast=True), 8),
              ((128, 128, 1280), dict(fast=True), 8), ((192, 192, 640), dict(fast=True), 8),
              ((512, 512, 64), dict(fast=True), 1), ((256, 256, 256), dict(split_xy=True), 1),
-             ((256, 256, 256), dict(), 4),
+             ((256, 256, 256), dict(), 1), ((256, 256, 256), dict(pipeline=4), 4),
              ((64, 64, 96), dict(pipeline=3), 3),
              ((256, 256, 256), dict(fast=True, pipeline=False), 1), ((64, 64, 96), dict(fast=True, pipeline=6), 6)]
     for (nx, ny, nz), kw, want in cases:
