@@ -861,6 +861,10 @@ template <bool FIRST>
 __global__ __launch_bounds__(256) MPCF_XY_WAVES
 void k_sweep_xy(const Geom g, const StageArgs A, const int chunk)
 {
+    // where a pipelined step's z sweep shares a SIMD with these waves, they
+    // issue first (wave priority, not a queue priority): 512^3 19.92 ->
+    // 19.85 ms a step; the z sweep's waves first instead was 7 % slower
+    __builtin_amdgcn_s_setprio(2);
     const int gx = gridDim.x;
     const int t = xcd_order(blockIdx.x + blockIdx.y * gx, gx * gridDim.y);
     sweep_xy_body<FIRST>(g, A, chunk, (t % gx) * 4 + threadIdx.y, t / gx);
